@@ -1,0 +1,667 @@
+/* Test infrastructure: the oracle (CPU restatement of pyglass query-time
+ * graph search).  Only tests/, __graft_entry__.smoke() and bench.py's CPU
+ * baseline leg load it, and only as the checker.
+ *
+ * Built with -fno-fast-math -ffp-contract=off: every rounding step of the
+ * float paths is written out explicitly (fmaf where the compiled reference
+ * fuses, a fixed reduction tree where it reduces).
+ *
+ * How each part is pinned (tests/test_oracle_*.py):
+ *   calibration + encoders       codes equal the compiled reference's codes
+ *                                (tests/golden/*.npz) and the textbook
+ *                                quantile/rounding formula (numpy)
+ *   integer distances            closed forms sum (x-y)^2 / -sum x*y in int64
+ *   float distances              bit-equal to the compiled reference on golden
+ *                                vectors; summation order shown to matter by a
+ *                                planted-reorder test
+ *   LinearPool / expansion       hypothesis model test against a sorted list,
+ *                                brute force (ef >= N on a connected graph gives
+ *                                the exact quantized kNN), and golden ids/dists
+ *                                of the compiled reference Searcher
+ *   HNSW initializer             hand-built 3-level toy graph + golden vectors
+ *   refiner                      total order (dist, id) + golden vectors
+ */
+#include "glass_oracle.h"
+
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* ------------------------------------------------------------------ */
+/* fp16 / e5m2 (helpa/helpa/types.hpp:83-151 with F16C, glass/types.hpp:19-46) */
+
+uint16_t orq_f32_to_f16(float f) {
+    uint32_t x;
+    memcpy(&x, &f, 4);
+    uint32_t sign = (x >> 16) & 0x8000u;
+    uint32_t ax = x & 0x7fffffffu;
+    if (ax >= 0x7f800000u) {
+        if (ax > 0x7f800000u) return (uint16_t)(sign | 0x7e00u | ((ax >> 13) & 0x3ffu));
+        return (uint16_t)(sign | 0x7c00u);
+    }
+    if (ax >= 0x477ff000u) return (uint16_t)(sign | 0x7c00u); /* >= 65520 rounds to inf */
+    if (ax < 0x38800000u) {                                     /* below 2^-14: fp16 subnormal */
+        uint32_t e = ax >> 23;
+        if (e < 102) return (uint16_t)sign;
+        uint32_t M = (ax & 0x7fffffu) | 0x800000u;
+        int shift = 126 - (int)e;
+        uint32_t q = M >> shift, rem = M & ((1u << shift) - 1u), half = 1u << (shift - 1);
+        if (rem > half || (rem == half && (q & 1u))) q++;
+        return (uint16_t)(sign | q);
+    }
+    uint32_t r = ax + 0x0fffu + ((ax >> 13) & 1u);
+    r -= 0x38000000u;
+    return (uint16_t)(sign | (r >> 13));
+}
+
+float orq_f16_to_f32(uint16_t h) {
+    uint32_t sign = (uint32_t)(h & 0x8000u) << 16;
+    uint32_t e = (h >> 10) & 0x1fu, m = h & 0x3ffu, bits;
+    if (e == 0) {
+        if (m == 0) {
+            bits = sign;
+        } else { /* subnormal: normalise */
+            int s = 0;
+            while (!(m & 0x400u)) {
+                m <<= 1;
+                s++;
+            }
+            m &= 0x3ffu;
+            bits = sign | ((uint32_t)(113 - s) << 23) | (m << 13);
+        }
+    } else if (e == 31) {
+        bits = sign | 0x7f800000u | (m << 13);
+    } else {
+        bits = sign | ((e + 112u) << 23) | (m << 13);
+    }
+    float f;
+    memcpy(&f, &bits, 4);
+    return f;
+}
+
+/* ------------------------------------------------------------------ */
+/* Calibration (glass/quant/calibrator.hpp:14-91, glass/quant/utils.hpp:31-70) */
+
+static int cmp_float(const void *a, const void *b) {
+    float x = *(const float *)a, y = *(const float *)b;
+    return (x > y) - (x < y);
+}
+
+/* The reference keeps the `top` smallest values in a MaxHeap and returns its
+ * root, i.e. the top-th smallest value (order statistic).  Restated with a
+ * sort of a copy. */
+static void order_stats(const float *data, int64_t nitems, int64_t top, float *kth_small, float *kth_large) {
+    float *tmp = (float *)malloc(sizeof(float) * (size_t)nitems);
+    memcpy(tmp, data, sizeof(float) * (size_t)nitems);
+    qsort(tmp, (size_t)nitems, sizeof(float), cmp_float);
+    if (top > nitems) top = nitems;
+    *kth_small = tmp[top - 1];
+    *kth_large = tmp[nitems - top];
+    free(tmp);
+}
+
+/* find_minmax (glass/quant/utils.hpp:31-40): top = int64(float(nitems)*ratio)+1 */
+static void find_minmax(const float *data, int64_t nitems, float ratio, float *mn, float *mx) {
+    int64_t top = (int64_t)((float)nitems * ratio) + 1;
+    order_stats(data, nitems, top, mn, mx);
+}
+
+/* find_absmax (glass/quant/utils.hpp:42-50) */
+static float find_absmax(const float *data, int64_t nitems, float ratio) {
+    int64_t top = (int64_t)((float)nitems * ratio) + 1;
+    float *tmp = (float *)malloc(sizeof(float) * (size_t)nitems);
+    for (int64_t i = 0; i < nitems; ++i) tmp[i] = fabsf(data[i]);
+    float a, b;
+    order_stats(tmp, nitems, top, &a, &b);
+    free(tmp);
+    return b;
+}
+
+int orq_quant_init(orq_quant *q, int32_t kind, int32_t metric, int32_t dim) {
+    memset(q, 0, sizeof(*q));
+    q->kind = kind;
+    q->metric = metric;
+    q->dim = dim;
+    /* Quantizer<metric, AlignWidth, NBits> per quantizer
+     * (sq8u_quant.hpp:14, sq4u_quant.hpp:13, sq2u_quant.hpp:11,
+     *  e5m2_quant.hpp:11, sq8_quant.hpp:15-16, fp32_quant.hpp:8) */
+    switch (kind) {
+        case ORQ_SQ8U: q->align = 64; q->nbits = 8; break;
+        case ORQ_SQ4U: q->align = 128; q->nbits = 4; break;
+        case ORQ_SQ2U: q->align = 256; q->nbits = 2; break;
+        case ORQ_E5M2: q->align = 16; q->nbits = 8; break;
+        case ORQ_SQ8: q->align = 16; q->nbits = 8; break;
+        case ORQ_FP32: q->align = 16; q->nbits = 32; break;
+        default: return -1;
+    }
+    /* Tensor ctor (glass/storage/tensor.hpp:30-35) */
+    q->dim_align = (dim + q->align - 1) / q->align * q->align;
+    q->code_size = q->dim_align * q->nbits / 8;
+    return 0;
+}
+
+void orq_quant_free(orq_quant *q) {
+    free(q->mins);
+    free(q->difs);
+    q->mins = q->difs = NULL;
+}
+
+int orq_quant_train(orq_quant *q, const float *data, int64_t n) {
+    int64_t nd = n * (int64_t)q->dim;
+    float mn, mx;
+    switch (q->kind) {
+        case ORQ_SQ8U:
+            if (q->metric == ORQ_L2) { /* AffineCalibrator<127>, drop 0 (sq8u_quant.hpp:19-28) */
+                find_minmax(data, nd, 0.0f, &mn, &mx);
+                q->min = mn;
+                q->dif = mx - mn;
+            } else { /* SymCalibrator<127> */
+                q->max = find_absmax(data, nd, 0.0f);
+            }
+            break;
+        case ORQ_SQ4U: { /* Affine<15>, first 10% of items, drop 1% (sq4u_quant.hpp:85-97) */
+            int64_t nitems = (int64_t)((float)nd * 0.1f);
+            find_minmax(data, nitems, 0.01f, &mn, &mx);
+            q->min = mn;
+            q->dif = mx - mn;
+            break;
+        }
+        case ORQ_SQ2U: { /* Affine<3>, first 10% of items, drop 5% (sq2u_quant.hpp:164-176) */
+            int64_t nitems = (int64_t)((float)nd * 0.1f);
+            find_minmax(data, nitems, 0.05f, &mn, &mx);
+            q->min = mn;
+            q->dif = mx - mn;
+            break;
+        }
+        case ORQ_SQ8: {
+            /* AffinePerDimCalibrator<255> via find_minmax_perdim
+             * (glass/quant/utils.hpp:52-70): "dimension" j is fed the
+             * flattened items [j*n, (j+1)*n), not column j. */
+            q->mins = (float *)calloc((size_t)q->dim_align, sizeof(float));
+            q->difs = (float *)calloc((size_t)q->dim_align, sizeof(float));
+            for (int32_t j = 0; j < q->dim; ++j) {
+                const float *p = data + (int64_t)j * n;
+                float lo = p[0], hi = p[0];
+                for (int64_t i = 1; i < n; ++i) {
+                    if (p[i] < lo) lo = p[i];
+                    if (p[i] > hi) hi = p[i];
+                }
+                q->mins[j] = lo;
+                q->difs[j] = hi - lo;
+            }
+            break;
+        }
+        default:
+            break;
+    }
+    return 0;
+}
+
+/* limit_range / limit_range_sym (glass/quant/utils.hpp:8-26) + std::round */
+static int32_t affine_code(float x, float mn, float dif, int32_t mul) {
+    x = (x - mn) / dif;
+    if (x < 0.0f) x = 0.0f;
+    if (x > 1.0f) x = 1.0f;
+    return (int32_t)roundf(x * (float)mul);
+}
+
+static int32_t sym_code(float x, float mx, int32_t mul) {
+    x = x / mx;
+    if (x < -1.0f) x = -1.0f;
+    if (x > 1.0f) x = 1.0f;
+    return (int32_t)roundf(x * (float)mul);
+}
+
+void orq_encode(const orq_quant *q, const float *x, uint8_t *out) {
+    memset(out, 0, (size_t)q->code_size);
+    int32_t d = q->dim;
+    switch (q->kind) {
+        case ORQ_SQ8U: /* sq8u_quant.hpp:38-42 */
+            for (int32_t j = 0; j < d; ++j) {
+                int32_t c = q->metric == ORQ_L2 ? affine_code(x[j], q->min, q->dif, 127) : sym_code(x[j], q->max, 127);
+                out[j] = (uint8_t)(int8_t)c;
+            }
+            break;
+        case ORQ_SQ4U: /* sq4u_quant.hpp:107-117: dim j -> nibble (j&1 ? hi : lo) of byte j/2 */
+            for (int32_t j = 0; j < d; ++j) {
+                uint8_t y = (uint8_t)affine_code(x[j], q->min, q->dif, 15);
+                out[j / 2] |= (uint8_t)((j & 1) ? (y << 4) : y);
+            }
+            break;
+        case ORQ_SQ2U: /* sq2u_quant.hpp:186-200: dim j -> bits 2*(j%4) of byte j/4 */
+            for (int32_t j = 0; j < d; ++j) {
+                uint8_t y = (uint8_t)affine_code(x[j], q->min, q->dif, 3);
+                out[j / 4] |= (uint8_t)(y << (2 * (j % 4)));
+            }
+            break;
+        case ORQ_E5M2: /* e5m2_quant.hpp:30-34: high byte of fp16(x) */
+            for (int32_t j = 0; j < d; ++j) out[j] = (uint8_t)(orq_f32_to_f16(x[j]) >> 8);
+            break;
+        case ORQ_SQ8: /* sq8_quant.hpp:43-47 */
+            for (int32_t j = 0; j < d; ++j) out[j] = (uint8_t)affine_code(x[j], q->mins[j], q->difs[j], 255);
+            break;
+        case ORQ_FP32:
+            memcpy(out, x, sizeof(float) * (size_t)d);
+            break;
+    }
+}
+
+void orq_encode_rows(const orq_quant *q, const float *data, int64_t n, uint8_t *codes) {
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < n; ++i) orq_encode(q, data + i * q->dim, codes + i * q->code_size);
+}
+
+/* ComputerImpl ctor (glass/quant/computer.hpp:54-63): integer quantizers
+ * encode the query with the base calibrator; E5M2/SQ8/FP32 copy it (MemCpyTag)
+ * into a zeroed dim_align float buffer. */
+int64_t orq_query_size(const orq_quant *q) {
+    if (q->kind == ORQ_SQ8U || q->kind == ORQ_SQ4U || q->kind == ORQ_SQ2U) return q->code_size;
+    return (int64_t)q->dim_align * 4;
+}
+
+void orq_prepare_query(const orq_quant *q, const float *x, uint8_t *qbuf) {
+    if (q->kind == ORQ_SQ8U || q->kind == ORQ_SQ4U || q->kind == ORQ_SQ2U) {
+        orq_encode(q, x, qbuf);
+    } else {
+        memset(qbuf, 0, (size_t)q->dim_align * 4);
+        memcpy(qbuf, x, sizeof(float) * (size_t)q->dim);
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* Distances */
+
+/* glass::reduce_add_f32x16 (glass/simd/avx512_utils.hpp:12-22):
+ * 16 -> 8 -> 4 by halves, then hadd: (c0+c1)+(c2+c3). */
+static float reduce16_glass(const float a[16]) {
+    float b[8], c[4];
+    for (int i = 0; i < 8; ++i) b[i] = a[i] + a[i + 8];
+    for (int i = 0; i < 4; ++i) c[i] = b[i] + b[i + 4];
+    return (c[0] + c[1]) + (c[2] + c[3]);
+}
+
+/* helpa reduce_add_f32x16 = GCC _mm512_reduce_add_ps
+ * (helpa/platform/x86/utils.hpp:61-64; avx512fintrin.h __MM512_REDUCE_OP):
+ * 16 -> 8 -> 4 by halves, then (c0+c2)+(c1+c3). */
+static float reduce16_helpa(const float a[16]) {
+    float b[8], c[4];
+    for (int i = 0; i < 8; ++i) b[i] = a[i + 8] + a[i];
+    for (int i = 0; i < 4; ++i) c[i] = b[i + 4] + b[i];
+    return (c[0] + c[2]) + (c[1] + c[3]);
+}
+
+/* ctx selects the FMA contraction GCC -Ofast chose where the reference's
+ * SQ8 L2 kernel is inlined: ORQ_CTX_COMPUTER (a bare computer call) and
+ * ORQ_CTX_SEARCH (Search/SearchImpl1) fuse as fma(-c, dif, 256x - 256min);
+ * ORQ_CTX_BATCH (SearchBatch/SearchImpl2) fuses yy = fma(c, dif, 256min),
+ * t = 256x - yy.  Every other kernel compiles the same in all contexts. */
+double orq_distance_ctx(const orq_quant *q, const uint8_t *qbuf, const uint8_t *code, int32_t ctx) {
+    int32_t da = q->dim_align;
+    switch (q->kind) {
+        case ORQ_SQ8U: {
+            const int8_t *x = (const int8_t *)qbuf, *y = (const int8_t *)code;
+            int64_t s = 0;
+            if (q->metric == ORQ_L2) { /* helpa::l2_s7_s7 (x86/l2_impl.hpp:44-48,222-255) */
+                for (int32_t j = 0; j < da; ++j) {
+                    int32_t t = (int32_t)x[j] - (int32_t)y[j];
+                    s += t * t;
+                }
+                return (double)(int32_t)s;
+            }
+            /* helpa::dot_s8_s8 (x86/dot_impl.hpp:46-50,210-229) */
+            for (int32_t j = 0; j < da; ++j) s += (int32_t)x[j] * (int32_t)y[j];
+            return (double)(int32_t)(-s);
+        }
+        case ORQ_SQ4U: { /* helpa::l2a_u4_u4 (x86/l2_impl.hpp:257-298) */
+            int64_t s = 0;
+            for (int32_t j = 0; j < da / 2; ++j) {
+                int32_t a = qbuf[j] & 15, b = code[j] & 15, c = qbuf[j] >> 4, e = code[j] >> 4;
+                s += (a - b) * (a - b) + (c - e) * (c - e);
+            }
+            return (double)(int32_t)s;
+        }
+        case ORQ_SQ2U: { /* helpa::l2a_u2_u2 (x86/l2_impl.hpp:300-367) */
+            int64_t s = 0;
+            for (int32_t j = 0; j < da / 4; ++j) {
+                for (int sh = 0; sh < 8; sh += 2) {
+                    int32_t a = (qbuf[j] >> sh) & 3, b = (code[j] >> sh) & 3;
+                    s += (a - b) * (a - b);
+                }
+            }
+            return (double)(int32_t)s;
+        }
+        case ORQ_E5M2: { /* glass::L2SqrE5M2 / IPE5M2 (glass/simd/avx512.hpp:15-38) */
+            const float *x = (const float *)qbuf;
+            float acc[16] = {0};
+            for (int32_t i = 0; i < da; i += 16) {
+                for (int l = 0; l < 16; ++l) {
+                    float y = orq_f16_to_f32((uint16_t)(code[i + l] << 8));
+                    if (q->metric == ORQ_L2) {
+                        float t = x[i + l] - y;
+                        acc[l] = fmaf(t, t, acc[l]);
+                    } else {
+                        acc[l] = fmaf(x[i + l], y, acc[l]);
+                    }
+                }
+            }
+            float r = reduce16_glass(acc);
+            return q->metric == ORQ_L2 ? (double)r : (double)(-r);
+        }
+        case ORQ_SQ8: { /* glass::L2SqrSQ8_ext / IPSQ8_ext (glass/simd/avx512.hpp:40-77) */
+            const float *x = (const float *)qbuf;
+            float acc[16] = {0};
+            for (int32_t i = 0; i < da; i += 16) {
+                for (int l = 0; l < 16; ++l) {
+                    float c = (float)code[i + l] + 0.5f;
+                    if (q->metric == ORQ_L2) {
+                        float t;
+                        if (ctx == ORQ_CTX_BATCH) {
+                            t = x[i + l] * 256.0f - fmaf(c, q->difs[i + l], q->mins[i + l] * 256.0f);
+                        } else {
+                            t = fmaf(-c, q->difs[i + l], x[i + l] * 256.0f - q->mins[i + l] * 256.0f);
+                        }
+                        acc[l] = fmaf(t, t, acc[l]);
+                    } else {
+                        float yy = fmaf(c, q->difs[i + l], q->mins[i + l] * 256.0f);
+                        acc[l] = fmaf(x[i + l], yy, acc[l]);
+                    }
+                }
+            }
+            float r = reduce16_glass(acc);
+            return q->metric == ORQ_L2 ? (double)r : (double)(-r);
+        }
+        case ORQ_FP32: { /* helpa::l2_fp32_fp32 / dot_fp32_fp32 (x86/l2_impl.hpp:14-18,50-71; x86/dot_impl.hpp:16-20,52-71) */
+            const float *x = (const float *)qbuf, *y = (const float *)code;
+            float acc[16] = {0};
+            for (int32_t i = 0; i < da; i += 16) {
+                for (int l = 0; l < 16; ++l) {
+                    if (q->metric == ORQ_L2) {
+                        float t = x[i + l] - y[i + l];
+                        acc[l] = fmaf(t, t, acc[l]);
+                    } else {
+                        acc[l] = fmaf(x[i + l], y[i + l], acc[l]);
+                    }
+                }
+            }
+            float r = reduce16_helpa(acc);
+            return q->metric == ORQ_L2 ? (double)r : (double)(-r);
+        }
+    }
+    return 0.0;
+}
+
+double orq_distance(const orq_quant *q, const uint8_t *qbuf, const uint8_t *code) {
+    return orq_distance_ctx(q, qbuf, code, ORQ_CTX_COMPUTER);
+}
+
+/* ------------------------------------------------------------------ */
+/* Graph I/O (glass/graph.hpp:106-153, glass/hnsw/HNSWInitializer.hpp:64-86) */
+
+void orq_graph_free(orq_graph *g) {
+    free(g->adj);
+    free(g->eps);
+    free(g->levels);
+    free(g->offsets);
+    free(g->lists);
+    memset(g, 0, sizeof(*g));
+}
+
+int orq_graph_from_arrays(orq_graph *g, int32_t N, int32_t K, const int32_t *adj, int32_t nep, const int32_t *eps) {
+    memset(g, 0, sizeof(*g));
+    g->N = N;
+    g->K = K;
+    g->adj = (int32_t *)malloc(sizeof(int32_t) * (size_t)N * (size_t)K);
+    memcpy(g->adj, adj, sizeof(int32_t) * (size_t)N * (size_t)K);
+    g->nep = nep;
+    g->eps = (int32_t *)malloc(sizeof(int32_t) * (size_t)(nep > 0 ? nep : 1));
+    if (nep > 0) memcpy(g->eps, eps, sizeof(int32_t) * (size_t)nep);
+    return 0;
+}
+
+int orq_graph_load(orq_graph *g, const char *path) {
+    memset(g, 0, sizeof(*g));
+    FILE *f = fopen(path, "rb");
+    if (!f) return -1;
+    int32_t nep;
+    if (fread(&nep, 4, 1, f) != 1 || nep < 0) goto fail;
+    g->nep = nep;
+    g->eps = (int32_t *)malloc(sizeof(int32_t) * (size_t)(nep > 0 ? nep : 1));
+    if (nep && fread(g->eps, 4, (size_t)nep, f) != (size_t)nep) goto fail;
+    if (fread(&g->N, 4, 1, f) != 1 || fread(&g->K, 4, 1, f) != 1) goto fail;
+    g->adj = (int32_t *)malloc(sizeof(int32_t) * (size_t)g->N * (size_t)g->K);
+    if (fread(g->adj, 4, (size_t)g->N * (size_t)g->K, f) != (size_t)g->N * (size_t)g->K) goto fail;
+    {
+        int32_t hdr[3];
+        if (fread(hdr, 4, 3, f) == 3) {
+            g->has_init = 1;
+            g->init_K = hdr[1];
+            g->ep = hdr[2];
+            int32_t n = hdr[0];
+            g->levels = (int32_t *)calloc((size_t)n, sizeof(int32_t));
+            g->offsets = (int64_t *)calloc((size_t)n + 1, sizeof(int64_t));
+            size_t capn = 1024, used = 0;
+            g->lists = (int32_t *)malloc(sizeof(int32_t) * capn);
+            for (int32_t i = 0; i < n; ++i) {
+                int32_t cur;
+                if (fread(&cur, 4, 1, f) != 1) goto fail;
+                g->levels[i] = cur / g->init_K;
+                g->offsets[i] = (int64_t)used;
+                if (used + (size_t)cur > capn) {
+                    while (used + (size_t)cur > capn) capn *= 2;
+                    g->lists = (int32_t *)realloc(g->lists, sizeof(int32_t) * capn);
+                }
+                if (cur && fread(g->lists + used, 4, (size_t)cur, f) != (size_t)cur) goto fail;
+                used += (size_t)cur;
+            }
+            g->offsets[n] = (int64_t)used;
+        }
+    }
+    fclose(f);
+    return 0;
+fail:
+    fclose(f);
+    orq_graph_free(g);
+    return -2;
+}
+
+/* ------------------------------------------------------------------ */
+/* LinearPool (glass/neighbor.hpp:320-433) */
+
+typedef struct {
+    int32_t size, cur, ef, cap;
+    double *dist;
+    int32_t *id; /* bit 31 = checked */
+    uint8_t *vis;
+} pool_t;
+
+static void pool_insert(pool_t *p, int32_t u, double d) {
+    /* insert (neighbor.hpp:382-396): reject when full and d >= last */
+    if (p->size == p->cap && d >= p->dist[p->size - 1]) return;
+    /* find_bsearch (neighbor.hpp:361-371): first index with dist > d */
+    int32_t lo = 0, hi = p->size;
+    while (lo < hi) {
+        int32_t mid = (lo + hi) / 2;
+        if (p->dist[mid] > d) hi = mid;
+        else lo = mid + 1;
+    }
+    /* shift right; the slot past cap is scratch (data_ has cap+1 entries) */
+    int32_t n = p->size - lo;
+    memmove(&p->dist[lo + 1], &p->dist[lo], sizeof(double) * (size_t)n);
+    memmove(&p->id[lo + 1], &p->id[lo], sizeof(int32_t) * (size_t)n);
+    p->dist[lo] = d;
+    p->id[lo] = u;
+    if (p->size < p->cap) p->size++;
+    if (lo < p->cur) p->cur = lo;
+}
+
+static int32_t pool_pop(pool_t *p) {
+    /* pop (neighbor.hpp:398-405) */
+    p->id[p->cur] |= (int32_t)0x80000000u;
+    int32_t pre = p->cur;
+    while (p->cur < p->size && (p->id[p->cur] & (int32_t)0x80000000u)) p->cur++;
+    return p->id[pre] & 0x7fffffff;
+}
+
+static int pool_has_next(const pool_t *p) { return p->cur < p->size && p->cur < p->ef; }
+
+/* ------------------------------------------------------------------ */
+/* Search */
+
+static double eval(const orq_quant *q, const uint8_t *qbuf, const uint8_t *codes, int32_t u, orq_counters *cnt,
+                   int32_t ctx) {
+    cnt->dist_cmps++;
+    return orq_distance_ctx(q, qbuf, codes + (int64_t)u * q->code_size, ctx);
+}
+
+void orq_search(const orq_graph *g, const orq_quant *q, const uint8_t *codes, const float *x, int32_t k, int32_t ef,
+                int32_t batch_mode, int32_t *ids, float *dists, orq_counters *cnt) {
+    orq_counters local = {0, 0, 0};
+    if (!cnt) cnt = &local;
+    int32_t ctx = batch_mode ? ORQ_CTX_BATCH : ORQ_CTX_SEARCH;
+    uint8_t *qbuf = (uint8_t *)calloc(1, (size_t)orq_query_size(q));
+    orq_prepare_query(q, x, qbuf);
+    /* Search: reset(nb, max(k,ef), max(k,ef)) (graph_searcher.hpp:134);
+     * SearchBatch: reset(nb, ef, max(k,ef)) (graph_searcher.hpp:152) */
+    int32_t cap = k > ef ? k : ef;
+    pool_t p;
+    p.size = p.cur = 0;
+    p.cap = cap;
+    p.ef = batch_mode ? ef : cap;
+    p.dist = (double *)calloc((size_t)cap + 1, sizeof(double));
+    p.id = (int32_t *)calloc((size_t)cap + 1, sizeof(int32_t));
+    p.vis = (uint8_t *)calloc((size_t)g->N, 1);
+
+    /* initialize_search (graph.hpp:95-104) */
+    if (g->has_init) {
+        /* HNSWInitializer::initialize (HNSWInitializer.hpp:41-62): the list
+         * scanned in one pass is the one of the node current at pass start. */
+        int32_t u = g->ep;
+        double du = eval(q, qbuf, codes, u, cnt, ctx);
+        for (int32_t level = g->levels[u]; level > 0; --level) {
+            int changed = 1;
+            while (changed) {
+                changed = 0;
+                const int32_t *list = g->lists + g->offsets[u] + (int64_t)(level - 1) * g->init_K;
+                for (int32_t i = 0; i < g->init_K && list[i] != -1; ++i) {
+                    int32_t v = list[i];
+                    double dv = eval(q, qbuf, codes, v, cnt, ctx);
+                    cnt->upper_evals++;
+                    if (dv < du) {
+                        du = dv;
+                        u = v;
+                        changed = 1;
+                    }
+                }
+            }
+        }
+        pool_insert(&p, u, du);
+        p.vis[u] = 1;
+    } else {
+        for (int32_t i = 0; i < g->nep; ++i) {
+            pool_insert(&p, g->eps[i], eval(q, qbuf, codes, g->eps[i], cnt, ctx));
+            p.vis[g->eps[i]] = 1;
+        }
+    }
+
+    /* SearchImpl2 (graph_searcher.hpp:196-224); SearchImpl1 inserts the same
+     * sequence (161-194) */
+    while (pool_has_next(&p)) {
+        int32_t u = pool_pop(&p);
+        cnt->expansions++;
+        const int32_t *row = g->adj + (int64_t)u * g->K;
+        for (int32_t i = 0; i < g->K; ++i) {
+            int32_t v = row[i];
+            if (v == -1) break;
+            if (p.vis[v]) continue;
+            p.vis[v] = 1;
+            pool_insert(&p, v, eval(q, qbuf, codes, v, cnt, ctx));
+        }
+    }
+
+    /* to_sorted (neighbor.hpp:421-428); entries past size are -1/+inf here */
+    for (int32_t i = 0; i < k; ++i) {
+        if (i < p.size) {
+            ids[i] = p.id[i] & 0x7fffffff;
+            if (dists) dists[i] = (float)p.dist[i];
+        } else {
+            ids[i] = -1;
+            if (dists) dists[i] = INFINITY;
+        }
+    }
+    free(p.dist);
+    free(p.id);
+    free(p.vis);
+    free(qbuf);
+}
+
+float orq_refine_factor(int32_t kind) {
+    if (kind == ORQ_SQ8U || kind == ORQ_SQ8 || kind == ORQ_SQ4U) return 1.5f;
+    if (kind == ORQ_SQ2U) return 3.0f;
+    return 1.0f;
+}
+
+void orq_search_refine(const orq_graph *g, const orq_quant *q, const uint8_t *codes, const orq_quant *rq,
+                       const uint8_t *rcodes, float factor, const float *x, int32_t k, int32_t ef, int32_t batch_mode,
+                       int32_t *ids, float *dists, orq_counters *cnt) {
+    /* reorder_k = (int32_t)(k * reorder_mul) (refiner.hpp:46,73) */
+    int32_t rk = (int32_t)((float)k * factor);
+    if (!rq || rk == k) {
+        orq_search(g, q, codes, x, k, ef, batch_mode, ids, dists, cnt);
+        return;
+    }
+    int32_t *cand = (int32_t *)malloc(sizeof(int32_t) * (size_t)rk);
+    orq_search(g, q, codes, x, rk, ef, batch_mode, cand, NULL, cnt);
+    uint8_t *qbuf = (uint8_t *)calloc(1, (size_t)orq_query_size(rq));
+    orq_prepare_query(rq, x, qbuf);
+    /* RefineImpl (refiner.hpp:92-110): MaxHeap(k) by (dist, id)
+     * (neighbor.hpp:197-258) == the k smallest (dist, id) pairs, ascending. */
+    float *cd = (float *)malloc(sizeof(float) * (size_t)rk);
+    int32_t m = 0;
+    for (int32_t j = 0; j < rk; ++j) {
+        if (cand[j] < 0) continue;
+        cand[m] = cand[j];
+        cd[m] = (float)orq_distance(rq, qbuf, rcodes + (int64_t)cand[j] * rq->code_size);
+        m++;
+    }
+    for (int32_t i = 1; i < m; ++i) { /* insertion sort by (dist, id) */
+        float dv = cd[i];
+        int32_t iv = cand[i];
+        int32_t j = i - 1;
+        while (j >= 0 && (cd[j] > dv || (cd[j] == dv && cand[j] > iv))) {
+            cd[j + 1] = cd[j];
+            cand[j + 1] = cand[j];
+            j--;
+        }
+        cd[j + 1] = dv;
+        cand[j + 1] = iv;
+    }
+    for (int32_t i = 0; i < k; ++i) {
+        ids[i] = i < m ? cand[i] : -1;
+        if (dists) dists[i] = i < m ? cd[i] : INFINITY;
+    }
+    free(cand);
+    free(cd);
+    free(qbuf);
+}
+
+void orq_search_many(const orq_graph *g, const orq_quant *q, const uint8_t *codes, const orq_quant *rq,
+                     const uint8_t *rcodes, float factor, const float *xs, int64_t nq, int32_t k, int32_t ef,
+                     int32_t batch_mode, int32_t *ids, float *dists, int64_t *dist_cmps, int64_t *expansions,
+                     int32_t nthreads) {
+#ifdef _OPENMP
+    if (nthreads <= 0) nthreads = omp_get_max_threads();
+#pragma omp parallel for schedule(dynamic) num_threads(nthreads)
+#endif
+    for (int64_t i = 0; i < nq; ++i) {
+        orq_counters c = {0, 0, 0};
+        orq_search_refine(g, q, codes, rq, rcodes, factor, xs + i * q->dim, k, ef, batch_mode, ids + i * k,
+                          dists ? dists + i * k : NULL, &c);
+        if (dist_cmps) dist_cmps[i] = c.dist_cmps;
+        if (expansions) expansions[i] = c.expansions;
+    }
+}
