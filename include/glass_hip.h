@@ -1,0 +1,122 @@
+/* glass_hip.h — C ABI of the MI355X query-time graph search library
+ * (libglass_hip.so).  Plain pointers and sizes only; every function returns
+ * 0 on success or a negative status, with a message in glass_hip_last_error().
+ *
+ * The entry points are the ones the reference's Python binding and C++
+ * searcher interface bind for this path:
+ *   glass.Graph               python/bindings.cc:40-58, glass/graph.hpp:106-153
+ *   glass.Searcher            python/bindings.cc:90-154,258-269
+ *   GraphSearcherBase         glass/searcher/searcher_base.hpp:7-19
+ *   create_searcher           glass/searcher/graph_searcher.hpp:256-391
+ * Host pointers are expected unless a name ends in _dev (device pointers).
+ */
+#ifndef GLASS_HIP_H
+#define GLASS_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct glass_hip_graph glass_hip_graph;
+typedef struct glass_hip_searcher glass_hip_searcher;
+
+enum {
+    GLASS_HIP_OK = 0,
+    GLASS_HIP_EINVAL = -1,   /* bad argument (unknown metric/quantizer, shape, id range) */
+    GLASS_HIP_EIO = -2,      /* graph file unreadable or malformed */
+    GLASS_HIP_ENOMEM = -3,   /* device allocation failed */
+    GLASS_HIP_EDEVICE = -4,  /* HIP runtime / kernel launch error, or no device */
+    GLASS_HIP_ESTATE = -5    /* call out of order (e.g. search before set_data) */
+};
+
+/* Message of the last failing call on this thread. */
+const char *glass_hip_last_error(void);
+/* Library version string. */
+const char *glass_hip_version(void);
+/* Number of visible HIP devices (0 when none). */
+int glass_hip_device_count(void);
+
+/* ---- Graph (glass::Graph<int32_t>) ------------------------------------ */
+
+/* Graph(filename, format) / Graph::load: format "glass" (graph.hpp:136-153),
+ * "nsg" (graph.hpp:220-245) or "diskann" (graph.hpp:155-182). */
+int glass_hip_graph_load(const char *path, const char *format, glass_hip_graph **out);
+/* Graph::save (graph.hpp:106-119), .glass layout incl. HNSW upper layers. */
+int glass_hip_graph_save(const glass_hip_graph *g, const char *path);
+/* Build from arrays: adj is N*K int32, -1 padded; eps the entry points. */
+int glass_hip_graph_create(int32_t n, int32_t k, const int32_t *adj, int32_t nep, const int32_t *eps,
+                           glass_hip_graph **out);
+/* Attach HNSW upper layers (HNSWInitializer, HNSWInitializer.hpp:13-87):
+ * levels[n]; lists concatenated per node, levels[i]*up_k ints each. */
+int glass_hip_graph_set_upper(glass_hip_graph *g, int32_t up_k, int32_t ep, const int32_t *levels,
+                              const int32_t *lists);
+int glass_hip_graph_info(const glass_hip_graph *g, int32_t *n, int32_t *k, int32_t *nep, int32_t *has_upper);
+/* Copy adjacency (n*k) / entry points out. */
+int glass_hip_graph_copy(const glass_hip_graph *g, int32_t *adj, int32_t *eps);
+void glass_hip_graph_free(glass_hip_graph *g);
+
+/* ---- Searcher (GraphSearcherBase) -------------------------------------- */
+
+/* create_searcher(graph, metric, quantizer, refine_quant): metric "L2"/"IP";
+ * quantizer "SQ8U", "SQ4U", "SQ2U", "E5M2" (alias "FP8_E5M2"), "SQ8", "FP32";
+ * refine_quant "" or "FP32".  The graph's contents move into the searcher
+ * and `g` is left empty (python/bindings.cc:94-98). */
+int glass_hip_searcher_create(glass_hip_graph *g, const char *metric, const char *quantizer, const char *refine_quant,
+                              glass_hip_searcher **out);
+/* SetData: train the quantizer(s), encode every row on the device and keep
+ * codes, adjacency and FP32 rows resident in HBM. */
+int glass_hip_searcher_set_data(glass_hip_searcher *s, const float *data, int32_t n, int32_t dim);
+int glass_hip_searcher_set_ef(glass_hip_searcher *s, int32_t ef);
+int32_t glass_hip_searcher_get_ef(const glass_hip_searcher *s);
+/* Search: one query, GraphSearcher::Search semantics (pool (max(k,ef),max(k,ef))),
+ * one launch.  dst[k] ids, scores[k] distances (scores may be NULL). */
+int glass_hip_searcher_search(glass_hip_searcher *s, const float *q, int32_t k, int32_t *dst, float *scores);
+/* SearchBatch: nq queries, SearchBatch semantics (pool (ef, max(k,ef))),
+ * one persistent launch; dst[nq*k], scores[nq*k] (may be NULL). */
+int glass_hip_searcher_search_batch(glass_hip_searcher *s, const float *qs, int32_t nq, int32_t k, int32_t *dst,
+                                    float *scores);
+/* Optimize: autotune the launch configuration on sampled base rows. */
+int glass_hip_searcher_optimize(glass_hip_searcher *s, int32_t num_threads);
+/* GetLastSearchAvgDistCmps: mean distance evaluations of the last batch. */
+double glass_hip_searcher_last_avg_dist_cmps(const glass_hip_searcher *s);
+void glass_hip_searcher_free(glass_hip_searcher *s);
+
+/* ---- Extensions beyond the reference interface ------------------------ */
+
+/* Batch with per-query ef[nq] and k[nq] (SearchBatch semantics per query);
+ * results of query i start at out_off[i] (exclusive prefix sum of k), so dst
+ * and scores hold sum(k) entries.  counters (nullable) gets 3 ints per query:
+ * distance evaluations, expansions, upper-layer evaluations. */
+int glass_hip_searcher_search_batch_efk(glass_hip_searcher *s, const float *qs, int32_t nq, const int32_t *ef,
+                                        const int32_t *k, int32_t *dst, float *scores, int32_t *counters);
+/* Device-resident batch: qs_dev[nq*dim], dst_dev[nq*k], scores_dev (nullable),
+ * counters_dev (nullable, 3 per query), all device pointers; enqueued on
+ * `stream` (a hipStream_t, NULL = the searcher's stream) without host sync.
+ * mode 0 = Search semantics, 1 = SearchBatch semantics. */
+int glass_hip_searcher_search_batch_dev(glass_hip_searcher *s, const float *qs_dev, int32_t nq, int32_t k,
+                                        int32_t mode, int32_t *dst_dev, float *scores_dev, int32_t *counters_dev,
+                                        void *stream);
+/* Launch configuration (set by optimize): team = dwords per lane (4, 8, 16)
+ * for integer codes, waves per block, blocks of the persistent grid (0 =
+ * fill the device).  Query with _get. */
+int glass_hip_searcher_set_launch(glass_hip_searcher *s, int32_t team, int32_t waves_per_block, int32_t blocks);
+int glass_hip_searcher_get_launch(const glass_hip_searcher *s, int32_t *team, int32_t *waves_per_block,
+                                  int32_t *blocks, int32_t *slots);
+/* Sizes for accounting: code bytes per row, FP32 re-rank bytes per row,
+ * graph degree, number of rows, refine factor. */
+int glass_hip_searcher_info(const glass_hip_searcher *s, int32_t *code_size, int32_t *rerank_row_bytes,
+                            int32_t *degree, int32_t *n, float *refine_factor);
+/* Copy the device-resident codes (n*code_size bytes) back to the host. */
+int glass_hip_searcher_get_codes(const glass_hip_searcher *s, uint8_t *codes);
+/* Calibration parameters: [min, dif] (affine), [max] (symmetric) or
+ * [mins[dim_align], difs[dim_align]] (SQ8 per-dim); returns count via *n. */
+int glass_hip_searcher_get_calibration(const glass_hip_searcher *s, float *params, int32_t *n);
+/* Synchronize the searcher's stream. */
+int glass_hip_searcher_sync(glass_hip_searcher *s);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,240 @@
+"""pyglass_amd: MI355X-native query-time graph search with pyglass's API.
+
+`Graph` and `Searcher` mirror glass.Graph and glass.Searcher
+(python/bindings.cc:40-58,90-154,242-269 of the reference): same
+constructors, method names, argument meaning and return shapes.  Every call
+goes through the C ABI of libglass_hip.so (include/glass_hip.h), whose HIP
+kernels run the search on the GPU.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import GlassError
+
+__all__ = ["Graph", "Searcher", "GlassError", "set_num_threads", "device_count", "version"]
+
+_NUM_THREADS = 0
+
+
+def _p(a: np.ndarray):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def _as_f32(x) -> np.ndarray:
+    return np.ascontiguousarray(np.asarray(x, dtype=np.float32))
+
+
+def set_num_threads(num_threads: int):
+    """glass.set_num_threads (bindings.cc:38).  Host threads are only used by
+    graph preparation; the search runs on the GPU."""
+    global _NUM_THREADS
+    _NUM_THREADS = int(num_threads)
+
+
+def device_count() -> int:
+    return int(_lib.load().glass_hip_device_count())
+
+
+def version() -> str:
+    return _lib.load().glass_hip_version().decode()
+
+
+class Graph:
+    """glass.Graph: fixed-degree adjacency (N x K int32, -1 padded), entry
+    points and optional HNSW upper layers (glass/graph.hpp:26-246)."""
+
+    def __init__(self, filename: str | None = None, format: str = "glass"):
+        self._L = _lib.load()
+        self._h = C.c_void_p()
+        if filename is not None:
+            self.load(filename, format)
+
+    def _free(self):
+        if self._h:
+            self._L.glass_hip_graph_free(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self._free()
+        except Exception:
+            pass
+
+    def load(self, filename: str, format: str = "glass"):
+        self._free()
+        h = C.c_void_p()
+        _lib.check(self._L.glass_hip_graph_load(str(filename).encode(), format.encode(), C.byref(h)))
+        self._h = h
+
+    def save(self, filename: str):
+        if not self._h:
+            raise GlassError("empty graph")
+        _lib.check(self._L.glass_hip_graph_save(self._h, str(filename).encode()))
+
+    @classmethod
+    def from_arrays(cls, adj: np.ndarray, eps=(), upper=None) -> "Graph":
+        """adj: (N, K) int32, -1 padded.  upper: optional (up_K, ep, levels[N],
+        lists) for HNSW (lists concatenated per node, levels[i]*up_K each)."""
+        g = cls()
+        adj = np.ascontiguousarray(adj, dtype=np.int32)
+        eps = np.ascontiguousarray(np.asarray(eps, dtype=np.int32).reshape(-1))
+        h = C.c_void_p()
+        _lib.check(g._L.glass_hip_graph_create(adj.shape[0], adj.shape[1], _p(adj), len(eps), _p(eps), C.byref(h)))
+        g._h = h
+        if upper is not None:
+            up_k, ep, levels, lists = upper
+            levels = np.ascontiguousarray(levels, dtype=np.int32)
+            lists = np.ascontiguousarray(np.asarray(lists, dtype=np.int32).reshape(-1))
+            _lib.check(g._L.glass_hip_graph_set_upper(h, int(up_k), int(ep), _p(levels), _p(lists)))
+        return g
+
+    def info(self):
+        n, k, nep, up = C.c_int32(), C.c_int32(), C.c_int32(), C.c_int32()
+        _lib.check(self._L.glass_hip_graph_info(self._h, C.byref(n), C.byref(k), C.byref(nep), C.byref(up)))
+        return {"N": n.value, "K": k.value, "nep": nep.value, "has_upper": bool(up.value)}
+
+    def arrays(self):
+        inf = self.info()
+        adj = np.empty((inf["N"], inf["K"]), np.int32)
+        eps = np.empty(inf["nep"], np.int32)
+        _lib.check(self._L.glass_hip_graph_copy(self._h, _p(adj), _p(eps)))
+        return adj, eps
+
+    def _take(self):
+        h, self._h = self._h, C.c_void_p()
+        return h
+
+
+class Searcher:
+    """glass.Searcher(graph, data, metric, quantizer, refine_quant="").
+
+    The graph moves into the searcher (the Python Graph is left empty, as in
+    bindings.cc:94-98); `data` is trained on, encoded on the device and may be
+    freed by the caller afterwards."""
+
+    def __init__(self, graph: Graph, data, metric: str, quantizer: str, refine_quant: str = ""):
+        self._L = _lib.load()
+        self._h = C.c_void_p()
+        if not graph._h:
+            raise GlassError("empty graph")
+        h = C.c_void_p()
+        _lib.check(self._L.glass_hip_searcher_create(graph._h, metric.encode(), quantizer.encode(),
+                                                    refine_quant.encode(), C.byref(h)))
+        graph._free()  # contents moved; release the emptied shell
+        self._h = h
+        x = _as_f32(data)
+        if x.ndim == 1:
+            x = x.reshape(1, -1)
+        self.dim = x.shape[1]
+        _lib.check(self._L.glass_hip_searcher_set_data(self._h, _p(x), x.shape[0], x.shape[1]))
+
+    def __del__(self):
+        try:
+            if self._h:
+                self._L.glass_hip_searcher_free(self._h)
+                self._h = C.c_void_p()
+        except Exception:
+            pass
+
+    def set_ef(self, ef: int):
+        _lib.check(self._L.glass_hip_searcher_set_ef(self._h, int(ef)))
+
+    def get_ef(self) -> int:
+        return int(self._L.glass_hip_searcher_get_ef(self._h))
+
+    def _query(self, q) -> np.ndarray:
+        q = _as_f32(q)
+        if q.shape[-1] != self.dim:
+            raise GlassError(f"query dimension {q.shape[-1]} != data dimension {self.dim}")
+        return q
+
+    def search(self, query, k: int):
+        """One query, one launch (GraphSearcher::Search semantics)."""
+        q = self._query(query).reshape(-1)
+        ids = np.empty(int(k), np.int32)
+        dis = np.empty(int(k), np.float32)
+        _lib.check(self._L.glass_hip_searcher_search(self._h, _p(q), int(k), _p(ids), _p(dis)))
+        return ids, dis
+
+    def batch_search(self, query, k: int, num_threads: int = 0):
+        """All queries in one persistent launch (SearchBatch semantics)."""
+        q = self._query(query)
+        if q.ndim == 1:
+            q = q.reshape(1, -1)
+        nq = q.shape[0]
+        ids = np.empty((nq, int(k)), np.int32)
+        dis = np.empty((nq, int(k)), np.float32)
+        _lib.check(self._L.glass_hip_searcher_search_batch(self._h, _p(q), nq, int(k), _p(ids), _p(dis)))
+        return ids, dis
+
+    def batch_search_efk(self, query, ef, k, with_counters: bool = False):
+        """Per-query ef and k in one launch.  Returns lists of id / distance
+        arrays (and an (nq, 3) counter array: distance evaluations,
+        expansions, upper-layer evaluations)."""
+        q = self._query(query)
+        nq = q.shape[0]
+        ef = np.ascontiguousarray(np.broadcast_to(np.asarray(ef, np.int32), (nq,)))
+        kk = np.ascontiguousarray(np.broadcast_to(np.asarray(k, np.int32), (nq,)))
+        total = int(kk.sum())
+        ids = np.empty(total, np.int32)
+        dis = np.empty(total, np.float32)
+        cnt = np.zeros((nq, 3), np.int32)
+        _lib.check(self._L.glass_hip_searcher_search_batch_efk(self._h, _p(q), nq, _p(ef), _p(kk), _p(ids), _p(dis),
+                                                               _p(cnt)))
+        off = np.concatenate([[0], np.cumsum(kk)])
+        I = [ids[off[i]:off[i + 1]] for i in range(nq)]
+        D = [dis[off[i]:off[i + 1]] for i in range(nq)]
+        return (I, D, cnt) if with_counters else (I, D)
+
+    def search_batch_dev(self, q_ptr: int, nq: int, k: int, ids_ptr: int, dists_ptr: int = 0,
+                         counters_ptr: int = 0, stream: int = 0, mode: int = 1):
+        """Device-resident batch: all pointers are device addresses (e.g.
+        torch tensor .data_ptr()); enqueued on `stream` without a host sync."""
+        _lib.check(self._L.glass_hip_searcher_search_batch_dev(
+            self._h, C.c_void_p(q_ptr), int(nq), int(k), int(mode), C.c_void_p(ids_ptr),
+            C.c_void_p(dists_ptr or None), C.c_void_p(counters_ptr or None), C.c_void_p(stream or None)))
+
+    def optimize(self, num_threads: int = 0):
+        """Autotune the launch configuration (GraphSearcher::Optimize)."""
+        _lib.check(self._L.glass_hip_searcher_optimize(self._h, int(num_threads)))
+
+    def get_last_search_dist_cmps(self) -> float:
+        return float(self._L.glass_hip_searcher_last_avg_dist_cmps(self._h))
+
+    # ---- introspection -------------------------------------------------
+    def set_launch(self, team: int, waves_per_block: int, blocks: int = 0):
+        _lib.check(self._L.glass_hip_searcher_set_launch(self._h, int(team), int(waves_per_block), int(blocks)))
+
+    def get_launch(self):
+        t, w, b, s = C.c_int32(), C.c_int32(), C.c_int32(), C.c_int32()
+        _lib.check(self._L.glass_hip_searcher_get_launch(self._h, C.byref(t), C.byref(w), C.byref(b), C.byref(s)))
+        return {"team": t.value, "waves_per_block": w.value, "blocks": b.value, "slots": s.value}
+
+    def info(self):
+        cs, rb, deg, n = C.c_int32(), C.c_int32(), C.c_int32(), C.c_int32()
+        f = C.c_float()
+        _lib.check(self._L.glass_hip_searcher_info(self._h, C.byref(cs), C.byref(rb), C.byref(deg), C.byref(n),
+                                                   C.byref(f)))
+        return {"code_size": cs.value, "rerank_row_bytes": rb.value, "degree": deg.value, "n": n.value,
+                "refine_factor": f.value}
+
+    def codes(self) -> np.ndarray:
+        inf = self.info()
+        out = np.empty((inf["n"], inf["code_size"]), np.uint8)
+        _lib.check(self._L.glass_hip_searcher_get_codes(self._h, _p(out)))
+        return out
+
+    def calibration(self) -> np.ndarray:
+        n = C.c_int32()
+        _lib.check(self._L.glass_hip_searcher_get_calibration(self._h, None, C.byref(n)))
+        out = np.empty(max(1, n.value), np.float32)
+        _lib.check(self._L.glass_hip_searcher_get_calibration(self._h, _p(out), C.byref(n)))
+        return out[: n.value]
+
+    def sync(self):
+        _lib.check(self._L.glass_hip_searcher_sync(self._h))

@@ -1,0 +1,113 @@
+// Device-side SetData: encode every base row with the trained calibrator
+// (Quantizer::add, e.g. glass/quant/sq4u_quant.hpp:99-117) and lay out the
+// FP32 re-rank rows (FP32Quantizer::add, glass/quant/fp32_quant.hpp:25-31).
+// One thread per output byte; the arithmetic is the reference's
+// transform(): (x - min) / dif, clamp to [0,1], round half away from zero.
+#include "glass_hip_internal.h"
+
+namespace glass_hip {
+
+namespace {
+
+__device__ __forceinline__ uint32_t affine_code(float x, float mn, float dif, float mul) {
+    x = (x - mn) / dif;
+    x = x < 0.0f ? 0.0f : x;
+    x = x > 1.0f ? 1.0f : x;
+    return (uint32_t)(int32_t)roundf(x * mul);
+}
+
+__device__ __forceinline__ uint32_t sym_code(float x, float mx) {
+    x = x / mx;
+    x = x < -1.0f ? -1.0f : x;
+    x = x > 1.0f ? 1.0f : x;
+    return (uint32_t)(int32_t)roundf(x * 127.0f);
+}
+
+__global__ void encode_kernel(const float *data, int64_t n, int32_t dim, int32_t code_size, int32_t quant,
+                              int32_t metric, float mn, float dif, float mx, const float *mins, const float *difs,
+                              uint8_t *codes) {
+    const int64_t total = n * (int64_t)code_size;
+    for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+         idx += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t row = idx / code_size;
+        const int32_t b = (int32_t)(idx - row * code_size);
+        const float *x = data + row * dim;
+        uint32_t byte = 0;
+        switch (quant) {
+            case QK_SQ8U:  // sq8u_quant.hpp:38-42
+                if (b < dim) byte = (metric == MK_L2 ? affine_code(x[b], mn, dif, 127.0f) : sym_code(x[b], mx)) & 0xffu;
+                break;
+            case QK_SQ4U: {  // sq4u_quant.hpp:107-117
+                int j = 2 * b;
+                if (j < dim) byte |= affine_code(x[j], mn, dif, 15.0f) & 0xfu;
+                if (j + 1 < dim) byte |= (affine_code(x[j + 1], mn, dif, 15.0f) & 0xfu) << 4;
+                break;
+            }
+            case QK_SQ2U: {  // sq2u_quant.hpp:186-200
+                for (int s = 0; s < 4; ++s) {
+                    int j = 4 * b + s;
+                    if (j < dim) byte |= (affine_code(x[j], mn, dif, 3.0f) & 0x3u) << (2 * s);
+                }
+                break;
+            }
+            case QK_E5M2:  // e5m2_quant.hpp:30-34: high byte of the RNE fp16
+                if (b < dim) byte = ((uint32_t)__half_as_ushort(__float2half_rn(x[b])) >> 8) & 0xffu;
+                break;
+            case QK_SQ8:  // sq8_quant.hpp:43-47, AffinePerDimCalibrator<255>
+                if (b < dim) byte = affine_code(x[b], mins[b], difs[b], 255.0f) & 0xffu;
+                break;
+            default:
+                break;
+        }
+        codes[idx] = (uint8_t)byte;
+    }
+}
+
+__global__ void pad_rows_kernel(const float *data, int64_t n, int32_t dim, int32_t dim_align, float *out) {
+    const int64_t total = n * (int64_t)dim_align;
+    for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+         idx += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t row = idx / dim_align;
+        const int32_t j = (int32_t)(idx - row * dim_align);
+        out[idx] = j < dim ? data[row * dim + j] : 0.0f;
+    }
+}
+
+__global__ void fill_i32_kernel(int32_t *p, int64_t n, int32_t v) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+int grid_for(int64_t total) {
+    int64_t g = (total + 255) / 256;
+    if (g > 65536) g = 65536;
+    if (g < 1) g = 1;
+    return (int)g;
+}
+
+}  // namespace
+
+hipError_t launch_encode(const float *data_dev, int64_t n, int32_t dim, int32_t code_size, int32_t quant,
+                         int32_t metric, float mn, float dif, float mx, const float *mins, const float *difs,
+                         uint8_t *codes_dev, hipStream_t stream) {
+    const int64_t total = n * (int64_t)code_size;
+    hipLaunchKernelGGL(encode_kernel, dim3(grid_for(total)), dim3(256), 0, stream, data_dev, n, dim, code_size, quant,
+                       metric, mn, dif, mx, mins, difs, codes_dev);
+    return hipGetLastError();
+}
+
+hipError_t launch_pad_rows(const float *data_dev, int64_t n, int32_t dim, int32_t dim_align, float *out,
+                           hipStream_t stream) {
+    const int64_t total = n * (int64_t)dim_align;
+    hipLaunchKernelGGL(pad_rows_kernel, dim3(grid_for(total)), dim3(256), 0, stream, data_dev, n, dim, dim_align,
+                       out);
+    return hipGetLastError();
+}
+
+// Graph-capture-safe clear of small device counters (no hipMemsetAsync).
+hipError_t launch_fill_i32(int32_t *p, int64_t n, int32_t v, hipStream_t stream) {
+    hipLaunchKernelGGL(fill_i32_kernel, dim3(grid_for(n)), dim3(256), 0, stream, p, n, v);
+    return hipGetLastError();
+}
+
+}  // namespace glass_hip

@@ -1,0 +1,949 @@
+// Host runtime of libglass_hip.so: the C ABI declared in include/glass_hip.h.
+//
+// It owns the graph on the host, trains the scalar calibrators
+// (glass/quant/calibrator.hpp) on the CPU, uploads the base rows once, encodes
+// them on the device, keeps adjacency + codes + FP32 rows resident in HBM and
+// drives the HIP search kernels.  Searcher semantics follow
+// glass/searcher/graph_searcher.hpp and glass/searcher/refiner.hpp.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <memory>
+#include <mutex>
+#include <numeric>
+#include <random>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../../include/glass_hip.h"
+#include "glass_hip_internal.h"
+
+namespace glass_hip {
+hipError_t launch_encode(const float *data_dev, int64_t n, int32_t dim, int32_t code_size, int32_t quant,
+                         int32_t metric, float mn, float dif, float mx, const float *mins, const float *difs,
+                         uint8_t *codes_dev, hipStream_t stream);
+hipError_t launch_pad_rows(const float *data_dev, int64_t n, int32_t dim, int32_t dim_align, float *out,
+                           hipStream_t stream);
+hipError_t launch_fill_i32(int32_t *p, int64_t n, int32_t v, hipStream_t stream);
+}  // namespace glass_hip
+
+using namespace glass_hip;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const char *fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_last_error = buf;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                          \
+    do {                                                                                       \
+        hipError_t _e = (expr);                                                                \
+        if (_e != hipSuccess) {                                                                \
+            return fail(_e == hipErrorOutOfMemory ? GLASS_HIP_ENOMEM : GLASS_HIP_EDEVICE,      \
+                        "%s failed: %s", #expr, hipGetErrorString(_e));                        \
+        }                                                                                      \
+    } while (0)
+
+template <typename T>
+struct DevBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    ~DevBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    hipError_t alloc(size_t count) {
+        if (count <= n && p) return hipSuccess;
+        release();
+        hipError_t e = hipMalloc((void **)&p, std::max<size_t>(count, 1) * sizeof(T));
+        if (e == hipSuccess) n = count;
+        else p = nullptr;
+        return e;
+    }
+};
+
+// quantizer_map (glass/quant/quant.hpp:25-43) restricted to the kinds with
+// device kernels; "FP8_E5M2" is accepted as the README spelling of E5M2.
+int32_t parse_quant(const std::string &s) {
+    if (s == "SQ8U") return QK_SQ8U;
+    if (s == "SQ4U") return QK_SQ4U;
+    if (s == "SQ2U") return QK_SQ2U;
+    if (s == "E5M2" || s == "FP8_E5M2") return QK_E5M2;
+    if (s == "SQ8") return QK_SQ8;
+    if (s == "FP32") return QK_FP32;
+    return QK_NONE;
+}
+
+// Quantizer<metric, AlignWidth, NBits> per kind (glass/quant/*_quant.hpp).
+void quant_layout(int32_t kind, int32_t dim, int32_t *dim_align, int32_t *code_size) {
+    int32_t align = 16, nbits = 8;
+    switch (kind) {
+        case QK_SQ8U: align = 64; nbits = 8; break;
+        case QK_SQ4U: align = 128; nbits = 4; break;
+        case QK_SQ2U: align = 256; nbits = 2; break;
+        case QK_E5M2: align = 16; nbits = 8; break;
+        case QK_SQ8: align = 16; nbits = 8; break;
+        case QK_FP32: align = 16; nbits = 32; break;
+    }
+    *dim_align = (dim + align - 1) / align * align;  // Tensor ctor, glass/storage/tensor.hpp:30-35
+    *code_size = *dim_align * nbits / 8;
+}
+
+// get_refine_factor (glass/searcher/graph_searcher.hpp:227-238)
+float refine_factor_of(int32_t kind) {
+    if (kind == QK_SQ8U || kind == QK_SQ8 || kind == QK_SQ4U) return 1.5f;
+    if (kind == QK_SQ2U) return 3.0f;
+    return 1.0f;
+}
+
+// k-th smallest / k-th largest of v (the MaxHeap order statistic of
+// find_minmax, glass/quant/utils.hpp:31-40).
+std::pair<float, float> kth_extremes(std::vector<float> &v, int64_t top) {
+    top = std::min<int64_t>(top, (int64_t)v.size());
+    std::nth_element(v.begin(), v.begin() + (top - 1), v.end());
+    float lo = v[top - 1];
+    std::nth_element(v.begin(), v.begin() + ((int64_t)v.size() - top), v.end());
+    float hi = v[(int64_t)v.size() - top];
+    return {lo, hi};
+}
+
+}  // namespace
+
+struct glass_hip_graph {
+    int32_t N = 0, K = 0;
+    std::vector<int32_t> adj;
+    std::vector<int32_t> eps;
+    bool has_upper = false;
+    int32_t up_K = 0, ep = 0;
+    std::vector<int32_t> levels;
+    std::vector<int64_t> up_off;
+    std::vector<int32_t> up_lists;
+};
+
+struct glass_hip_searcher {
+    glass_hip_graph graph;
+    int32_t metric = MK_L2, quant = QK_SQ8U, refine = QK_NONE;
+    int32_t dim = 0, dim_align = 0, code_size = 0, r_dim_align = 0;
+    int32_t nb = 0;
+    float qmin = 0, qdif = 0, qmax = 0;
+    std::vector<float> sq8_mins, sq8_difs;
+    int32_t ef = 32;
+    double last_avg_dist_cmps = 0.0;
+    bool has_data = false;
+
+    // launch configuration
+    int32_t team = 8, waves_per_block = 4, blocks = 0;
+    int32_t cu_count = 0;
+
+    // device-resident index
+    DevBuf<int32_t> d_adj, d_levels, d_up_lists, d_eps;
+    DevBuf<int64_t> d_up_off;
+    DevBuf<uint8_t> d_codes;
+    DevBuf<float> d_rerank, d_mins, d_difs;
+    // per-slot state
+    DevBuf<uint32_t> d_visited;
+    int64_t vis_stride = 0;
+    int32_t nslots = 0;
+    DevBuf<int32_t> d_counter;
+    // staging for host-pointer calls
+    DevBuf<float> d_q;
+    DevBuf<int32_t> d_ids, d_counters, d_ef, d_k;
+    DevBuf<float> d_dists;
+    DevBuf<int64_t> d_off;
+    hipStream_t stream = nullptr;
+    std::mutex mu;
+
+    // Optimize sample (GraphSearcher::SetData keeps <=1000 base rows, graph_searcher.hpp:62-69)
+    std::vector<float> optimize_queries;
+    int32_t sample_points_num = 0;
+
+    ~glass_hip_searcher() {
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+
+    DeviceIndex index() const {
+        DeviceIndex ix{};
+        ix.adj = d_adj.p;
+        ix.N = graph.N;
+        ix.K = graph.K;
+        ix.has_init = graph.has_upper ? 1 : 0;
+        ix.up_K = graph.up_K;
+        ix.ep = graph.ep;
+        ix.nep = (int32_t)graph.eps.size();
+        ix.levels = d_levels.p;
+        ix.up_off = d_up_off.p;
+        ix.up_lists = d_up_lists.p;
+        ix.eps = d_eps.p;
+        ix.quant = quant;
+        ix.metric = metric;
+        ix.dim = dim;
+        ix.dim_align = dim_align;
+        ix.code_size = code_size;
+        ix.codes = d_codes.p;
+        ix.qmin = qmin;
+        ix.qdif = qdif;
+        ix.qmax = qmax;
+        ix.sq8_mins = d_mins.p;
+        ix.sq8_difs = d_difs.p;
+        ix.rerank = refine == QK_FP32 ? d_rerank.p : nullptr;
+        ix.r_dim_align = refine == QK_FP32 ? r_dim_align : 0;
+        ix.refine_factor = refine == QK_FP32 ? refine_factor_of(quant) : 1.0f;
+        return ix;
+    }
+};
+
+namespace {
+
+// Graph rows as the kernels expect them: ids in range, each row cut at its
+// first -1 and later duplicates of an id removed.  A repeated id in a row is
+// skipped by the visited test in SearchImpl1/2, so dropping it leaves the
+// visit order unchanged.
+int sanitize_graph(glass_hip_graph &g) {
+    const int32_t N = g.N, K = g.K;
+    std::atomic<int> bad{0};
+    auto work = [&](int64_t lo, int64_t hi) {
+        std::vector<int32_t> row(K), tmp(K);
+        for (int64_t i = lo; i < hi; ++i) {
+            int32_t *r = g.adj.data() + i * K;
+            int n = 0;
+            while (n < K && r[n] != -1) ++n;
+            for (int j = 0; j < n; ++j) {
+                if (r[j] < 0 || r[j] >= N) bad = 1;
+            }
+            std::copy(r, r + n, tmp.begin());
+            std::sort(tmp.begin(), tmp.begin() + n);
+            bool dup = std::adjacent_find(tmp.begin(), tmp.begin() + n) != tmp.begin() + n;
+            if (dup) {
+                int m = 0;
+                for (int j = 0; j < n; ++j) {
+                    bool seen = false;
+                    for (int t = 0; t < m; ++t) seen |= row[t] == r[j];
+                    if (!seen) row[m++] = r[j];
+                }
+                std::copy(row.begin(), row.begin() + m, r);
+                n = m;
+            }
+            for (int j = n; j < K; ++j) r[j] = -1;
+        }
+    };
+    unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    std::vector<std::thread> th;
+    int64_t chunk = ((int64_t)N + nt - 1) / nt;
+    for (unsigned t = 0; t < nt; ++t) {
+        int64_t lo = t * chunk, hi = std::min<int64_t>(N, lo + chunk);
+        if (lo < hi) th.emplace_back(work, lo, hi);
+    }
+    for (auto &t : th) t.join();
+    if (bad) return fail(GLASS_HIP_EINVAL, "graph has neighbour ids outside [0, %d)", N);
+    for (int32_t e : g.eps) {
+        if (e < 0 || e >= N) return fail(GLASS_HIP_EINVAL, "graph entry point %d outside [0, %d)", e, N);
+    }
+    if (g.has_upper) {
+        if (g.ep < 0 || g.ep >= N) return fail(GLASS_HIP_EINVAL, "HNSW entry point out of range");
+        if ((int32_t)g.levels.size() != N) return fail(GLASS_HIP_EINVAL, "HNSW levels size mismatch");
+        // every node listed at level l must itself have a level-l list, so the
+        // greedy descent never reads past a node's lists
+        for (int32_t u = 0; u < N; ++u) {
+            for (int32_t lv = 1; lv <= g.levels[u]; ++lv) {
+                const int32_t *list = g.up_lists.data() + g.up_off[u] + (int64_t)(lv - 1) * g.up_K;
+                for (int32_t i = 0; i < g.up_K && list[i] != -1; ++i) {
+                    const int32_t v = list[i];
+                    if (v < 0 || v >= N || g.levels[v] < lv)
+                        return fail(GLASS_HIP_EINVAL, "HNSW upper-layer id %d invalid at level %d", v, lv);
+                }
+            }
+        }
+    }
+    return GLASS_HIP_OK;
+}
+
+bool read_all(std::ifstream &f, void *p, size_t bytes) {
+    f.read((char *)p, (std::streamsize)bytes);
+    return (size_t)f.gcount() == bytes;
+}
+
+// Graph::load (glass/graph.hpp:136-153) + HNSWInitializer::load (HNSWInitializer.hpp:64-75),
+// with 64-bit byte counts.
+int load_glass(const char *path, glass_hip_graph &g) {
+    std::ifstream f(path, std::ios::binary);
+    if (!f) return fail(GLASS_HIP_EIO, "cannot open %s", path);
+    int32_t nep = 0;
+    if (!read_all(f, &nep, 4) || nep < 0 || nep > (1 << 24)) return fail(GLASS_HIP_EIO, "%s: bad header", path);
+    g.eps.resize(nep);
+    if (nep && !read_all(f, g.eps.data(), 4 * (size_t)nep)) return fail(GLASS_HIP_EIO, "%s: truncated eps", path);
+    if (!read_all(f, &g.N, 4) || !read_all(f, &g.K, 4) || g.N <= 0 || g.K <= 0)
+        return fail(GLASS_HIP_EIO, "%s: bad N/K", path);
+    g.adj.resize((size_t)g.N * g.K);
+    if (!read_all(f, g.adj.data(), (size_t)g.N * g.K * 4)) return fail(GLASS_HIP_EIO, "%s: truncated adjacency", path);
+    int32_t hdr[3];
+    f.read((char *)hdr, 12);
+    if (f.gcount() == 12) {
+        if (hdr[0] != g.N || hdr[1] <= 0) return fail(GLASS_HIP_EIO, "%s: bad upper-layer header", path);
+        g.has_upper = true;
+        g.up_K = hdr[1];
+        g.ep = hdr[2];
+        g.levels.assign(g.N, 0);
+        g.up_off.assign(g.N, 0);
+        g.up_lists.clear();
+        for (int32_t i = 0; i < g.N; ++i) {
+            int32_t cur = 0;
+            if (!read_all(f, &cur, 4) || cur < 0 || cur % g.up_K) return fail(GLASS_HIP_EIO, "%s: bad level", path);
+            g.levels[i] = cur / g.up_K;
+            g.up_off[i] = (int64_t)g.up_lists.size();
+            if (cur) {
+                size_t o = g.up_lists.size();
+                g.up_lists.resize(o + cur);
+                if (!read_all(f, g.up_lists.data() + o, 4 * (size_t)cur))
+                    return fail(GLASS_HIP_EIO, "%s: truncated upper lists", path);
+            }
+        }
+    }
+    return GLASS_HIP_OK;
+}
+
+// Graph::load_nsg (graph.hpp:220-245): K, ep, then per node (cnt, ids[cnt]).
+int load_nsg(const char *path, glass_hip_graph &g) {
+    std::ifstream f(path, std::ios::binary | std::ios::ate);
+    if (!f) return fail(GLASS_HIP_EIO, "cannot open %s", path);
+    int64_t size = f.tellg();
+    f.seekg(0);
+    int32_t K = 0, ep = 0;
+    if (!read_all(f, &K, 4) || !read_all(f, &ep, 4) || K <= 0) return fail(GLASS_HIP_EIO, "%s: bad nsg header", path);
+    std::vector<std::vector<int32_t>> rows;
+    while ((int64_t)f.tellg() < size) {
+        int32_t c = 0;
+        if (!read_all(f, &c, 4) || c < 0 || c > K) return fail(GLASS_HIP_EIO, "%s: bad nsg row", path);
+        rows.emplace_back(c);
+        if (c && !read_all(f, rows.back().data(), 4 * (size_t)c)) return fail(GLASS_HIP_EIO, "%s: truncated", path);
+    }
+    g.N = (int32_t)rows.size();
+    g.K = K;
+    g.eps = {ep};
+    g.adj.assign((size_t)g.N * K, -1);
+    for (int32_t i = 0; i < g.N; ++i) std::copy(rows[i].begin(), rows[i].end(), g.adj.begin() + (size_t)i * K);
+    return GLASS_HIP_OK;
+}
+
+// Graph::load_diskann (graph.hpp:155-182): u64 size, K, ep, u64, rows (cnt, ids).
+int load_diskann(const char *path, glass_hip_graph &g) {
+    std::ifstream f(path, std::ios::binary);
+    if (!f) return fail(GLASS_HIP_EIO, "cannot open %s", path);
+    uint64_t size = 0, x = 0;
+    int32_t K = 0, ep = 0;
+    if (!read_all(f, &size, 8) || !read_all(f, &K, 4) || !read_all(f, &ep, 4) || !read_all(f, &x, 8) || K <= 0)
+        return fail(GLASS_HIP_EIO, "%s: bad diskann header", path);
+    std::vector<std::vector<int32_t>> rows;
+    while ((uint64_t)f.tellg() < size) {
+        int32_t c = 0;
+        if (!read_all(f, &c, 4) || c < 0 || c > K) return fail(GLASS_HIP_EIO, "%s: bad diskann row", path);
+        rows.emplace_back(c);
+        if (c && !read_all(f, rows.back().data(), 4 * (size_t)c)) return fail(GLASS_HIP_EIO, "%s: truncated", path);
+    }
+    g.N = (int32_t)rows.size();
+    g.K = K;
+    g.eps = {ep};
+    g.adj.assign((size_t)g.N * K, -1);
+    for (int32_t i = 0; i < g.N; ++i) std::copy(rows[i].begin(), rows[i].end(), g.adj.begin() + (size_t)i * K);
+    return GLASS_HIP_OK;
+}
+
+// Host calibration (train) for the search quantizer.
+int train_quant(glass_hip_searcher &s, const float *data, int64_t n) {
+    const int64_t nd = n * (int64_t)s.dim;
+    switch (s.quant) {
+        case QK_SQ8U:
+            if (s.metric == MK_L2) {  // AffineCalibrator<127>, drop 0 (sq8u_quant.hpp:19-28)
+                auto mm = std::minmax_element(data, data + nd);
+                s.qmin = *mm.first;
+                s.qdif = *mm.second - *mm.first;
+            } else {  // SymCalibrator<127>: absolute maximum
+                float m = 0.0f;
+                for (int64_t i = 0; i < nd; ++i) m = std::max(m, std::fabs(data[i]));
+                s.qmax = m;
+            }
+            break;
+        case QK_SQ4U:
+        case QK_SQ2U: {
+            // first 10% of the flattened items; drop 1% (SQ4U) / 5% (SQ2U) tails;
+            // counts computed in float as in sq4u_quant.hpp:95-97 and utils.hpp:31-33
+            const int64_t nitems = (int64_t)((float)nd * 0.1f);
+            const float ratio = s.quant == QK_SQ4U ? 0.01f : 0.05f;
+            const int64_t top = (int64_t)((float)nitems * ratio) + 1;
+            if (nitems <= 0) return fail(GLASS_HIP_EINVAL, "too few rows to calibrate");
+            std::vector<float> v(data, data + nitems);
+            auto mm = kth_extremes(v, top);
+            s.qmin = mm.first;
+            s.qdif = mm.second - mm.first;
+            break;
+        }
+        case QK_SQ8: {
+            // AffinePerDimCalibrator<255> through find_minmax_perdim
+            // (glass/quant/utils.hpp:52-70), which bins flattened item i into
+            // "dimension" i / n; padding dims keep min = dif = 0.
+            s.sq8_mins.assign(s.dim_align, 0.0f);
+            s.sq8_difs.assign(s.dim_align, 0.0f);
+            for (int32_t j = 0; j < s.dim; ++j) {
+                auto mm = std::minmax_element(data + (int64_t)j * n, data + (int64_t)(j + 1) * n);
+                s.sq8_mins[j] = *mm.first;
+                s.sq8_difs[j] = *mm.second - *mm.first;
+            }
+            break;
+        }
+        default:
+            break;
+    }
+    return GLASS_HIP_OK;
+}
+
+// GenRandom (glass/utils.hpp:13-27) with a default-seeded mt19937: the rows
+// GraphSearcher::SetData samples for Optimize.
+void gen_random(std::mt19937 &rng, int32_t *addr, int32_t size, int32_t N) {
+    for (int32_t i = 0; i < size; ++i) addr[i] = (int32_t)(rng() % (uint32_t)(N - size));
+    std::sort(addr, addr + size);
+    for (int32_t i = 1; i < size; ++i) {
+        if (addr[i] <= addr[i - 1]) addr[i] = addr[i - 1] + 1;
+    }
+    int32_t off = (int32_t)(rng() % (uint32_t)N);
+    for (int32_t i = 0; i < size; ++i) addr[i] = (addr[i] + off) % N;
+}
+
+int ensure_stream(glass_hip_searcher &s) {
+    if (!s.stream) HIP_TRY(hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking));
+    if (!s.cu_count) {
+        int dev = 0;
+        HIP_TRY(hipGetDevice(&dev));
+        hipDeviceProp_t prop;
+        HIP_TRY(hipGetDeviceProperties(&prop, dev));
+        s.cu_count = prop.multiProcessorCount;
+    }
+    return GLASS_HIP_OK;
+}
+
+// Persistent grid size: enough waves to fill the device at the kernel's
+// occupancy (bounded by LDS per wave), never more slots than queries.
+int32_t grid_blocks(const glass_hip_searcher &s, int32_t nq, int32_t cap_max) {
+    const size_t lds_wave = lds_bytes_per_wave(s.index(), cap_max);
+    const int32_t wpb = s.waves_per_block;
+    int32_t by_lds = (int32_t)((160 * 1024) / std::max<size_t>(1, lds_wave * wpb));
+    int32_t by_waves = 32 / wpb;  // 8 waves per SIMD x 4 SIMDs
+    int32_t per_cu = std::max(1, std::min(by_lds, by_waves));
+    int32_t full = per_cu * std::max(1, s.cu_count);
+    if (s.blocks > 0) full = std::min(full, s.blocks);
+    int32_t need = (nq + wpb - 1) / wpb;
+    return std::max(1, std::min(full, need));
+}
+
+int ensure_slots(glass_hip_searcher &s, int32_t slots) {
+    if (slots <= s.nslots && s.d_visited.p) return GLASS_HIP_OK;
+    s.vis_stride = ((int64_t)(s.graph.N + 31) / 32 + 3) / 4 * 4;
+    HIP_TRY(s.d_visited.alloc((size_t)slots * (size_t)s.vis_stride));
+    s.nslots = slots;
+    return GLASS_HIP_OK;
+}
+
+int32_t round_cap(int32_t cap) {
+    int32_t c = 64;
+    while (c < cap) c <<= 1;
+    return c;
+}
+
+// Enqueue one batch: queries/outputs are device pointers.
+int run_batch(glass_hip_searcher &s, const float *qs_dev, int32_t nq, const int32_t *ef_dev, const int32_t *k_dev,
+              int32_t ef_uniform, int32_t k_uniform, int32_t max_cap, int32_t mode, const int64_t *off_dev,
+              int32_t *ids_dev, float *dists_dev, int32_t *counters_dev, hipStream_t stream, int32_t blocks_override,
+              int32_t wpb_override) {
+    if (!s.has_data) return fail(GLASS_HIP_ESTATE, "search before set_data");
+    if (nq <= 0) return GLASS_HIP_OK;
+    if (int r = ensure_stream(s)) return r;
+    if (!stream) stream = s.stream;
+    const int32_t cap_max = round_cap(max_cap);
+    const int32_t saved_wpb = s.waves_per_block;
+    if (wpb_override > 0) s.waves_per_block = wpb_override;
+    int32_t blocks = blocks_override > 0 ? blocks_override : grid_blocks(s, nq, cap_max);
+    LaunchConfig lc{};
+    lc.team = s.team;
+    lc.waves_per_block = s.waves_per_block;
+    lc.blocks = blocks;
+    lc.cap_max = cap_max;
+    s.waves_per_block = saved_wpb;
+    if (!team_valid(s.index(), lc.team)) lc.team = default_team(s.index());
+    const size_t lds = lds_bytes_per_wave(s.index(), cap_max) * (size_t)lc.waves_per_block;
+    if (lds > 160 * 1024) return fail(GLASS_HIP_EINVAL, "pool capacity %d needs %zu bytes of LDS", cap_max, lds);
+    if (int r = ensure_slots(s, blocks * lc.waves_per_block)) return r;
+    if (!s.d_counter.p) HIP_TRY(s.d_counter.alloc(1));
+    HIP_TRY(launch_fill_i32(s.d_counter.p, 1, 0, stream));
+    QueryBatch qb{};
+    qb.queries = qs_dev;
+    qb.nq = nq;
+    qb.dim = s.dim;
+    qb.ef = ef_dev;
+    qb.k = k_dev;
+    qb.ef_uniform = ef_uniform;
+    qb.k_uniform = k_uniform;
+    qb.mode = mode;
+    qb.out_off = off_dev;
+    qb.out_ids = ids_dev;
+    qb.out_dists = dists_dev;
+    qb.out_counters = counters_dev;
+    SlotState ss{};
+    ss.visited = s.d_visited.p;
+    ss.vis_stride = s.vis_stride;
+    ss.nslots = s.nslots;
+    ss.work_counter = s.d_counter.p;
+    HIP_TRY(launch_search(s.index(), qb, lc, ss, stream));
+    return GLASS_HIP_OK;
+}
+
+int32_t inner_k(const glass_hip_searcher &s, int32_t k) {
+    if (s.refine == QK_FP32) {
+        int32_t rk = (int32_t)((float)k * refine_factor_of(s.quant));
+        return std::max(rk, k);
+    }
+    return k;
+}
+
+// Host-pointer batch used by search / search_batch / search_batch_efk.
+int host_batch(glass_hip_searcher &s, const float *qs, int32_t nq, const int32_t *ef, const int32_t *k,
+               int32_t k_uniform, int32_t mode, int32_t *dst, float *scores, int32_t *counters_out,
+               bool record_avg) {
+    if (!s.has_data) return fail(GLASS_HIP_ESTATE, "search before set_data");
+    if (nq <= 0) return GLASS_HIP_OK;
+    if (int r = ensure_stream(s)) return r;
+    int64_t total = 0;
+    int32_t max_cap = 1;
+    std::vector<int64_t> off;
+    if (k) {
+        off.resize(nq);
+        for (int32_t i = 0; i < nq; ++i) {
+            if (k[i] <= 0 || ef[i] <= 0) return fail(GLASS_HIP_EINVAL, "k and ef must be positive");
+            off[i] = total;
+            total += k[i];
+            max_cap = std::max(max_cap, std::max(inner_k(s, k[i]), ef[i]));
+        }
+    } else {
+        if (k_uniform <= 0) return fail(GLASS_HIP_EINVAL, "k must be positive");
+        total = (int64_t)nq * k_uniform;
+        max_cap = std::max(inner_k(s, k_uniform), s.ef);
+    }
+    if (round_cap(max_cap) > 4096) return fail(GLASS_HIP_EINVAL, "max(k, ef) = %d exceeds 4096", max_cap);
+    HIP_TRY(s.d_q.alloc((size_t)nq * s.dim));
+    HIP_TRY(s.d_ids.alloc((size_t)total));
+    HIP_TRY(s.d_dists.alloc((size_t)total));
+    HIP_TRY(s.d_counters.alloc((size_t)nq * 3));
+    HIP_TRY(hipMemcpyAsync(s.d_q.p, qs, sizeof(float) * (size_t)nq * s.dim, hipMemcpyHostToDevice, s.stream));
+    const int32_t *d_ef = nullptr, *d_k = nullptr;
+    const int64_t *d_off = nullptr;
+    if (k) {
+        HIP_TRY(s.d_ef.alloc(nq));
+        HIP_TRY(s.d_k.alloc(nq));
+        HIP_TRY(s.d_off.alloc(nq));
+        HIP_TRY(hipMemcpyAsync(s.d_ef.p, ef, 4 * (size_t)nq, hipMemcpyHostToDevice, s.stream));
+        HIP_TRY(hipMemcpyAsync(s.d_k.p, k, 4 * (size_t)nq, hipMemcpyHostToDevice, s.stream));
+        HIP_TRY(hipMemcpyAsync(s.d_off.p, off.data(), 8 * (size_t)nq, hipMemcpyHostToDevice, s.stream));
+        d_ef = s.d_ef.p;
+        d_k = s.d_k.p;
+        d_off = s.d_off.p;
+    }
+    int r = run_batch(s, s.d_q.p, nq, d_ef, d_k, s.ef, k_uniform, max_cap, mode, d_off, s.d_ids.p, s.d_dists.p,
+                      s.d_counters.p, s.stream, 0, 0);
+    if (r) return r;
+    HIP_TRY(hipMemcpyAsync(dst, s.d_ids.p, 4 * (size_t)total, hipMemcpyDeviceToHost, s.stream));
+    if (scores)
+        HIP_TRY(hipMemcpyAsync(scores, s.d_dists.p, 4 * (size_t)total, hipMemcpyDeviceToHost, s.stream));
+    std::vector<int32_t> cnt((size_t)nq * 3);
+    HIP_TRY(hipMemcpyAsync(cnt.data(), s.d_counters.p, 4 * cnt.size(), hipMemcpyDeviceToHost, s.stream));
+    HIP_TRY(hipStreamSynchronize(s.stream));
+    if (counters_out) std::copy(cnt.begin(), cnt.end(), counters_out);
+    if (record_avg) {
+        int64_t sum = 0;
+        for (int32_t i = 0; i < nq; ++i) sum += cnt[(size_t)i * 3];
+        s.last_avg_dist_cmps = (double)sum / nq;
+    }
+    return GLASS_HIP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *glass_hip_last_error(void) { return g_last_error.c_str(); }
+
+const char *glass_hip_version(void) { return "glass_hip 0.1.0 (gfx950)"; }
+
+int glass_hip_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+int glass_hip_graph_load(const char *path, const char *format, glass_hip_graph **out) {
+    if (!path || !out) return fail(GLASS_HIP_EINVAL, "null argument");
+    std::string fmt = format ? format : "glass";
+    auto g = std::make_unique<glass_hip_graph>();
+    int r;
+    if (fmt == "glass") r = load_glass(path, *g);
+    else if (fmt == "nsg") r = load_nsg(path, *g);
+    else if (fmt == "diskann") r = load_diskann(path, *g);
+    else return fail(GLASS_HIP_EINVAL, "unknown graph format '%s'", fmt.c_str());
+    if (r) return r;
+    *out = g.release();
+    return GLASS_HIP_OK;
+}
+
+int glass_hip_graph_save(const glass_hip_graph *g, const char *path) {
+    if (!g || !path) return fail(GLASS_HIP_EINVAL, "null argument");
+    std::ofstream f(path, std::ios::binary);
+    if (!f) return fail(GLASS_HIP_EIO, "cannot write %s", path);
+    int32_t nep = (int32_t)g->eps.size();
+    f.write((const char *)&nep, 4);
+    f.write((const char *)g->eps.data(), 4 * (size_t)nep);
+    f.write((const char *)&g->N, 4);
+    f.write((const char *)&g->K, 4);
+    f.write((const char *)g->adj.data(), (std::streamsize)(4 * (size_t)g->N * g->K));
+    if (g->has_upper) {  // HNSWInitializer::save (HNSWInitializer.hpp:77-86)
+        f.write((const char *)&g->N, 4);
+        f.write((const char *)&g->up_K, 4);
+        f.write((const char *)&g->ep, 4);
+        for (int32_t i = 0; i < g->N; ++i) {
+            int32_t cur = g->levels[i] * g->up_K;
+            f.write((const char *)&cur, 4);
+            if (cur) f.write((const char *)(g->up_lists.data() + g->up_off[i]), 4 * (size_t)cur);
+        }
+    }
+    if (!f) return fail(GLASS_HIP_EIO, "write to %s failed", path);
+    return GLASS_HIP_OK;
+}
+
+int glass_hip_graph_create(int32_t n, int32_t k, const int32_t *adj, int32_t nep, const int32_t *eps,
+                           glass_hip_graph **out) {
+    if (n <= 0 || k <= 0 || !adj || nep < 0 || (nep && !eps) || !out) return fail(GLASS_HIP_EINVAL, "bad graph arrays");
+    auto g = std::make_unique<glass_hip_graph>();
+    g->N = n;
+    g->K = k;
+    g->adj.assign(adj, adj + (size_t)n * k);
+    g->eps.assign(eps, eps + nep);
+    *out = g.release();
+    return GLASS_HIP_OK;
+}
+
+int glass_hip_graph_set_upper(glass_hip_graph *g, int32_t up_k, int32_t ep, const int32_t *levels,
+                              const int32_t *lists) {
+    if (!g || up_k <= 0 || !levels) return fail(GLASS_HIP_EINVAL, "bad upper-layer arrays");
+    g->has_upper = true;
+    g->up_K = up_k;
+    g->ep = ep;
+    g->levels.assign(levels, levels + g->N);
+    g->up_off.assign(g->N, 0);
+    int64_t total = 0;
+    for (int32_t i = 0; i < g->N; ++i) {
+        if (levels[i] < 0) return fail(GLASS_HIP_EINVAL, "negative level");
+        g->up_off[i] = total;
+        total += (int64_t)levels[i] * up_k;
+    }
+    if (total && !lists) return fail(GLASS_HIP_EINVAL, "missing upper lists");
+    g->up_lists.assign(lists, lists + total);
+    return GLASS_HIP_OK;
+}
+
+int glass_hip_graph_info(const glass_hip_graph *g, int32_t *n, int32_t *k, int32_t *nep, int32_t *has_upper) {
+    if (!g) return fail(GLASS_HIP_EINVAL, "null graph");
+    if (n) *n = g->N;
+    if (k) *k = g->K;
+    if (nep) *nep = (int32_t)g->eps.size();
+    if (has_upper) *has_upper = g->has_upper ? 1 : 0;
+    return GLASS_HIP_OK;
+}
+
+int glass_hip_graph_copy(const glass_hip_graph *g, int32_t *adj, int32_t *eps) {
+    if (!g) return fail(GLASS_HIP_EINVAL, "null graph");
+    if (adj) std::copy(g->adj.begin(), g->adj.end(), adj);
+    if (eps) std::copy(g->eps.begin(), g->eps.end(), eps);
+    return GLASS_HIP_OK;
+}
+
+void glass_hip_graph_free(glass_hip_graph *g) { delete g; }
+
+int glass_hip_searcher_create(glass_hip_graph *g, const char *metric, const char *quantizer, const char *refine_quant,
+                              glass_hip_searcher **out) {
+    if (!g || !metric || !quantizer || !out) return fail(GLASS_HIP_EINVAL, "null argument");
+    if (g->N <= 0) return fail(GLASS_HIP_EINVAL, "empty graph (was it moved into another searcher?)");
+    std::string m(metric), q(quantizer), r(refine_quant ? refine_quant : "");
+    int32_t mk;
+    if (m == "L2") mk = MK_L2;
+    else if (m == "IP") mk = MK_IP;
+    else return fail(GLASS_HIP_EINVAL, "unknown metric '%s'", metric);
+    int32_t qk = parse_quant(q);
+    if (qk == QK_NONE) return fail(GLASS_HIP_EINVAL, "unsupported quantizer '%s'", quantizer);
+    int32_t rk = QK_NONE;
+    if (!r.empty()) {
+        if (r != "FP32") return fail(GLASS_HIP_EINVAL, "unsupported refine quantizer '%s'", refine_quant);
+        rk = QK_FP32;
+    }
+    if (g->K > 64) return fail(GLASS_HIP_EINVAL, "graph degree %d > 64 is not supported", g->K);
+    if (g->has_upper && g->up_K > 64) return fail(GLASS_HIP_EINVAL, "upper-layer degree %d > 64", g->up_K);
+    if (!g->has_upper && g->eps.empty()) return fail(GLASS_HIP_EINVAL, "graph has no entry point");
+    if (!g->has_upper && g->eps.size() > 64) return fail(GLASS_HIP_EINVAL, "more than 64 entry points");
+    auto s = std::make_unique<glass_hip_searcher>();
+    s->graph = std::move(*g);
+    *g = glass_hip_graph();  // the caller's graph is left empty (bindings.cc:94-98)
+    s->metric = mk;
+    // SQ2U with IP instantiates the L2 searcher (graph_searcher.hpp:357)
+    s->quant = qk;
+    s->refine = rk;
+    if (int rr = sanitize_graph(s->graph)) return rr;
+    *out = s.release();
+    return GLASS_HIP_OK;
+}
+
+int glass_hip_searcher_set_data(glass_hip_searcher *s, const float *data, int32_t n, int32_t dim) {
+    if (!s || !data || n <= 0 || dim <= 0) return fail(GLASS_HIP_EINVAL, "bad data");
+    if (n != s->graph.N) return fail(GLASS_HIP_EINVAL, "data has %d rows but the graph has %d nodes", n, s->graph.N);
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (int r = ensure_stream(*s)) return r;
+    s->nb = n;
+    s->dim = dim;
+    quant_layout(s->quant, dim, &s->dim_align, &s->code_size);
+    if (s->refine == QK_FP32) {
+        int32_t cs;
+        quant_layout(QK_FP32, dim, &s->r_dim_align, &cs);
+    }
+    if (int r = train_quant(*s, data, n)) return r;
+    // graph
+    const glass_hip_graph &g = s->graph;
+    HIP_TRY(s->d_adj.alloc(g.adj.size()));
+    HIP_TRY(hipMemcpy(s->d_adj.p, g.adj.data(), 4 * g.adj.size(), hipMemcpyHostToDevice));
+    HIP_TRY(s->d_eps.alloc(std::max<size_t>(1, g.eps.size())));
+    if (!g.eps.empty()) HIP_TRY(hipMemcpy(s->d_eps.p, g.eps.data(), 4 * g.eps.size(), hipMemcpyHostToDevice));
+    if (g.has_upper) {
+        HIP_TRY(s->d_levels.alloc(g.levels.size()));
+        HIP_TRY(hipMemcpy(s->d_levels.p, g.levels.data(), 4 * g.levels.size(), hipMemcpyHostToDevice));
+        HIP_TRY(s->d_up_off.alloc(g.up_off.size()));
+        HIP_TRY(hipMemcpy(s->d_up_off.p, g.up_off.data(), 8 * g.up_off.size(), hipMemcpyHostToDevice));
+        HIP_TRY(s->d_up_lists.alloc(std::max<size_t>(1, g.up_lists.size())));
+        if (!g.up_lists.empty())
+            HIP_TRY(hipMemcpy(s->d_up_lists.p, g.up_lists.data(), 4 * g.up_lists.size(), hipMemcpyHostToDevice));
+    }
+    // base rows -> device, encode there
+    DevBuf<float> raw;
+    HIP_TRY(raw.alloc((size_t)n * dim));
+    HIP_TRY(hipMemcpy(raw.p, data, sizeof(float) * (size_t)n * dim, hipMemcpyHostToDevice));
+    if (s->quant == QK_SQ8) {
+        HIP_TRY(s->d_mins.alloc(s->dim_align));
+        HIP_TRY(s->d_difs.alloc(s->dim_align));
+        HIP_TRY(hipMemcpy(s->d_mins.p, s->sq8_mins.data(), 4 * (size_t)s->dim_align, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(s->d_difs.p, s->sq8_difs.data(), 4 * (size_t)s->dim_align, hipMemcpyHostToDevice));
+    }
+    HIP_TRY(s->d_codes.alloc((size_t)n * s->code_size));
+    if (s->quant == QK_FP32) {
+        HIP_TRY(launch_pad_rows(raw.p, n, dim, s->dim_align, (float *)s->d_codes.p, s->stream));
+    } else {
+        HIP_TRY(launch_encode(raw.p, n, dim, s->code_size, s->quant, s->metric, s->qmin, s->qdif, s->qmax,
+                              s->d_mins.p, s->d_difs.p, s->d_codes.p, s->stream));
+    }
+    if (s->refine == QK_FP32) {
+        HIP_TRY(s->d_rerank.alloc((size_t)n * s->r_dim_align));
+        HIP_TRY(launch_pad_rows(raw.p, n, dim, s->r_dim_align, s->d_rerank.p, s->stream));
+    }
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    raw.release();
+    // Optimize sample (graph_searcher.hpp:62-69)
+    s->sample_points_num = std::min(1000, n - 1);
+    if (s->sample_points_num > 0) {
+        std::vector<int32_t> pts(s->sample_points_num);
+        std::mt19937 rng;
+        gen_random(rng, pts.data(), s->sample_points_num, n);
+        s->optimize_queries.resize((size_t)s->sample_points_num * dim);
+        for (int32_t i = 0; i < s->sample_points_num; ++i)
+            std::memcpy(s->optimize_queries.data() + (size_t)i * dim, data + (size_t)pts[i] * dim, 4 * (size_t)dim);
+    }
+    s->team = default_team(s->index());
+    s->has_data = true;
+    return GLASS_HIP_OK;
+}
+
+int glass_hip_searcher_set_ef(glass_hip_searcher *s, int32_t ef) {
+    if (!s || ef <= 0) return fail(GLASS_HIP_EINVAL, "ef must be positive");
+    s->ef = ef;
+    return GLASS_HIP_OK;
+}
+
+int32_t glass_hip_searcher_get_ef(const glass_hip_searcher *s) { return s ? s->ef : 0; }
+
+int glass_hip_searcher_search(glass_hip_searcher *s, const float *q, int32_t k, int32_t *dst, float *scores) {
+    if (!s || !q || !dst || k <= 0) return fail(GLASS_HIP_EINVAL, "bad search arguments");
+    std::lock_guard<std::mutex> lk(s->mu);
+    // one query per launch: a single wave
+    const int32_t saved = s->blocks;
+    const int32_t saved_wpb = s->waves_per_block;
+    s->blocks = 1;
+    s->waves_per_block = 1;
+    int r = host_batch(*s, q, 1, nullptr, nullptr, k, MODE_SEARCH, dst, scores, nullptr, false);
+    s->blocks = saved;
+    s->waves_per_block = saved_wpb;
+    return r;
+}
+
+int glass_hip_searcher_search_batch(glass_hip_searcher *s, const float *qs, int32_t nq, int32_t k, int32_t *dst,
+                                    float *scores) {
+    if (!s || (!qs && nq) || (!dst && nq) || nq < 0 || k <= 0) return fail(GLASS_HIP_EINVAL, "bad batch arguments");
+    std::lock_guard<std::mutex> lk(s->mu);
+    return host_batch(*s, qs, nq, nullptr, nullptr, k, MODE_BATCH, dst, scores, nullptr, true);
+}
+
+int glass_hip_searcher_search_batch_efk(glass_hip_searcher *s, const float *qs, int32_t nq, const int32_t *ef,
+                                        const int32_t *k, int32_t *dst, float *scores, int32_t *counters) {
+    if (!s || nq < 0 || (nq && (!qs || !ef || !k || !dst))) return fail(GLASS_HIP_EINVAL, "bad batch arguments");
+    std::lock_guard<std::mutex> lk(s->mu);
+    return host_batch(*s, qs, nq, ef, k, 0, MODE_BATCH, dst, scores, counters, true);
+}
+
+int glass_hip_searcher_search_batch_dev(glass_hip_searcher *s, const float *qs_dev, int32_t nq, int32_t k,
+                                        int32_t mode, int32_t *dst_dev, float *scores_dev, int32_t *counters_dev,
+                                        void *stream) {
+    if (!s || nq < 0 || k <= 0 || (nq && (!qs_dev || !dst_dev))) return fail(GLASS_HIP_EINVAL, "bad batch arguments");
+    if (mode != MODE_SEARCH && mode != MODE_BATCH) return fail(GLASS_HIP_EINVAL, "mode must be 0 or 1");
+    std::lock_guard<std::mutex> lk(s->mu);
+    const int32_t max_cap = std::max(inner_k(*s, k), s->ef);
+    if (round_cap(max_cap) > 4096) return fail(GLASS_HIP_EINVAL, "max(k, ef) = %d exceeds 4096", max_cap);
+    return run_batch(*s, qs_dev, nq, nullptr, nullptr, s->ef, k, max_cap, mode, nullptr, dst_dev, scores_dev,
+                     counters_dev, (hipStream_t)stream, 0, 0);
+}
+
+int glass_hip_searcher_optimize(glass_hip_searcher *s, int32_t num_threads) {
+    (void)num_threads;  // unused by the reference too (graph_searcher.hpp:77-79)
+    if (!s) return fail(GLASS_HIP_EINVAL, "null searcher");
+    if (!s->has_data) return fail(GLASS_HIP_ESTATE, "optimize before set_data");
+    std::lock_guard<std::mutex> lk(s->mu);
+    const int32_t nq = s->sample_points_num;
+    if (nq <= 0) return GLASS_HIP_OK;
+    const int32_t kTryK = 10;
+    // Grid over the launch knobs, each point timed on the sampled rows with
+    // SearchBatch at k=10, as GraphSearcher::Optimize times prefetch knobs.
+    std::vector<int32_t> teams;
+    for (int32_t t : {4, 8, 16}) {
+        if (team_valid(s->index(), t)) teams.push_back(t);
+    }
+    std::vector<int32_t> wpbs = {1, 2, 4};
+    std::vector<int32_t> ids((size_t)nq * kTryK);
+    auto timeit = [&]() -> double {
+        auto t0 = std::chrono::high_resolution_clock::now();
+        int r = host_batch(*s, s->optimize_queries.data(), nq, nullptr, nullptr, kTryK, MODE_BATCH, ids.data(),
+                           nullptr, nullptr, false);
+        auto t1 = std::chrono::high_resolution_clock::now();
+        return r ? 1e30 : std::chrono::duration<double>(t1 - t0).count();
+    };
+    timeit();
+    double best = 1e30;
+    int32_t bt = s->team, bw = s->waves_per_block;
+    for (int32_t t : teams) {
+        for (int32_t w : wpbs) {
+            s->team = t;
+            s->waves_per_block = w;
+            timeit();
+            double e = std::min(timeit(), timeit());
+            if (e < best) {
+                best = e;
+                bt = t;
+                bw = w;
+            }
+        }
+    }
+    s->team = bt;
+    s->waves_per_block = bw;
+    return GLASS_HIP_OK;
+}
+
+double glass_hip_searcher_last_avg_dist_cmps(const glass_hip_searcher *s) { return s ? s->last_avg_dist_cmps : 0.0; }
+
+void glass_hip_searcher_free(glass_hip_searcher *s) { delete s; }
+
+int glass_hip_searcher_set_launch(glass_hip_searcher *s, int32_t team, int32_t waves_per_block, int32_t blocks) {
+    if (!s) return fail(GLASS_HIP_EINVAL, "null searcher");
+    if (waves_per_block < 1 || waves_per_block > 4 || blocks < 0) return fail(GLASS_HIP_EINVAL, "bad launch config");
+    if (s->has_data && !team_valid(s->index(), team)) return fail(GLASS_HIP_EINVAL, "team %d invalid for this layout", team);
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->team = team;
+    s->waves_per_block = waves_per_block;
+    s->blocks = blocks;
+    return GLASS_HIP_OK;
+}
+
+int glass_hip_searcher_get_launch(const glass_hip_searcher *s, int32_t *team, int32_t *waves_per_block,
+                                  int32_t *blocks, int32_t *slots) {
+    if (!s) return fail(GLASS_HIP_EINVAL, "null searcher");
+    if (team) *team = s->team;
+    if (waves_per_block) *waves_per_block = s->waves_per_block;
+    if (blocks) *blocks = s->blocks;
+    if (slots) *slots = s->nslots;
+    return GLASS_HIP_OK;
+}
+
+int glass_hip_searcher_info(const glass_hip_searcher *s, int32_t *code_size, int32_t *rerank_row_bytes,
+                            int32_t *degree, int32_t *n, float *refine_factor) {
+    if (!s) return fail(GLASS_HIP_EINVAL, "null searcher");
+    if (code_size) *code_size = s->code_size;
+    if (rerank_row_bytes) *rerank_row_bytes = s->refine == QK_FP32 ? 4 * s->r_dim_align : 0;
+    if (degree) *degree = s->graph.K;
+    if (n) *n = s->graph.N;
+    if (refine_factor) *refine_factor = s->refine == QK_FP32 ? refine_factor_of(s->quant) : 1.0f;
+    return GLASS_HIP_OK;
+}
+
+int glass_hip_searcher_get_codes(const glass_hip_searcher *s, uint8_t *codes) {
+    if (!s || !codes) return fail(GLASS_HIP_EINVAL, "null argument");
+    if (!s->has_data) return fail(GLASS_HIP_ESTATE, "no data");
+    HIP_TRY(hipMemcpy(codes, s->d_codes.p, (size_t)s->nb * s->code_size, hipMemcpyDeviceToHost));
+    return GLASS_HIP_OK;
+}
+
+int glass_hip_searcher_get_calibration(const glass_hip_searcher *s, float *params, int32_t *n) {
+    if (!s || !n) return fail(GLASS_HIP_EINVAL, "null argument");
+    if (s->quant == QK_SQ8) {
+        *n = 2 * s->dim_align;
+        if (params) {
+            std::copy(s->sq8_mins.begin(), s->sq8_mins.end(), params);
+            std::copy(s->sq8_difs.begin(), s->sq8_difs.end(), params + s->dim_align);
+        }
+    } else if (s->quant == QK_SQ8U && s->metric == MK_IP) {
+        *n = 1;
+        if (params) params[0] = s->qmax;
+    } else if (s->quant == QK_SQ8U || s->quant == QK_SQ4U || s->quant == QK_SQ2U) {
+        *n = 2;
+        if (params) {
+            params[0] = s->qmin;
+            params[1] = s->qdif;
+        }
+    } else {
+        *n = 0;
+    }
+    return GLASS_HIP_OK;
+}
+
+int glass_hip_searcher_sync(glass_hip_searcher *s) {
+    if (!s) return fail(GLASS_HIP_EINVAL, "null searcher");
+    if (s->stream) HIP_TRY(hipStreamSynchronize(s->stream));
+    return GLASS_HIP_OK;
+}
+
+}  // extern "C"

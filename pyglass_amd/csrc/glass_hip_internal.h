@@ -1,0 +1,85 @@
+// Internal interface between the host runtime (glass_hip.cpp) and the HIP
+// kernels (search_kernels.hip).  Not part of the public C ABI.
+#pragma once
+
+#include <hip/hip_fp16.h>
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace glass_hip {
+
+// Quantizer kinds; the numbering follows glass/quant/quant.hpp:23 (QuantizerType).
+enum QuantKind : int32_t { QK_FP32 = 0, QK_FP16 = 1, QK_BF16 = 2, QK_E5M2 = 3, QK_SQ8U = 4, QK_SQ8 = 5,
+                           QK_SQ4U = 7, QK_SQ2U = 11, QK_NONE = -1 };
+enum MetricKind : int32_t { MK_L2 = 0, MK_IP = 1 };
+
+// Search-mode semantics of one query (glass/searcher/graph_searcher.hpp):
+// MODE_SEARCH   = GraphSearcher::Search      pool (max(k,ef), max(k,ef))  (:134)
+// MODE_BATCH    = GraphSearcher::SearchBatch pool (ef, max(k,ef))         (:152)
+enum SearchMode : int32_t { MODE_SEARCH = 0, MODE_BATCH = 1 };
+
+struct DeviceIndex {
+    // graph (glass/graph.hpp): N x K int32, -1 padded; rows deduplicated
+    const int32_t *adj;
+    int32_t N, K;
+    // entry: HNSW upper layers (glass/hnsw/HNSWInitializer.hpp) or eps list
+    int32_t has_init, up_K, ep, nep;
+    const int32_t *levels;   // N
+    const int64_t *up_off;   // N (offset into up_lists, valid when levels[u] > 0)
+    const int32_t *up_lists; // concatenated lists, level l at (l-1)*up_K
+    const int32_t *eps;      // nep
+    // search quantizer
+    int32_t quant, metric, dim, dim_align, code_size;
+    const uint8_t *codes;    // N x code_size
+    float qmin, qdif, qmax;  // affine / symmetric calibration
+    const float *sq8_mins;   // dim_align (SQ8 per-dim)
+    const float *sq8_difs;
+    // FP32 re-rank rows (Refiner<FP32Quantizer>), or null
+    const float *rerank;
+    int32_t r_dim_align;
+    float refine_factor;     // get_refine_factor(quantizer)
+};
+
+struct QueryBatch {
+    const float *queries;    // nq x dim, fp32, device
+    int32_t nq;
+    int32_t dim;
+    const int32_t *ef;       // per-query ef (device) or null -> ef_uniform
+    const int32_t *k;        // per-query k (device) or null -> k_uniform
+    int32_t ef_uniform, k_uniform;
+    int32_t mode;            // SearchMode
+    const int64_t *out_off;  // per-query output offset (device) or null -> q * k_uniform
+    int32_t *out_ids;
+    float *out_dists;        // may be null
+    int32_t *out_counters;   // nq x 3: dist_cmps, expansions, upper evals; may be null
+};
+
+struct LaunchConfig {
+    int32_t team;            // lanes per candidate (integer quantizers)
+    int32_t waves_per_block;
+    int32_t blocks;          // persistent grid size
+    int32_t cap_max;         // LDS pool capacity per wave (>= every query's max(k',ef))
+    int32_t spec_gather;     // reserved
+};
+
+struct SlotState {
+    uint32_t *visited;       // nslots x vis_stride words
+    int64_t vis_stride;      // words per slot (multiple of 4)
+    int32_t nslots;
+    int32_t *work_counter;   // device counter for the persistent queue
+};
+
+// Launches the persistent search kernel for one batch on `stream`.
+// Returns hipSuccess or the launch error.
+hipError_t launch_search(const DeviceIndex &ix, const QueryBatch &qb, const LaunchConfig &lc, const SlotState &ss,
+                         hipStream_t stream);
+
+// Per-wave dynamic LDS bytes for a configuration.
+size_t lds_bytes_per_wave(const DeviceIndex &ix, int32_t cap_max);
+
+// Valid team widths for the index's code layout (powers of two dividing
+// code_size/4 with at most 16 dwords per lane).
+int32_t default_team(const DeviceIndex &ix);
+bool team_valid(const DeviceIndex &ix, int32_t team);
+
+}  // namespace glass_hip

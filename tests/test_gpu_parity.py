@@ -1,0 +1,106 @@
+"""HIP search vs the oracle, through the C ABI, on the golden graphs.
+
+Bar: ids bit-exact; distances bit-exact (integer quantizers, E5M2, SQ8 and the
+FP32 re-rank are all reproduced in the reference's evaluation order, so the
+1e-5 relative tolerance the FP32 re-rank is allowed is not needed); per-query
+distance-evaluation counts equal the oracle's (dist_cmps semantics,
+glass/quant/computer.hpp:67-72).
+"""
+
+import os
+
+import numpy as np
+import pytest
+
+import pyglass_amd as pg
+from oracle import oracle as O
+from pyglass_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+QUANTS = ["SQ8U", "SQ4U", "SQ2U", "E5M2", "SQ8", "FP32"]
+
+
+def _datasets():
+    X = synth.gaussian(10000, 128, synth.BASE_SEED)
+    Q = synth.gaussian(1000, 128, synth.QUERY_SEED)
+    Xn = (X[:5000] / np.linalg.norm(X[:5000], axis=1, keepdims=True)).astype(np.float32)
+    Qn = (Q / np.linalg.norm(Q, axis=1, keepdims=True)).astype(np.float32)
+    return {"c1_hnsw_l2": (X, Q, "L2"), "ip_hnsw": (Xn, Qn, "IP"), "nsg_l2": (X[:5000].copy(), Q, "L2")}
+
+
+@pytest.fixture(scope="module")
+def dsets():
+    return _datasets()
+
+
+@pytest.fixture(scope="module", autouse=True)
+def need_gpu():
+    if pg.device_count() < 1:
+        pytest.skip("no HIP device")
+
+
+@pytest.mark.parametrize("metric", ["L2", "IP"])
+@pytest.mark.parametrize("quant", QUANTS)
+def test_device_codes_match_oracle(dsets, golden_dir, quant, metric):
+    X, _, _ = dsets["c1_hnsw_l2"]
+    print(f"codes {quant} {metric}", flush=True)
+    g = pg.Graph(os.path.join(golden_dir, "c1_hnsw_l2.glass"))
+    s = pg.Searcher(g, X, metric, quant)
+    oq = O.Quant(quant, metric, X.shape[1]).fit(X)
+    np.testing.assert_array_equal(s.codes(), oq.codes)
+    cal = s.calibration()
+    ref = oq.params()
+    if quant == "SQ8":
+        np.testing.assert_array_equal(cal, ref[: 2 * oq.dim_align])
+    elif quant == "SQ8U" and metric == "IP":
+        np.testing.assert_array_equal(cal, ref[:1])
+    elif quant in ("SQ8U", "SQ4U", "SQ2U"):
+        np.testing.assert_array_equal(cal, ref[:2])
+
+
+CASES = []
+for gname in ["c1_hnsw_l2", "ip_hnsw", "nsg_l2"]:
+    for q in QUANTS:
+        for r in [""] + (["FP32"] if q in ("SQ8U", "SQ4U", "SQ2U", "SQ8") else []):
+            CASES.append((gname, q, r))
+
+
+@pytest.mark.parametrize("gname,quant,refine", CASES)
+def test_search_matches_oracle(dsets, golden_dir, gname, quant, refine):
+    XX, QQ, metric = dsets[gname]
+    path = os.path.join(golden_dir, f"{gname}.glass")
+    s = pg.Searcher(pg.Graph(path), XX, metric, quant, refine)
+    o = O.Searcher(O.Graph(path), XX, metric, quant, refine)
+    nq = 200
+    for ef in [16, 40, 96]:
+        for k in [1, 10, 50]:
+            print(f"{gname} {quant} {refine or 'none'} ef={ef} k={k}", flush=True)
+            s.set_ef(ef)
+            o.set_ef(ef)
+            I, D = s.batch_search(QQ[:nq], k)
+            Io, Do = o.batch_search(QQ[:nq], k)
+            np.testing.assert_array_equal(I, Io)
+            np.testing.assert_array_equal(D, Do)
+            assert s.get_last_search_dist_cmps() == o.get_last_search_dist_cmps()
+            for i in range(0, nq, 40):
+                a, b = s.search(QQ[i], k)
+                c, d = o.search(QQ[i], k)
+                np.testing.assert_array_equal(a, c)
+                np.testing.assert_array_equal(b, d)
+
+
+def test_golden_reference_ids_directly(dsets, golden_dir):
+    """HIP ids equal the compiled reference's stored ids (not only the oracle's)."""
+    z = np.load(os.path.join(golden_dir, "search.npz"))
+    XX, QQ, metric = dsets["c1_hnsw_l2"]
+    for quant, refine in [("SQ8U", "none"), ("SQ4U", "FP32"), ("SQ2U", "FP32")]:
+        s = pg.Searcher(pg.Graph(os.path.join(golden_dir, "c1_hnsw_l2.glass")), XX, metric, quant,
+                        "" if refine == "none" else refine)
+        for ef in [16, 40, 96]:
+            for k in [1, 10, 50]:
+                key = f"c1_hnsw_l2/{quant}/{refine}/{ef}/{k}"
+                s.set_ef(ef)
+                I, D = s.batch_search(QQ[:64], k)
+                np.testing.assert_array_equal(I, z[f"{key}/batch_ids"], err_msg=key)
+                np.testing.assert_array_equal(D, z[f"{key}/batch_dists"], err_msg=key)
