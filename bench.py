@@ -1,0 +1,425 @@
+"""Benchmark: QPS at recall@10 = 0.95 for query-time graph search on one MI355X.
+
+Workload (1M x 128): base rows from a seeded mixture of 1,024 isotropic
+Gaussians (centers U[0,255]^128, sigma 20, clipped to [0,255], rounded),
+10,000 queries from the same mixture, HNSW graph R=32 L=200 built on the CPU,
+L2 metric, SQ4U codes for traversal plus FP32 re-rank, ef swept 16-256, k=10,
+the whole query set in one batch.
+
+A step is one pass of the search over the 10,000 queries with the queries
+already resident in HBM: one persistent launch of the search kernel.  The ef
+sweep times W warm-up + K timed steps per ef; recall@10 is measured against
+an exact brute-force kNN computed on the GPU (checked in FP64 on a sample),
+and "QPS at recall 0.95" is log-QPS interpolated between the two ef points
+that bracket 0.95.  Beside it: the plain form (one query per launch, host
+buffers, GraphSearcher::Search semantics) and the reference CPU Searcher,
+compiled from the reference sources, timed on the host's cores on the same
+graph and data.
+
+Usage: python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu] [--small]
+Prints one JSON line.
+"""
+
+from __future__ import annotations
+
+import argparse
+import hashlib
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+EFS = [16, 24, 32, 48, 64, 96, 128, 192, 256]
+FACTORS = [4.0, 8.0]
+TARGET = 0.95
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def recall_at(ids: np.ndarray, gt: np.ndarray, k: int) -> float:
+    """examples/main.py:181-185: sum |res_i intersect gt_i| / (nq k)."""
+    hit = 0
+    for a, b in zip(ids[:, :k], gt[:, :k]):
+        hit += np.intersect1d(a, b).size
+    return hit / (ids.shape[0] * k)
+
+
+def qps_at_recall(points, target=TARGET):
+    """points: [(recall, qps)] by increasing ef.  Log-QPS interpolated between
+    the two points bracketing `target`; None when the sweep never reaches it."""
+    pts = sorted(points)
+    for (r0, q0), (r1, q1) in zip(pts, pts[1:]):
+        if r0 <= target <= r1 and r1 > r0:
+            t = (target - r0) / (r1 - r0)
+            return math.exp(math.log(q0) + t * (math.log(q1) - math.log(q0)))
+    for r, q in pts:
+        if abs(r - target) < 1e-12:
+            return q
+    return None
+
+
+def exact_knn_gpu(X: np.ndarray, Q: np.ndarray, k: int, device):
+    """Exact L2 kNN with torch on the GPU (fp32 GEMM, no TF32), ties by id."""
+    import torch
+
+    torch.backends.cuda.matmul.allow_tf32 = False
+    xb = torch.from_numpy(X).to(device)
+    xn = (xb * xb).sum(1)
+    out = np.empty((Q.shape[0], k), np.int64)
+    for s in range(0, Q.shape[0], 1024):
+        q = torch.from_numpy(Q[s:s + 1024]).to(device)
+        d = (q * q).sum(1, keepdim=True) - 2.0 * (q @ xb.T) + xn[None, :]
+        # stable ordering by (distance, id): take a margin, then sort on the host
+        cand = torch.topk(d, k + 16, dim=1, largest=False).indices.cpu().numpy()
+        for i in range(cand.shape[0]):
+            qi = Q[s + i].astype(np.float64)
+            dd = ((X[cand[i]].astype(np.float64) - qi) ** 2).sum(1)
+            order = np.lexsort((cand[i], dd))
+            out[s + i] = cand[i][order[:k]]
+    del xb, xn
+    torch.cuda.empty_cache()
+    return out
+
+
+def fp64_check(X, Q, gt, k, device, n=100, seed=0):
+    """Re-check a random sample of GT rows by a full FP64 scan on the GPU."""
+    import torch
+
+    rng = np.random.default_rng(seed)
+    idx = rng.choice(Q.shape[0], size=min(n, Q.shape[0]), replace=False)
+    xb = torch.from_numpy(X).to(device, dtype=torch.float64)
+    q = torch.from_numpy(Q[idx]).to(device, dtype=torch.float64)
+    d = ((q * q).sum(1, keepdim=True) - 2.0 * (q @ xb.T) + (xb * xb).sum(1)[None, :]).cpu().numpy()
+    del xb, q
+    torch.cuda.empty_cache()
+    bad = 0
+    for j, i in enumerate(idx):
+        top = np.lexsort((np.arange(d.shape[1]), d[j]))[:k]
+        bad += int(not np.array_equal(np.sort(top), np.sort(gt[i, :k])))
+    return int(len(idx)), bad
+
+
+def load_or_build_graph(X, path, R=32, L=200):
+    import pyglass_amd as pg
+
+    meta = path + ".json"
+    digest = hashlib.sha256(X[:: max(1, X.shape[0] // 4096)].tobytes()).hexdigest()
+    if os.path.exists(path) and os.path.exists(meta):
+        m = json.load(open(meta))
+        if m.get("digest") == digest and m.get("R") == R and m.get("L") == L:
+            log(f"graph: cached {path} (built in {m.get('seconds', 0):.1f}s by {m.get('builder')})")
+            return m
+    log(f"graph: building HNSW R={R} L={L} on {os.cpu_count()} cores ...")
+    os.makedirs(os.path.dirname(path), exist_ok=True)
+    t = time.time()
+    secs = pg.build_graph("HNSW", X, path, "L2", "FP32", R, L)
+    m = {"digest": digest, "R": R, "L": L, "seconds": secs, "builder": "pyglass_amd HNSW (CPU)",
+         "wall": time.time() - t}
+    os.makedirs(os.path.dirname(path), exist_ok=True)
+    json.dump(m, open(meta, "w"))
+    log(f"graph: built in {secs:.1f}s")
+    return m
+
+
+def time_batched(searcher, q_dev, nq, k, ids_dev, d_dev, cnt_dev, steps, warmup, torch, stream):
+    """W warm-up + K timed steps on `stream`; the kernel time comes from HIP
+    events recorded on that same stream around each launch."""
+    sp = stream.cuda_stream
+    for _ in range(warmup):
+        searcher.search_batch_dev(q_dev.data_ptr(), nq, k, ids_dev.data_ptr(), d_dev.data_ptr(), cnt_dev.data_ptr(),
+                                  stream=sp)
+    torch.cuda.synchronize()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
+    t0 = time.perf_counter()
+    for i in range(steps):
+        ev[i][0].record(stream)
+        searcher.search_batch_dev(q_dev.data_ptr(), nq, k, ids_dev.data_ptr(), d_dev.data_ptr(), cnt_dev.data_ptr(),
+                                  stream=sp)
+        ev[i][1].record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    kern_ms = [a.elapsed_time(b) for a, b in ev]
+    return wall, kern_ms
+
+
+def gpu_sweep(s, q_dev, nq, k, gt, ids_dev, d_dev, cnt_dev, args, torch, stream, factor, efs):
+    s.set_refine_factor(factor)
+    pts = []
+    for ef in efs:
+        s.set_ef(ef)
+        wall, kern_ms = time_batched(s, q_dev, nq, k, ids_dev, d_dev, cnt_dev, args.steps, args.warmup, torch, stream)
+        ids = ids_dev.cpu().numpy()
+        cnt = cnt_dev.cpu().numpy().astype(np.int64)
+        r = recall_at(ids, gt, k)
+        ms = wall / args.steps * 1e3
+        pts.append({"factor": factor, "ef": ef, "recall": r, "ms_per_step": ms, "qps": nq / (ms / 1e3),
+                    "kernel_ms": float(np.mean(kern_ms)), "dist_cmps": float(cnt[:, 0].mean()),
+                    "expansions": float(cnt[:, 1].mean()), "upper": float(cnt[:, 2].mean()),
+                    "ids": ids, "cnt": cnt})
+        log(f"gpu: factor={factor} ef={ef} recall={r:.4f} qps={nq / (ms / 1e3):.0f} "
+            f"kernel={np.mean(kern_ms):.3f}ms dist_cmps={cnt[:, 0].mean():.0f} exp={cnt[:, 1].mean():.1f}")
+        if r >= 0.99:
+            break
+    return pts
+
+
+class stdout_to_stderr:
+    """Route C-level stdout (the reference's printf progress lines) to stderr
+    so stdout carries only the JSON result."""
+
+    def __enter__(self):
+        sys.stdout.flush()
+        self.saved = os.dup(1)
+        os.dup2(2, 1)
+
+    def __exit__(self, *exc):
+        import ctypes
+
+        sys.stdout.flush()
+        ctypes.CDLL(None).fflush(None)  # C stdio buffers still point at fd 1
+        os.dup2(self.saved, 1)
+        os.close(self.saved)
+
+
+def cpu_baseline(graph_path, X, Q, gt, k, efs, factor, budget_s=12.0):
+    """The reference CPU Searcher (oracle/_ref, compiled from the reference
+    sources with AVX-512) on all host cores: optimize() first, then per ef
+    the best of up to 5 batch_search runs over a bounded query sample
+    (examples/main.py:164-187)."""
+    from oracle import refwrap as R
+
+    if not R.available(portable=True):
+        return None
+    nthreads = os.cpu_count() or 1
+    R.set_num_threads(nthreads, portable=True)
+    t = time.time()
+    s = R.RefSearcher(graph_path, X, "L2", "SQ4U", "FP32", portable=True, refine_factor=factor)
+    s.optimize()
+    log(f"cpu: reference searcher ready in {time.time() - t:.1f}s ({nthreads} threads)")
+    nq = Q.shape[0]
+    pts = []
+    res_ids = {}
+    spent = 0.0
+    for ef in efs:
+        s.set_ef(ef)
+        best = None
+        for run in range(5):
+            t0 = time.perf_counter()
+            ids, _ = s.batch_search(Q, k)
+            dt = time.perf_counter() - t0
+            spent += dt
+            best = dt if best is None else min(best, dt)
+            if run == 0:
+                res_ids[ef] = ids
+            if spent > budget_s:
+                break
+        r = recall_at(res_ids[ef], gt, k)
+        pts.append((r, nq / best))
+        log(f"cpu: factor={factor} ef={ef} recall={r:.4f} qps={nq / best:.0f}")
+        if r >= 0.99 or spent > budget_s:
+            break
+    return {"points": pts, "threads": nthreads, "ids": res_ids}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--small", action="store_true", help="100k-row version for quick checks")
+    ap.add_argument("--plain", type=int, default=200, help="queries timed in the plain one-query form")
+    ap.add_argument("--calib-copy", action="store_true", help="with --point: also run a 512 MiB device copy")
+    ap.add_argument("--point", type=str, default=None,
+                    help="FACTOR:EF - time only this operating point (for profiler runs)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build measures one GPU only; --gpus must be 1 "
+                                   "(queries are independent: multi-GPU is query sharding, see DESIGN.md)"}))
+        sys.exit(2)
+
+    import torch
+
+    import pyglass_amd as pg
+    from pyglass_amd import synth
+
+    dev = torch.device("cuda:0")
+    n = 100_000 if args.small else 1_000_000
+    nq = 10_000
+    k = 10
+    t = time.time()
+    X = synth.uint8_mixture(n, 128, synth.BASE_SEED)
+    Q = synth.uint8_mixture(nq, 128, synth.QUERY_SEED)
+    log(f"data: {n}x128 base, {nq} queries in {time.time() - t:.1f}s")
+    gpath = os.path.join(ROOT, "data", f"mix{n // 1000}k_hnsw_R32_L200.glass")
+    gmeta = load_or_build_graph(X, gpath)
+
+    t = time.time()
+    gt = exact_knn_gpu(X, Q, k, dev)
+    nchk, nbad = fp64_check(X, Q, gt, k, dev, n=100)
+    log(f"gt: exact kNN in {time.time() - t:.1f}s, fp64 recheck {nbad}/{nchk} mismatches")
+
+    t = time.time()
+    s = pg.Searcher(pg.Graph(gpath), X, "L2", "SQ4U", "FP32")
+    s.optimize()
+    launch = s.get_launch()
+    info = s.info()
+    log(f"searcher: set_data+optimize {time.time() - t:.1f}s, launch {launch}, info {info}")
+
+    q_dev = torch.from_numpy(Q).to(dev)
+    ids_dev = torch.empty((nq, k), dtype=torch.int32, device=dev)
+    d_dev = torch.empty((nq, k), dtype=torch.float32, device=dev)
+    cnt_dev = torch.empty((nq, 3), dtype=torch.int32, device=dev)
+
+    stream = torch.cuda.Stream(device=dev)
+    # the reference's default re-rank depth (factor 1.5 for SQ4U) plateaus
+    # below recall 0.95 on this workload; factors 4 and 8 are swept for the
+    # headline (same knob as Refiner's constructor argument)
+    if args.point:
+        pf, pe = args.point.split(":")
+        pts = gpu_sweep(s, q_dev, nq, k, gt, ids_dev, d_dev, cnt_dev, args, torch, stream, float(pf), [int(pe)])
+        p = pts[0]
+        res = {kk: v for kk, v in p.items() if kk not in ("ids", "cnt")}
+        if args.calib_copy:
+            # a plain device copy of known size, to scale the HBM byte counters
+            nbytes = 512 << 20
+            a = torch.empty(nbytes // 4, dtype=torch.float32, device=dev).fill_(1.0)
+            b = torch.empty_like(a)
+            torch.cuda.synchronize()
+            for _ in range(3):
+                b.copy_(a)
+            torch.cuda.synchronize()
+            res["calib_copy_bytes"] = nbytes
+        s_code, r_bytes, R = info["code_size"], info["rerank_row_bytes"], info["degree"]
+        cnt = p["cnt"]
+        bq = cnt[:, 0] * s_code + cnt[:, 1] * 4 * R + cnt[:, 2] * 4 + int(k * float(pf)) * r_bytes + 4 * 128 + 8 * k
+        res["algorithmic_bytes_per_launch"] = float(bq.sum())
+        print(json.dumps(res))
+        return
+    default_curve = gpu_sweep(s, q_dev, nq, k, gt, ids_dev, d_dev, cnt_dev, args, torch, stream, 1.5,
+                              [32, 128, 256])
+    sweeps = {}
+    for f in FACTORS:
+        sweeps[f] = gpu_sweep(s, q_dev, nq, k, gt, ids_dev, d_dev, cnt_dev, args, torch, stream, f, EFS)
+    best_f, value = None, None
+    for f, pts in sweeps.items():
+        v = qps_at_recall([(p["recall"], p["qps"]) for p in pts])
+        if v is not None and (value is None or v > value):
+            best_f, value = f, v
+    if best_f is None:
+        best_f = FACTORS[-1]
+    sweep = sweeps[best_f]
+    op = next((p for p in sweep if p["recall"] >= TARGET), sweep[-1])
+    s.set_refine_factor(best_f)
+
+    # roofline of the search kernel at the operating point: algorithmic bytes
+    # B_q = C_q*s_code + E_q*4R + U_q*4 + r_q*4*d_f + 4d + 8k
+    s_code, r_bytes, R = info["code_size"], info["rerank_row_bytes"], info["degree"]
+    rq = int(k * best_f)
+    cnt = op["cnt"]
+    bq = cnt[:, 0] * s_code + cnt[:, 1] * 4 * R + cnt[:, 2] * 4 + rq * r_bytes + 4 * 128 + 8 * k
+    alg_bytes = float(bq.sum())
+    achieved = alg_bytes / (op["kernel_ms"] / 1e3) / 1e9
+
+    # plain form: one query per launch through the host API
+    s.set_ef(op["ef"])
+    npl = min(args.plain, nq)
+    for i in range(5):
+        s.search(Q[i], k)
+    tt = 0.0
+    plain_ids = np.empty((npl, k), np.int32)
+    for i in range(npl):
+        t0 = time.perf_counter()
+        plain_ids[i], _ = s.search(Q[i], k)
+        tt += time.perf_counter() - t0
+    plain_qps = npl / tt
+    plain_recall = recall_at(plain_ids, gt[:npl], k)
+    log(f"plain: {npl} queries, factor={best_f} ef={op['ef']}: {plain_qps:.0f} QPS, recall {plain_recall:.4f}")
+
+    cpu = None
+    parity = None
+    cpu_value, cpu_f = None, None
+    if not args.no_cpu:
+        cpu = {}
+        for f in FACTORS:
+            with stdout_to_stderr():
+                c = cpu_baseline(gpath, X, Q, gt, k, [p["ef"] for p in sweeps[f]], f)
+            if c is None:
+                cpu = None
+                break
+            cpu[f] = c
+            v = qps_at_recall(c["points"])
+            if v is not None and (cpu_value is None or v > cpu_value):
+                cpu_value, cpu_f = v, f
+        if cpu:
+            eq = [float(np.mean(np.all(cpu[f]["ids"][p["ef"]] == p["ids"], axis=1)))
+                  for f in FACTORS for p in sweeps[f] if p["ef"] in cpu[f]["ids"]]
+            parity = min(eq) if eq else None
+
+    out = {
+        "metric": "QPS at recall@10=0.95 (batched GPU search, 1M x 128 SQ4U+FP32 rerank)",
+        "value": value,
+        "unit": "queries/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": op["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u4 codes, int32 distances; fp32 rerank",
+        "data": "synthetic (seeded Gaussian mixture; no downloads)",
+        "config": {
+            "workload": f"{n // 1000}k x 128 uint8-valued Gaussian mixture, HNSW R=32 L=200, L2, SQ4U + FP32 "
+                        f"rerank, k=10, {nq} queries per step",
+            "refine_factor": best_f,
+            "operating_ef": op["ef"],
+            "operating_recall": op["recall"],
+            "ef_sweep": {str(f): [{kk: v for kk, v in p.items() if kk not in ("ids", "cnt")} for p in pts]
+                         for f, pts in sweeps.items()},
+            "reference_default_factor_1_5": [{kk: v for kk, v in p.items() if kk in ("ef", "recall", "qps")}
+                                             for p in default_curve],
+            "launch": launch,
+            "graph_build_s": gmeta.get("seconds"),
+            "plain_one_query_per_launch": {"qps": plain_qps, "recall": plain_recall, "queries": npl,
+                                           "ef": op["ef"]},
+            "batched_over_plain": (value / plain_qps) if value else None,
+            "batched_over_cpu": (value / cpu_value) if (value and cpu_value) else None,
+            "ids_equal_cpu_reference": parity,
+            "gt_fp64_recheck": {"checked": nchk, "mismatches": nbad},
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None,
+            "algorithmic_bytes_per_launch": alg_bytes,
+            "kernel_ms": op["kernel_ms"],
+        },
+        "cpu_baseline": None if not cpu else {
+            "value": cpu_value,
+            "unit": "queries/s",
+            "cores": cpu[FACTORS[0]]["threads"],
+            "kind": "reference",
+            "sample": f"{nq} queries per ef, ef swept until recall 0.99 or ~12 s per factor, best of <=5 "
+                      f"batch_search runs each, reference Searcher (SQ4U + FP32 Refiner, factors {FACTORS}) after "
+                      f"optimize(); best factor {cpu_f}",
+            "points": {str(f): c["points"] for f, c in cpu.items()},
+        },
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

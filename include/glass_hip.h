@@ -56,6 +56,13 @@ int glass_hip_graph_info(const glass_hip_graph *g, int32_t *n, int32_t *k, int32
 /* Copy adjacency (n*k) / entry points out. */
 int glass_hip_graph_copy(const glass_hip_graph *g, int32_t *adj, int32_t *eps);
 void glass_hip_graph_free(glass_hip_graph *g);
+/* Index(index_type, metric, R, L).build(data) / build_graph
+ * (python/bindings.cc:60-88,231-237): CPU graph construction.  "HNSW"
+ * restates glass/hnsw/hnsw.hpp:27-69 (M = R/2, efConstruction = L) with
+ * fp32 distances; num_threads <= 0 uses every core.  *seconds (nullable)
+ * receives the construction time. */
+int glass_hip_build_graph(const char *index_type, const float *data, int32_t n, int32_t dim, const char *metric,
+                          int32_t R, int32_t L, int32_t num_threads, glass_hip_graph **out, double *seconds);
 
 /* ---- Searcher (GraphSearcherBase) -------------------------------------- */
 
@@ -98,6 +105,11 @@ int glass_hip_searcher_search_batch_efk(glass_hip_searcher *s, const float *qs, 
 int glass_hip_searcher_search_batch_dev(glass_hip_searcher *s, const float *qs_dev, int32_t nq, int32_t k,
                                         int32_t mode, int32_t *dst_dev, float *scores_dev, int32_t *counters_dev,
                                         void *stream);
+/* Re-rank depth: reorder_k = int(k * factor).  0 restores the reference's
+ * get_refine_factor(quantizer) (graph_searcher.hpp:227-238: 1.5 for
+ * SQ8U/SQ8/SQ4U, 3.0 for SQ2U), which Refiner takes as its constructor
+ * argument (refiner.hpp:22-24). */
+int glass_hip_searcher_set_refine_factor(glass_hip_searcher *s, float factor);
 /* Launch configuration (set by optimize): team = dwords per lane (4, 8, 16)
  * for integer codes, waves per block, blocks of the persistent grid (0 =
  * fill the device).  Query with _get. */

@@ -179,13 +179,16 @@ class Graph:
 class Searcher:
     """Oracle counterpart of glass.Searcher (python/bindings.cc:90-154)."""
 
-    def __init__(self, graph: Graph, data: np.ndarray, metric: str, quantizer: str, refine_quant: str = ""):
+    def __init__(self, graph: Graph, data: np.ndarray, metric: str, quantizer: str, refine_quant: str = "",
+                 refine_factor: float | None = None):
         self.L = lib()
         self.graph = graph
         self.dim = data.shape[1]
         self.q = Quant(quantizer, metric, self.dim).fit(data)
         self.rq = Quant(refine_quant, metric, self.dim).fit(data) if refine_quant else None
         self.factor = self.L.orq_refine_factor(KINDS[quantizer]) if refine_quant else 1.0
+        if refine_quant and refine_factor:
+            self.factor = float(refine_factor)
         self.ef = 32
         self.last_dist_cmps = 0.0
 

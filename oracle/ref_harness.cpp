@@ -124,6 +124,30 @@ void *ref_searcher_new(const char *graph_path, const float *data, int n, int d, 
     return h;
 }
 
+// Same searcher with the re-rank depth multiplier chosen by the caller: the
+// reference's make_refiner (glass/searcher/graph_searcher.hpp:240-254) takes
+// it as an argument; create_searcher passes get_refine_factor(quantizer).
+void *ref_searcher_new_factor(const char *graph_path, const float *data, int n, int d, const char *metric,
+                              const char *quant, const char *refine, float factor) {
+    glass::Graph<int> g;
+    g.load(graph_path);
+    auto *h = new RefSearcher;
+    h->s = glass::create_searcher(std::move(g), metric, quant, "");
+    if (!h->s) {
+        delete h;
+        return nullptr;
+    }
+    if (refine && refine[0]) {
+        if (std::string(metric) == "IP") {
+            h->s = glass::make_refiner<glass::Metric::IP>(std::move(h->s), refine, factor);
+        } else {
+            h->s = glass::make_refiner<glass::Metric::L2>(std::move(h->s), refine, factor);
+        }
+    }
+    h->s->SetData(data, n, d);
+    return h;
+}
+
 void ref_searcher_free(void *h) { delete (RefSearcher *)h; }
 
 void ref_set_ef(void *h, int ef) { ((RefSearcher *)h)->s->SetEf(ef); }

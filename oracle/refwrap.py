@@ -39,6 +39,9 @@ def load(portable: bool = False):
     lib.ref_build_graph.restype = C.c_double
     lib.ref_searcher_new.argtypes = [C.c_char_p, f32p, C.c_int, C.c_int, C.c_char_p, C.c_char_p, C.c_char_p]
     lib.ref_searcher_new.restype = C.c_void_p
+    lib.ref_searcher_new_factor.argtypes = [C.c_char_p, f32p, C.c_int, C.c_int, C.c_char_p, C.c_char_p, C.c_char_p,
+                                            C.c_float]
+    lib.ref_searcher_new_factor.restype = C.c_void_p
     lib.ref_searcher_free.argtypes = [C.c_void_p]
     lib.ref_set_ef.argtypes = [C.c_void_p, C.c_int]
     lib.ref_optimize.argtypes = [C.c_void_p, C.c_int]
@@ -72,12 +75,17 @@ class RefSearcher:
     """Mirror of glass.Searcher (python/bindings.cc:90-154) over the harness."""
 
     def __init__(self, graph_path: str, data: np.ndarray, metric: str, quantizer: str, refine_quant: str = "",
-                 portable: bool = False):
+                 portable: bool = False, refine_factor: float | None = None):
         self.lib = load(portable)
         data = np.ascontiguousarray(data, dtype=np.float32)
         self.dim = data.shape[1]
-        self.h = self.lib.ref_searcher_new(graph_path.encode(), data, data.shape[0], data.shape[1], metric.encode(),
-                                           quantizer.encode(), refine_quant.encode())
+        if refine_factor is None:
+            self.h = self.lib.ref_searcher_new(graph_path.encode(), data, data.shape[0], data.shape[1],
+                                               metric.encode(), quantizer.encode(), refine_quant.encode())
+        else:
+            self.h = self.lib.ref_searcher_new_factor(graph_path.encode(), data, data.shape[0], data.shape[1],
+                                                      metric.encode(), quantizer.encode(), refine_quant.encode(),
+                                                      float(refine_factor))
         if not self.h:
             raise RuntimeError("reference create_searcher returned null")
 

@@ -16,7 +16,7 @@ import numpy as np
 from . import _lib
 from ._lib import GlassError
 
-__all__ = ["Graph", "Searcher", "GlassError", "set_num_threads", "device_count", "version"]
+__all__ = ["Graph", "Searcher", "Index", "build_graph", "GlassError", "set_num_threads", "device_count", "version"]
 
 _NUM_THREADS = 0
 
@@ -108,6 +108,40 @@ class Graph:
     def _take(self):
         h, self._h = self._h, C.c_void_p()
         return h
+
+
+class Index:
+    """glass.Index(index_type, metric, quant="FP32", R=32, L=200)
+    (python/bindings.cc:60-88): CPU graph construction.  Only "HNSW" is
+    built here; distances are fp32 on the raw rows whatever `quant` says."""
+
+    def __init__(self, index_type: str, metric: str, quant: str = "FP32", R: int = 32, L: int = 200):
+        self.index_type, self.metric, self.quant, self.R, self.L = index_type, metric, quant, int(R), int(L)
+        self.construction_time = 0.0
+
+    def build(self, data) -> Graph:
+        x = _as_f32(data)
+        L = _lib.load()
+        h = C.c_void_p()
+        secs = C.c_double()
+        _lib.check(L.glass_hip_build_graph(self.index_type.encode(), _p(x), x.shape[0], x.shape[1],
+                                           self.metric.encode(), self.R, self.L or 200, _NUM_THREADS, C.byref(h),
+                                           C.byref(secs)))
+        self.construction_time = secs.value
+        g = Graph()
+        g._h = h
+        return g
+
+    def get_construction_time(self) -> float:
+        return self.construction_time
+
+
+def build_graph(index_type: str, input, graph_file: str, metric: str, quant: str = "FP32", R: int = 32,
+                L: int = 200) -> float:
+    """glass.build_graph (bindings.cc:231-237): build and save; returns seconds."""
+    idx = Index(index_type, metric, quant, R, L)
+    idx.build(input).save(graph_file)
+    return idx.get_construction_time()
 
 
 class Searcher:
@@ -205,6 +239,11 @@ class Searcher:
 
     def get_last_search_dist_cmps(self) -> float:
         return float(self._L.glass_hip_searcher_last_avg_dist_cmps(self._h))
+
+    def set_refine_factor(self, factor: float):
+        """Re-rank depth multiplier (reorder_k = int(k * factor)); 0 = the
+        reference's get_refine_factor(quantizer)."""
+        _lib.check(self._L.glass_hip_searcher_set_refine_factor(self._h, float(factor)))
 
     # ---- introspection -------------------------------------------------
     def set_launch(self, team: int, waves_per_block: int, blocks: int = 0):

@@ -28,6 +28,8 @@ SIGNATURES = {
     "glass_hip_graph_info": (C.c_int, [_vp, _pi32, _pi32, _pi32, _pi32]),
     "glass_hip_graph_copy": (C.c_int, [_vp, _vp, _vp]),
     "glass_hip_graph_free": (None, [_vp]),
+    "glass_hip_build_graph": (C.c_int, [C.c_char_p, _vp, _i32, _i32, C.c_char_p, _i32, _i32, _i32, C.POINTER(_vp),
+                                        C.POINTER(C.c_double)]),
     "glass_hip_searcher_create": (C.c_int, [_vp, C.c_char_p, C.c_char_p, C.c_char_p, C.POINTER(_vp)]),
     "glass_hip_searcher_set_data": (C.c_int, [_vp, _vp, _i32, _i32]),
     "glass_hip_searcher_set_ef": (C.c_int, [_vp, _i32]),
@@ -39,6 +41,7 @@ SIGNATURES = {
     "glass_hip_searcher_free": (None, [_vp]),
     "glass_hip_searcher_search_batch_efk": (C.c_int, [_vp, _vp, _i32, _vp, _vp, _vp, _vp, _vp]),
     "glass_hip_searcher_search_batch_dev": (C.c_int, [_vp, _vp, _i32, _i32, _i32, _vp, _vp, _vp, _vp]),
+    "glass_hip_searcher_set_refine_factor": (C.c_int, [_vp, C.c_float]),
     "glass_hip_searcher_set_launch": (C.c_int, [_vp, _i32, _i32, _i32]),
     "glass_hip_searcher_get_launch": (C.c_int, [_vp, _pi32, _pi32, _pi32, _pi32]),
     "glass_hip_searcher_info": (C.c_int, [_vp, _pi32, _pi32, _pi32, _pi32, _pf32]),

@@ -17,6 +17,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libglass_hip.so")
 
 SOURCES = ["search_kernels.hip", "encode_kernels.hip", "glass_hip.cpp"]
+HOST_SOURCES = ["graph_build.cpp"]
 ARCH = os.environ.get("GLASS_HIP_ARCH", "gfx950")
 
 HIPCC_FLAGS = [
@@ -54,6 +55,18 @@ def build(verbose: bool = False) -> str:
             raise RuntimeError(f"hipcc failed for {src}:\n{errs}")
         if verbose and errs.strip():
             print(errs)
+        objs.append(obj)
+    # CPU graph builder: plain C++ for the host (AVX2/FMA baseline shared by
+    # the build container and the GPU hosts)
+    for src in HOST_SOURCES:
+        obj = os.path.join(obj_dir, os.path.splitext(src)[0] + ".o")
+        cmd = ["g++", "-O3", "-std=c++17", "-fPIC", "-march=x86-64-v3", "-pthread", "-c", os.path.join(CSRC, src),
+               "-o", obj]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"g++ failed for {src}:\n{r.stderr}")
         objs.append(obj)
     tmp = LIB + ".tmp"
     cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs, "-lpthread"]

@@ -35,6 +35,12 @@ hipError_t launch_pad_rows(const float *data_dev, int64_t n, int32_t dim, int32_
 hipError_t launch_fill_i32(int32_t *p, int64_t n, int32_t v, hipStream_t stream);
 }  // namespace glass_hip
 
+namespace glass_build {
+int build_hnsw(const float *X, int32_t n, int32_t d, bool ip, int32_t R, int32_t L, int32_t num_threads,
+               std::vector<int32_t> &adj, int32_t &ep, std::vector<int32_t> &levels, std::vector<int32_t> &lists,
+               double *seconds);
+}  // namespace glass_build
+
 using namespace glass_hip;
 
 namespace {
@@ -182,6 +188,10 @@ struct glass_hip_searcher {
         if (stream) (void)hipStreamDestroy(stream);
     }
 
+    // re-rank depth multiplier: get_refine_factor(quantizer) unless overridden
+    float refine_override = 0.0f;
+    float factor() const { return refine_override > 0.0f ? refine_override : refine_factor_of(quant); }
+
     DeviceIndex index() const {
         DeviceIndex ix{};
         ix.adj = d_adj.p;
@@ -208,7 +218,7 @@ struct glass_hip_searcher {
         ix.sq8_difs = d_difs.p;
         ix.rerank = refine == QK_FP32 ? d_rerank.p : nullptr;
         ix.r_dim_align = refine == QK_FP32 ? r_dim_align : 0;
-        ix.refine_factor = refine == QK_FP32 ? refine_factor_of(quant) : 1.0f;
+        ix.refine_factor = refine == QK_FP32 ? factor() : 1.0f;
         return ix;
     }
 };
@@ -440,17 +450,14 @@ int ensure_stream(glass_hip_searcher &s) {
     return GLASS_HIP_OK;
 }
 
-// Persistent grid size: enough waves to fill the device at the kernel's
-// occupancy (bounded by LDS per wave), never more slots than queries.
-int32_t grid_blocks(const glass_hip_searcher &s, int32_t nq, int32_t cap_max) {
-    const size_t lds_wave = lds_bytes_per_wave(s.index(), cap_max);
-    const int32_t wpb = s.waves_per_block;
-    int32_t by_lds = (int32_t)((160 * 1024) / std::max<size_t>(1, lds_wave * wpb));
-    int32_t by_waves = 32 / wpb;  // 8 waves per SIMD x 4 SIMDs
-    int32_t per_cu = std::max(1, std::min(by_lds, by_waves));
+// Persistent grid size: as many blocks as can be resident at once (kernel
+// occupancy with this LDS footprint), never more slots than queries.
+int32_t grid_blocks(const glass_hip_searcher &s, const LaunchConfig &lc, int32_t nq) {
+    int32_t per_cu = max_blocks_per_cu(s.index(), lc);
+    if (per_cu <= 0) per_cu = 1;
     int32_t full = per_cu * std::max(1, s.cu_count);
     if (s.blocks > 0) full = std::min(full, s.blocks);
-    int32_t need = (nq + wpb - 1) / wpb;
+    int32_t need = (nq + lc.waves_per_block - 1) / lc.waves_per_block;
     return std::max(1, std::min(full, need));
 }
 
@@ -471,25 +478,19 @@ int32_t round_cap(int32_t cap) {
 // Enqueue one batch: queries/outputs are device pointers.
 int run_batch(glass_hip_searcher &s, const float *qs_dev, int32_t nq, const int32_t *ef_dev, const int32_t *k_dev,
               int32_t ef_uniform, int32_t k_uniform, int32_t max_cap, int32_t mode, const int64_t *off_dev,
-              int32_t *ids_dev, float *dists_dev, int32_t *counters_dev, hipStream_t stream, int32_t blocks_override,
-              int32_t wpb_override) {
+              int32_t *ids_dev, float *dists_dev, int32_t *counters_dev, hipStream_t stream) {
     if (!s.has_data) return fail(GLASS_HIP_ESTATE, "search before set_data");
     if (nq <= 0) return GLASS_HIP_OK;
     if (int r = ensure_stream(s)) return r;
     if (!stream) stream = s.stream;
-    const int32_t cap_max = round_cap(max_cap);
-    const int32_t saved_wpb = s.waves_per_block;
-    if (wpb_override > 0) s.waves_per_block = wpb_override;
-    int32_t blocks = blocks_override > 0 ? blocks_override : grid_blocks(s, nq, cap_max);
     LaunchConfig lc{};
-    lc.team = s.team;
+    lc.team = team_valid(s.index(), s.team) ? s.team : default_team(s.index());
     lc.waves_per_block = s.waves_per_block;
+    lc.cap_max = round_cap(max_cap);
+    const size_t lds = lds_bytes_per_wave(s.index(), lc.cap_max) * (size_t)lc.waves_per_block;
+    if (lds > 160 * 1024) return fail(GLASS_HIP_EINVAL, "pool capacity %d needs %zu bytes of LDS", lc.cap_max, lds);
+    const int32_t blocks = grid_blocks(s, lc, nq);
     lc.blocks = blocks;
-    lc.cap_max = cap_max;
-    s.waves_per_block = saved_wpb;
-    if (!team_valid(s.index(), lc.team)) lc.team = default_team(s.index());
-    const size_t lds = lds_bytes_per_wave(s.index(), cap_max) * (size_t)lc.waves_per_block;
-    if (lds > 160 * 1024) return fail(GLASS_HIP_EINVAL, "pool capacity %d needs %zu bytes of LDS", cap_max, lds);
     if (int r = ensure_slots(s, blocks * lc.waves_per_block)) return r;
     if (!s.d_counter.p) HIP_TRY(s.d_counter.alloc(1));
     HIP_TRY(launch_fill_i32(s.d_counter.p, 1, 0, stream));
@@ -517,7 +518,7 @@ int run_batch(glass_hip_searcher &s, const float *qs_dev, int32_t nq, const int3
 
 int32_t inner_k(const glass_hip_searcher &s, int32_t k) {
     if (s.refine == QK_FP32) {
-        int32_t rk = (int32_t)((float)k * refine_factor_of(s.quant));
+        int32_t rk = (int32_t)((float)k * s.factor());
         return std::max(rk, k);
     }
     return k;
@@ -566,7 +567,7 @@ int host_batch(glass_hip_searcher &s, const float *qs, int32_t nq, const int32_t
         d_off = s.d_off.p;
     }
     int r = run_batch(s, s.d_q.p, nq, d_ef, d_k, s.ef, k_uniform, max_cap, mode, d_off, s.d_ids.p, s.d_dists.p,
-                      s.d_counters.p, s.stream, 0, 0);
+                      s.d_counters.p, s.stream);
     if (r) return r;
     HIP_TRY(hipMemcpyAsync(dst, s.d_ids.p, 4 * (size_t)total, hipMemcpyDeviceToHost, s.stream));
     if (scores)
@@ -683,6 +684,26 @@ int glass_hip_graph_copy(const glass_hip_graph *g, int32_t *adj, int32_t *eps) {
 }
 
 void glass_hip_graph_free(glass_hip_graph *g) { delete g; }
+
+int glass_hip_build_graph(const char *index_type, const float *data, int32_t n, int32_t dim, const char *metric,
+                          int32_t R, int32_t L, int32_t num_threads, glass_hip_graph **out, double *seconds) {
+    if (!index_type || !data || n <= 0 || dim <= 0 || !metric || !out || R < 2 || R > 64)
+        return fail(GLASS_HIP_EINVAL, "bad build arguments");
+    std::string t(index_type), m(metric);
+    if (m != "L2" && m != "IP") return fail(GLASS_HIP_EINVAL, "unknown metric '%s'", metric);
+    if (t != "HNSW") return fail(GLASS_HIP_EINVAL, "unsupported index type '%s'", index_type);
+    auto g = std::make_unique<glass_hip_graph>();
+    std::vector<int32_t> levels, lists;
+    int32_t ep = 0;
+    if (glass_build::build_hnsw(data, n, dim, m == "IP", R, L, num_threads, g->adj, ep, levels, lists, seconds))
+        return fail(GLASS_HIP_EINVAL, "HNSW build failed");
+    g->N = n;
+    g->K = R;
+    g->eps = {ep};
+    if (int r = glass_hip_graph_set_upper(g.get(), R / 2, ep, levels.data(), lists.data())) return r;
+    *out = g.release();
+    return GLASS_HIP_OK;
+}
 
 int glass_hip_searcher_create(glass_hip_graph *g, const char *metric, const char *quantizer, const char *refine_quant,
                               glass_hip_searcher **out) {
@@ -827,7 +848,7 @@ int glass_hip_searcher_search_batch_dev(glass_hip_searcher *s, const float *qs_d
     const int32_t max_cap = std::max(inner_k(*s, k), s->ef);
     if (round_cap(max_cap) > 4096) return fail(GLASS_HIP_EINVAL, "max(k, ef) = %d exceeds 4096", max_cap);
     return run_batch(*s, qs_dev, nq, nullptr, nullptr, s->ef, k, max_cap, mode, nullptr, dst_dev, scores_dev,
-                     counters_dev, (hipStream_t)stream, 0, 0);
+                     counters_dev, (hipStream_t)stream);
 }
 
 int glass_hip_searcher_optimize(glass_hip_searcher *s, int32_t num_threads) {
@@ -878,6 +899,13 @@ double glass_hip_searcher_last_avg_dist_cmps(const glass_hip_searcher *s) { retu
 
 void glass_hip_searcher_free(glass_hip_searcher *s) { delete s; }
 
+int glass_hip_searcher_set_refine_factor(glass_hip_searcher *s, float factor) {
+    if (!s || !(factor >= 0.0f) || factor > 64.0f) return fail(GLASS_HIP_EINVAL, "refine factor must be in [0, 64]");
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->refine_override = factor;
+    return GLASS_HIP_OK;
+}
+
 int glass_hip_searcher_set_launch(glass_hip_searcher *s, int32_t team, int32_t waves_per_block, int32_t blocks) {
     if (!s) return fail(GLASS_HIP_EINVAL, "null searcher");
     if (waves_per_block < 1 || waves_per_block > 4 || blocks < 0) return fail(GLASS_HIP_EINVAL, "bad launch config");
@@ -906,7 +934,7 @@ int glass_hip_searcher_info(const glass_hip_searcher *s, int32_t *code_size, int
     if (rerank_row_bytes) *rerank_row_bytes = s->refine == QK_FP32 ? 4 * s->r_dim_align : 0;
     if (degree) *degree = s->graph.K;
     if (n) *n = s->graph.N;
-    if (refine_factor) *refine_factor = s->refine == QK_FP32 ? refine_factor_of(s->quant) : 1.0f;
+    if (refine_factor) *refine_factor = s->refine == QK_FP32 ? s->factor() : 1.0f;
     return GLASS_HIP_OK;
 }
 

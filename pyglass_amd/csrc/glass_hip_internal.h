@@ -77,6 +77,9 @@ hipError_t launch_search(const DeviceIndex &ix, const QueryBatch &qb, const Laun
 // Per-wave dynamic LDS bytes for a configuration.
 size_t lds_bytes_per_wave(const DeviceIndex &ix, int32_t cap_max);
 
+// Resident blocks per CU for the kernel `launch_search` would pick (0 on error).
+int max_blocks_per_cu(const DeviceIndex &ix, const LaunchConfig &lc);
+
 // Valid team widths for the index's code layout (powers of two dividing
 // code_size/4 with at most 16 dwords per lane).
 int32_t default_team(const DeviceIndex &ix);

@@ -728,6 +728,26 @@ hipError_t launch_one(const DeviceIndex &ix, const QueryBatch &qb, const LaunchC
     return hipGetLastError();
 }
 
+template <int V, int DPL>
+int occupancy_one(const DeviceIndex &ix, const LaunchConfig &lc) {
+    const size_t lds = wave_lds_bytes(ix, lc.cap_max) * (size_t)lc.waves_per_block;
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, (const void *)search_kernel<V, DPL>, 64 * lc.waves_per_block,
+                                                     lds) != hipSuccess)
+        return 0;
+    return nb;
+}
+
+template <int V>
+int occupancy_int(const DeviceIndex &ix, const LaunchConfig &lc) {
+    switch (lc.team) {
+        case 4: return occupancy_one<V, 4>(ix, lc);
+        case 8: return occupancy_one<V, 8>(ix, lc);
+        case 16: return occupancy_one<V, 16>(ix, lc);
+        default: return 0;
+    }
+}
+
 template <int V>
 hipError_t launch_int(const DeviceIndex &ix, const QueryBatch &qb, const LaunchConfig &lc, const SlotState &ss,
                       hipStream_t stream) {
@@ -742,6 +762,19 @@ hipError_t launch_int(const DeviceIndex &ix, const QueryBatch &qb, const LaunchC
 }  // namespace
 
 size_t lds_bytes_per_wave(const DeviceIndex &ix, int32_t cap_max) { return wave_lds_bytes(ix, cap_max); }
+
+int max_blocks_per_cu(const DeviceIndex &ix, const LaunchConfig &lc) {
+    const bool l2 = ix.metric == MK_L2;
+    switch (ix.quant) {
+        case QK_SQ8U: return l2 ? occupancy_int<V_SQ8U_L2>(ix, lc) : occupancy_int<V_SQ8U_IP>(ix, lc);
+        case QK_SQ4U: return occupancy_int<V_SQ4U>(ix, lc);
+        case QK_SQ2U: return occupancy_int<V_SQ2U>(ix, lc);
+        case QK_E5M2: return l2 ? occupancy_one<V_E5M2_L2, 4>(ix, lc) : occupancy_one<V_E5M2_IP, 4>(ix, lc);
+        case QK_SQ8: return l2 ? occupancy_one<V_SQ8_L2, 4>(ix, lc) : occupancy_one<V_SQ8_IP, 4>(ix, lc);
+        case QK_FP32: return l2 ? occupancy_one<V_FP32_L2, 4>(ix, lc) : occupancy_one<V_FP32_IP, 4>(ix, lc);
+        default: return 0;
+    }
+}
 
 bool team_valid(const DeviceIndex &ix, int32_t team) {
     if (ix.quant != QK_SQ8U && ix.quant != QK_SQ4U && ix.quant != QK_SQ2U) return team == 16;

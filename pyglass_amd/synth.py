@@ -39,10 +39,12 @@ def uint8_mixture(n: int, d: int, seed: int, n_centers: int = 1024, sigma: float
     for s in range(0, n, chunk):
         e = min(n, s + chunk)
         lab = rng.integers(0, n_centers, size=e - s)
-        x = centers[lab] + sigma * rng.standard_normal((e - s, d), dtype=np.float32)
+        x = out[s:e]
+        rng.standard_normal(out=x, dtype=np.float32)
+        x *= np.float32(sigma)
+        x += centers[lab]
         np.clip(x, 0.0, 255.0, out=x)
         np.rint(x, out=x)
-        out[s:e] = x
     return out
 
 

@@ -104,3 +104,22 @@ def test_golden_reference_ids_directly(dsets, golden_dir):
                 I, D = s.batch_search(QQ[:64], k)
                 np.testing.assert_array_equal(I, z[f"{key}/batch_ids"], err_msg=key)
                 np.testing.assert_array_equal(D, z[f"{key}/batch_dists"], err_msg=key)
+
+
+@pytest.mark.parametrize("quant,factor", [("SQ4U", 4.0), ("SQ2U", 4.0), ("SQ8U", 8.0)])
+def test_refine_factor_override_matches_oracle(dsets, golden_dir, quant, factor):
+    """Re-rank depth override (Refiner's constructor argument): HIP == oracle."""
+    XX, QQ, metric = dsets["c1_hnsw_l2"]
+    path = os.path.join(golden_dir, "c1_hnsw_l2.glass")
+    s = pg.Searcher(pg.Graph(path), XX, metric, quant, "FP32")
+    s.set_refine_factor(factor)
+    o = O.Searcher(O.Graph(path), XX, metric, quant, "FP32", refine_factor=factor)
+    for ef in [32, 96]:
+        for k in [1, 10, 30]:
+            print(f"factor {quant} {factor} ef={ef} k={k}", flush=True)
+            s.set_ef(ef)
+            o.set_ef(ef)
+            I, D = s.batch_search(QQ[:200], k)
+            Io, Do = o.batch_search(QQ[:200], k)
+            np.testing.assert_array_equal(I, Io)
+            np.testing.assert_array_equal(D, Do)
