@@ -166,7 +166,8 @@ def gpu_sweep(s, q_dev, nq, k, gt, ids_dev, d_dev, cnt_dev, args, torch, stream,
                     "expansions": float(cnt[:, 1].mean()), "upper": float(cnt[:, 2].mean()),
                     "ids": ids, "cnt": cnt})
         log(f"gpu: factor={factor} ef={ef} recall={r:.4f} qps={nq / (ms / 1e3):.0f} "
-            f"kernel={np.mean(kern_ms):.3f}ms dist_cmps={cnt[:, 0].mean():.0f} exp={cnt[:, 1].mean():.1f}")
+            f"kernel={np.mean(kern_ms):.3f}ms dist_cmps={cnt[:, 0].mean():.0f} exp={cnt[:, 1].mean():.1f} "
+            f"launch={s.get_launch()} kernel={s.kernel_info()}")
         if r >= 0.99:
             break
     return pts
@@ -274,6 +275,7 @@ def main():
     launch = s.get_launch()
     info = s.info()
     log(f"searcher: set_data+optimize {time.time() - t:.1f}s, launch {launch}, info {info}")
+    log(f"device: {pg.device_info()}")
 
     q_dev = torch.from_numpy(Q).to(dev)
     ids_dev = torch.empty((nq, k), dtype=torch.int32, device=dev)

@@ -110,12 +110,28 @@ int glass_hip_searcher_search_batch_dev(glass_hip_searcher *s, const float *qs_d
  * SQ8U/SQ8/SQ4U, 3.0 for SQ2U), which Refiner takes as its constructor
  * argument (refiner.hpp:22-24). */
 int glass_hip_searcher_set_refine_factor(glass_hip_searcher *s, float factor);
-/* Launch configuration (set by optimize): team = dwords per lane (4, 8, 16)
- * for integer codes, waves per block, blocks of the persistent grid (0 =
- * fill the device).  Query with _get. */
+/* Launch shape.  Optimize tunes, per ef bucket (<=16, 32, 64, 128, 256,
+ * 512, larger), the dwords each lane gathers per candidate row (team: 4, 8
+ * or 16), the waves per workgroup (1-4) and the LDS visited-table size.
+ * set_launch forces team / waves_per_block for every bucket (0 = tuned) and
+ * caps the persistent grid at `blocks` workgroups (0 = fill the device). */
 int glass_hip_searcher_set_launch(glass_hip_searcher *s, int32_t team, int32_t waves_per_block, int32_t blocks);
+/* LDS visited-table entries per query (power of two in [256, 16384]; 0 =
+ * tuned / sized from ef).  A query that visits more than half as many nodes
+ * continues on the global bitset of its slot. */
+int glass_hip_searcher_set_visited_capacity(glass_hip_searcher *s, int32_t entries);
+/* Shape used at the current ef: team, waves per workgroup, blocks of the
+ * last launch, visited-table entries. */
 int glass_hip_searcher_get_launch(const glass_hip_searcher *s, int32_t *team, int32_t *waves_per_block,
-                                  int32_t *blocks, int32_t *slots);
+                                  int32_t *blocks, int32_t *vis_entries);
+/* Last launch: kernel VGPRs, LDS bytes per workgroup, resident workgroups
+ * per CU the persistent grid was sized for. */
+int glass_hip_searcher_kernel_info(const glass_hip_searcher *s, int32_t *vgprs, int32_t *lds_per_block,
+                                   int32_t *blocks_per_cu);
+/* Current device: "name (arch)", compute units, LDS bytes per CU as the
+ * runtime reports it, HBM bytes, clock in kHz. */
+int glass_hip_device_info(char *name, int32_t name_len, int32_t *cu_count, int64_t *lds_per_cu, int64_t *hbm_bytes,
+                          int32_t *clock_khz);
 /* Sizes for accounting: code bytes per row, FP32 re-rank bytes per row,
  * graph degree, number of rows, refine factor. */
 int glass_hip_searcher_info(const glass_hip_searcher *s, int32_t *code_size, int32_t *rerank_row_bytes,

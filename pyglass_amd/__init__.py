@@ -44,6 +44,16 @@ def version() -> str:
     return _lib.load().glass_hip_version().decode()
 
 
+def device_info() -> dict:
+    """Name/arch, compute units, LDS per CU (as the runtime reports it), HBM bytes, clock."""
+    buf = C.create_string_buffer(256)
+    cu, clk = C.c_int32(), C.c_int32()
+    lds, hbm = C.c_int64(), C.c_int64()
+    _lib.check(_lib.load().glass_hip_device_info(buf, 256, C.byref(cu), C.byref(lds), C.byref(hbm), C.byref(clk)))
+    return {"name": buf.value.decode(errors="replace"), "cus": cu.value, "lds_per_cu": lds.value,
+            "hbm_bytes": hbm.value, "clock_khz": clk.value}
+
+
 class Graph:
     """glass.Graph: fixed-degree adjacency (N x K int32, -1 padded), entry
     points and optional HNSW upper layers (glass/graph.hpp:26-246)."""
@@ -245,14 +255,26 @@ class Searcher:
         reference's get_refine_factor(quantizer)."""
         _lib.check(self._L.glass_hip_searcher_set_refine_factor(self._h, float(factor)))
 
-    # ---- introspection -------------------------------------------------
-    def set_launch(self, team: int, waves_per_block: int, blocks: int = 0):
+    # ---- launch shape and introspection ---------------------------------
+    def set_launch(self, team: int = 0, waves_per_block: int = 0, blocks: int = 0):
+        """Force lanes-per-candidate team / waves per workgroup (0 = tuned) and
+        cap the persistent grid (0 = fill the device)."""
         _lib.check(self._L.glass_hip_searcher_set_launch(self._h, int(team), int(waves_per_block), int(blocks)))
 
+    def set_visited_capacity(self, entries: int = 0):
+        """LDS visited-table entries per query (0 = tuned / sized from ef)."""
+        _lib.check(self._L.glass_hip_searcher_set_visited_capacity(self._h, int(entries)))
+
+    def kernel_info(self):
+        """VGPRs, LDS bytes per workgroup and resident workgroups per CU of the last launch."""
+        v, l, b = C.c_int32(), C.c_int32(), C.c_int32()
+        _lib.check(self._L.glass_hip_searcher_kernel_info(self._h, C.byref(v), C.byref(l), C.byref(b)))
+        return {"vgprs": v.value, "lds_per_block": l.value, "blocks_per_cu": b.value}
+
     def get_launch(self):
-        t, w, b, s = C.c_int32(), C.c_int32(), C.c_int32(), C.c_int32()
-        _lib.check(self._L.glass_hip_searcher_get_launch(self._h, C.byref(t), C.byref(w), C.byref(b), C.byref(s)))
-        return {"team": t.value, "waves_per_block": w.value, "blocks": b.value, "slots": s.value}
+        t, w, b, v = C.c_int32(), C.c_int32(), C.c_int32(), C.c_int32()
+        _lib.check(self._L.glass_hip_searcher_get_launch(self._h, C.byref(t), C.byref(w), C.byref(b), C.byref(v)))
+        return {"team": t.value, "waves_per_block": w.value, "blocks": b.value, "vis_entries": v.value}
 
     def info(self):
         cs, rb, deg, n = C.c_int32(), C.c_int32(), C.c_int32(), C.c_int32()

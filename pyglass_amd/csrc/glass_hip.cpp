@@ -158,9 +158,30 @@ struct glass_hip_searcher {
     double last_avg_dist_cmps = 0.0;
     bool has_data = false;
 
-    // launch configuration
-    int32_t team = 8, waves_per_block = 4, blocks = 0;
+    // launch configuration, tuned per ef bucket by Optimize
+    struct Tune {
+        int32_t team = 8, waves_per_block = 4, vis_hash = 0;  // vis_hash 0: size from ef
+    };
+    static constexpr int kBuckets = 7;  // ef <= 16, 32, 64, 128, 256, 512, larger
+    Tune tune[kBuckets];
+    int32_t blocks = 0;           // persistent grid cap (0 = fill the device)
+    int32_t vis_override = 0;     // visited-table entries forced by the caller (0 = tuned/auto)
+    int32_t force_team = 0, force_wpb = 0;
     int32_t cu_count = 0;
+    LaunchConfig last_launch{};
+
+    static int bucket(int32_t ef) {
+        int b = 0;
+        for (int32_t e = 16; b < kBuckets - 1 && ef > e; e <<= 1) ++b;
+        return b;
+    }
+    // LDS visited table sized to stay under half full for a typical search
+    // visiting ~16-24 nodes per unit of ef.
+    static int32_t auto_vis(int32_t ef) {
+        int32_t v = 1024;
+        while (v < 48 * ef && v < 4096) v <<= 1;
+        return v;
+    }
 
     // device-resident index
     DevBuf<int32_t> d_adj, d_levels, d_up_lists, d_eps;
@@ -477,17 +498,20 @@ int32_t round_cap(int32_t cap) {
 
 // Enqueue one batch: queries/outputs are device pointers.
 int run_batch(glass_hip_searcher &s, const float *qs_dev, int32_t nq, const int32_t *ef_dev, const int32_t *k_dev,
-              int32_t ef_uniform, int32_t k_uniform, int32_t max_cap, int32_t mode, const int64_t *off_dev,
-              int32_t *ids_dev, float *dists_dev, int32_t *counters_dev, hipStream_t stream) {
+              int32_t ef_uniform, int32_t k_uniform, int32_t max_cap, int32_t max_ef, int32_t mode,
+              const int64_t *off_dev, int32_t *ids_dev, float *dists_dev, int32_t *counters_dev, hipStream_t stream) {
     if (!s.has_data) return fail(GLASS_HIP_ESTATE, "search before set_data");
     if (nq <= 0) return GLASS_HIP_OK;
     if (int r = ensure_stream(s)) return r;
     if (!stream) stream = s.stream;
     LaunchConfig lc{};
-    lc.team = team_valid(s.index(), s.team) ? s.team : default_team(s.index());
-    lc.waves_per_block = s.waves_per_block;
+    const glass_hip_searcher::Tune &tn = s.tune[glass_hip_searcher::bucket(max_ef)];
+    lc.team = s.force_team ? s.force_team : tn.team;
+    if (!team_valid(s.index(), lc.team)) lc.team = default_team(s.index());
+    lc.waves_per_block = s.force_wpb ? s.force_wpb : tn.waves_per_block;
     lc.cap_max = round_cap(max_cap);
-    const size_t lds = lds_bytes_per_wave(s.index(), lc.cap_max) * (size_t)lc.waves_per_block;
+    lc.vis_hash = s.vis_override ? s.vis_override : (tn.vis_hash ? tn.vis_hash : glass_hip_searcher::auto_vis(max_ef));
+    const size_t lds = lds_bytes_per_wave(s.index(), lc.cap_max, lc.vis_hash) * (size_t)lc.waves_per_block;
     if (lds > 160 * 1024) return fail(GLASS_HIP_EINVAL, "pool capacity %d needs %zu bytes of LDS", lc.cap_max, lds);
     const int32_t blocks = grid_blocks(s, lc, nq);
     lc.blocks = blocks;
@@ -513,6 +537,7 @@ int run_batch(glass_hip_searcher &s, const float *qs_dev, int32_t nq, const int3
     ss.nslots = s.nslots;
     ss.work_counter = s.d_counter.p;
     HIP_TRY(launch_search(s.index(), qb, lc, ss, stream));
+    s.last_launch = lc;
     return GLASS_HIP_OK;
 }
 
@@ -532,15 +557,17 @@ int host_batch(glass_hip_searcher &s, const float *qs, int32_t nq, const int32_t
     if (nq <= 0) return GLASS_HIP_OK;
     if (int r = ensure_stream(s)) return r;
     int64_t total = 0;
-    int32_t max_cap = 1;
+    int32_t max_cap = 1, max_ef = s.ef;
     std::vector<int64_t> off;
     if (k) {
         off.resize(nq);
+        max_ef = 1;
         for (int32_t i = 0; i < nq; ++i) {
             if (k[i] <= 0 || ef[i] <= 0) return fail(GLASS_HIP_EINVAL, "k and ef must be positive");
             off[i] = total;
             total += k[i];
             max_cap = std::max(max_cap, std::max(inner_k(s, k[i]), ef[i]));
+            max_ef = std::max(max_ef, ef[i]);
         }
     } else {
         if (k_uniform <= 0) return fail(GLASS_HIP_EINVAL, "k must be positive");
@@ -566,7 +593,7 @@ int host_batch(glass_hip_searcher &s, const float *qs, int32_t nq, const int32_t
         d_k = s.d_k.p;
         d_off = s.d_off.p;
     }
-    int r = run_batch(s, s.d_q.p, nq, d_ef, d_k, s.ef, k_uniform, max_cap, mode, d_off, s.d_ids.p, s.d_dists.p,
+    int r = run_batch(s, s.d_q.p, nq, d_ef, d_k, s.ef, k_uniform, max_cap, max_ef, mode, d_off, s.d_ids.p, s.d_dists.p,
                       s.d_counters.p, s.stream);
     if (r) return r;
     HIP_TRY(hipMemcpyAsync(dst, s.d_ids.p, 4 * (size_t)total, hipMemcpyDeviceToHost, s.stream));
@@ -798,7 +825,7 @@ int glass_hip_searcher_set_data(glass_hip_searcher *s, const float *data, int32_
         for (int32_t i = 0; i < s->sample_points_num; ++i)
             std::memcpy(s->optimize_queries.data() + (size_t)i * dim, data + (size_t)pts[i] * dim, 4 * (size_t)dim);
     }
-    s->team = default_team(s->index());
+    for (auto &t : s->tune) t.team = default_team(s->index());
     s->has_data = true;
     return GLASS_HIP_OK;
 }
@@ -815,13 +842,12 @@ int glass_hip_searcher_search(glass_hip_searcher *s, const float *q, int32_t k, 
     if (!s || !q || !dst || k <= 0) return fail(GLASS_HIP_EINVAL, "bad search arguments");
     std::lock_guard<std::mutex> lk(s->mu);
     // one query per launch: a single wave
-    const int32_t saved = s->blocks;
-    const int32_t saved_wpb = s->waves_per_block;
+    const int32_t saved = s->blocks, saved_wpb = s->force_wpb;
     s->blocks = 1;
-    s->waves_per_block = 1;
+    s->force_wpb = 1;
     int r = host_batch(*s, q, 1, nullptr, nullptr, k, MODE_SEARCH, dst, scores, nullptr, false);
     s->blocks = saved;
-    s->waves_per_block = saved_wpb;
+    s->force_wpb = saved_wpb;
     return r;
 }
 
@@ -847,7 +873,7 @@ int glass_hip_searcher_search_batch_dev(glass_hip_searcher *s, const float *qs_d
     std::lock_guard<std::mutex> lk(s->mu);
     const int32_t max_cap = std::max(inner_k(*s, k), s->ef);
     if (round_cap(max_cap) > 4096) return fail(GLASS_HIP_EINVAL, "max(k, ef) = %d exceeds 4096", max_cap);
-    return run_batch(*s, qs_dev, nq, nullptr, nullptr, s->ef, k, max_cap, mode, nullptr, dst_dev, scores_dev,
+    return run_batch(*s, qs_dev, nq, nullptr, nullptr, s->ef, k, max_cap, s->ef, mode, nullptr, dst_dev, scores_dev,
                      counters_dev, (hipStream_t)stream);
 }
 
@@ -856,42 +882,66 @@ int glass_hip_searcher_optimize(glass_hip_searcher *s, int32_t num_threads) {
     if (!s) return fail(GLASS_HIP_EINVAL, "null searcher");
     if (!s->has_data) return fail(GLASS_HIP_ESTATE, "optimize before set_data");
     std::lock_guard<std::mutex> lk(s->mu);
-    const int32_t nq = s->sample_points_num;
-    if (nq <= 0) return GLASS_HIP_OK;
+    if (s->sample_points_num <= 0) return GLASS_HIP_OK;
+    // GraphSearcher::Optimize (graph_searcher.hpp:76-129) times SearchBatch on
+    // the sampled base rows at k=10 over a grid of prefetch knobs and keeps
+    // the fastest.  Here the knobs are the launch shape (lanes per candidate,
+    // waves per workgroup, LDS visited-table size), tuned per ef bucket on the
+    // samples tiled to a device-filling batch.
     const int32_t kTryK = 10;
-    // Grid over the launch knobs, each point timed on the sampled rows with
-    // SearchBatch at k=10, as GraphSearcher::Optimize times prefetch knobs.
+    const int32_t base = s->sample_points_num;
+    const int32_t nq = std::max(base, std::min(8192, 8 * base));
+    std::vector<float> qs((size_t)nq * s->dim);
+    for (int32_t i = 0; i < nq; ++i)
+        std::memcpy(qs.data() + (size_t)i * s->dim, s->optimize_queries.data() + (size_t)(i % base) * s->dim,
+                    4 * (size_t)s->dim);
     std::vector<int32_t> teams;
     for (int32_t t : {4, 8, 16}) {
         if (team_valid(s->index(), t)) teams.push_back(t);
     }
-    std::vector<int32_t> wpbs = {1, 2, 4};
+    const int32_t wpbs[] = {1, 2, 4};
+    const int32_t vhs[] = {1024, 2048, 4096};
     std::vector<int32_t> ids((size_t)nq * kTryK);
+    const int32_t saved_ef = s->ef;
+    const int32_t saved_vo = s->vis_override, saved_ft = s->force_team, saved_fw = s->force_wpb;
     auto timeit = [&]() -> double {
         auto t0 = std::chrono::high_resolution_clock::now();
-        int r = host_batch(*s, s->optimize_queries.data(), nq, nullptr, nullptr, kTryK, MODE_BATCH, ids.data(),
-                           nullptr, nullptr, false);
+        int r = host_batch(*s, qs.data(), nq, nullptr, nullptr, kTryK, MODE_BATCH, ids.data(), nullptr, nullptr,
+                           false);
         auto t1 = std::chrono::high_resolution_clock::now();
         return r ? 1e30 : std::chrono::duration<double>(t1 - t0).count();
     };
-    timeit();
-    double best = 1e30;
-    int32_t bt = s->team, bw = s->waves_per_block;
-    for (int32_t t : teams) {
-        for (int32_t w : wpbs) {
-            s->team = t;
-            s->waves_per_block = w;
-            timeit();
-            double e = std::min(timeit(), timeit());
-            if (e < best) {
-                best = e;
-                bt = t;
-                bw = w;
+    for (int b = 0; b < glass_hip_searcher::kBuckets - 1; ++b) {
+        s->ef = 16 << b;  // upper edge of the bucket
+        double best = 1e30;
+        glass_hip_searcher::Tune bt = s->tune[b];
+        for (int32_t t : teams) {
+            for (int32_t w : wpbs) {
+                for (int32_t v : vhs) {
+                    s->force_team = t;
+                    s->force_wpb = w;
+                    s->vis_override = v;
+                    const size_t lds = lds_bytes_per_wave(s->index(), round_cap(std::max(inner_k(*s, kTryK), s->ef)),
+                                                          v) * (size_t)w;
+                    if (lds > 160 * 1024) continue;
+                    timeit();
+                    double e = std::min(timeit(), timeit());
+                    if (e < best) {
+                        best = e;
+                        bt.team = t;
+                        bt.waves_per_block = w;
+                        bt.vis_hash = v;
+                    }
+                }
             }
         }
+        s->tune[b] = bt;
     }
-    s->team = bt;
-    s->waves_per_block = bw;
+    s->tune[glass_hip_searcher::kBuckets - 1] = s->tune[glass_hip_searcher::kBuckets - 2];
+    s->ef = saved_ef;
+    s->vis_override = saved_vo;
+    s->force_team = saved_ft;
+    s->force_wpb = saved_fw;
     return GLASS_HIP_OK;
 }
 
@@ -908,22 +958,62 @@ int glass_hip_searcher_set_refine_factor(glass_hip_searcher *s, float factor) {
 
 int glass_hip_searcher_set_launch(glass_hip_searcher *s, int32_t team, int32_t waves_per_block, int32_t blocks) {
     if (!s) return fail(GLASS_HIP_EINVAL, "null searcher");
-    if (waves_per_block < 1 || waves_per_block > 4 || blocks < 0) return fail(GLASS_HIP_EINVAL, "bad launch config");
-    if (s->has_data && !team_valid(s->index(), team)) return fail(GLASS_HIP_EINVAL, "team %d invalid for this layout", team);
+    if (waves_per_block < 0 || waves_per_block > 4 || blocks < 0) return fail(GLASS_HIP_EINVAL, "bad launch config");
+    if (team && s->has_data && !team_valid(s->index(), team))
+        return fail(GLASS_HIP_EINVAL, "team %d invalid for this layout", team);
     std::lock_guard<std::mutex> lk(s->mu);
-    s->team = team;
-    s->waves_per_block = waves_per_block;
+    s->force_team = team;
+    s->force_wpb = waves_per_block;
     s->blocks = blocks;
+    return GLASS_HIP_OK;
+}
+
+int glass_hip_searcher_set_visited_capacity(glass_hip_searcher *s, int32_t entries) {
+    if (!s) return fail(GLASS_HIP_EINVAL, "null searcher");
+    if (entries != 0 && (entries < 256 || entries > 16384 || (entries & (entries - 1))))
+        return fail(GLASS_HIP_EINVAL, "visited table size must be 0 or a power of two in [256, 16384]");
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->vis_override = entries;
     return GLASS_HIP_OK;
 }
 
 int glass_hip_searcher_get_launch(const glass_hip_searcher *s, int32_t *team, int32_t *waves_per_block,
                                   int32_t *blocks, int32_t *slots) {
     if (!s) return fail(GLASS_HIP_EINVAL, "null searcher");
-    if (team) *team = s->team;
-    if (waves_per_block) *waves_per_block = s->waves_per_block;
-    if (blocks) *blocks = s->blocks;
-    if (slots) *slots = s->nslots;
+    const glass_hip_searcher::Tune &tn = s->tune[glass_hip_searcher::bucket(s->ef)];
+    if (team) *team = s->force_team ? s->force_team : tn.team;
+    if (waves_per_block) *waves_per_block = s->force_wpb ? s->force_wpb : tn.waves_per_block;
+    if (blocks) *blocks = s->last_launch.blocks;
+    if (slots) *slots = s->last_launch.vis_hash ? s->last_launch.vis_hash
+                                                : (tn.vis_hash ? tn.vis_hash : glass_hip_searcher::auto_vis(s->ef));
+    return GLASS_HIP_OK;
+}
+
+int glass_hip_searcher_kernel_info(const glass_hip_searcher *s, int32_t *vgprs, int32_t *lds_per_block,
+                                   int32_t *blocks_per_cu) {
+    if (!s) return fail(GLASS_HIP_EINVAL, "null searcher");
+    if (!s->has_data || !s->last_launch.team) return fail(GLASS_HIP_ESTATE, "no launch yet");
+    const LaunchConfig &lc = s->last_launch;
+    if (vgprs) *vgprs = kernel_vgprs(s->index(), lc);
+    if (lds_per_block)
+        *lds_per_block = (int32_t)(lds_bytes_per_wave(s->index(), lc.cap_max, lc.vis_hash) * lc.waves_per_block);
+    if (blocks_per_cu) *blocks_per_cu = max_blocks_per_cu(s->index(), lc);
+    return GLASS_HIP_OK;
+}
+
+int glass_hip_device_info(char *name, int32_t name_len, int32_t *cu_count, int64_t *lds_per_cu, int64_t *hbm_bytes,
+                          int32_t *clock_khz) {
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    hipDeviceProp_t p;
+    HIP_TRY(hipGetDeviceProperties(&p, dev));
+    if (name && name_len > 0) {
+        std::snprintf(name, (size_t)name_len, "%s (%s)", p.name, p.gcnArchName);
+    }
+    if (cu_count) *cu_count = p.multiProcessorCount;
+    if (lds_per_cu) *lds_per_cu = (int64_t)p.maxSharedMemoryPerMultiProcessor;
+    if (hbm_bytes) *hbm_bytes = (int64_t)p.totalGlobalMem;
+    if (clock_khz) *clock_khz = p.clockRate;
     return GLASS_HIP_OK;
 }
 

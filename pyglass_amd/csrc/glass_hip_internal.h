@@ -59,7 +59,7 @@ struct LaunchConfig {
     int32_t waves_per_block;
     int32_t blocks;          // persistent grid size
     int32_t cap_max;         // LDS pool capacity per wave (>= every query's max(k',ef))
-    int32_t spec_gather;     // reserved
+    int32_t vis_hash;        // LDS visited-table entries per wave (power of two >= 256)
 };
 
 struct SlotState {
@@ -75,10 +75,12 @@ hipError_t launch_search(const DeviceIndex &ix, const QueryBatch &qb, const Laun
                          hipStream_t stream);
 
 // Per-wave dynamic LDS bytes for a configuration.
-size_t lds_bytes_per_wave(const DeviceIndex &ix, int32_t cap_max);
+size_t lds_bytes_per_wave(const DeviceIndex &ix, int32_t cap_max, int32_t vis_hash);
 
 // Resident blocks per CU for the kernel `launch_search` would pick (0 on error).
 int max_blocks_per_cu(const DeviceIndex &ix, const LaunchConfig &lc);
+// VGPRs of the kernel `launch_search` would pick.
+int kernel_vgprs(const DeviceIndex &ix, const LaunchConfig &lc);
 
 // Valid team widths for the index's code layout (powers of two dividing
 // code_size/4 with at most 16 dwords per lane).

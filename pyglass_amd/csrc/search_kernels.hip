@@ -95,6 +95,7 @@ constexpr bool is_int_variant(int v) { return v < 10; }
 
 // Per-wave LDS layout.  All offsets are multiples of 16 bytes.
 struct WaveLds {
+    uint32_t *vt;   // visited hash table, id+1 per slot, 0 = empty [vis_hash]
     uint32_t *pk;   // pool keys [cap_max]
     uint32_t *pid;  // pool ids (bit 31 = checked) [cap_max]
     uint32_t *ck;   // candidate keys [64]
@@ -112,8 +113,8 @@ __host__ __device__ inline int32_t qf_len(const DeviceIndex &ix) {
     return round16(a);
 }
 
-__host__ __device__ inline size_t wave_lds_bytes(const DeviceIndex &ix, int32_t cap_max) {
-    size_t b = 0;
+__host__ __device__ inline size_t wave_lds_bytes(const DeviceIndex &ix, int32_t cap_max, int32_t vis_hash) {
+    size_t b = (size_t)vis_hash * 4;
     b += (size_t)round16((int64_t)cap_max * 4) * 2;
     b += 64 * 4 * 2;
     b += (size_t)qf_len(ix) * 4;
@@ -122,9 +123,10 @@ __host__ __device__ inline size_t wave_lds_bytes(const DeviceIndex &ix, int32_t 
     return b;
 }
 
-__device__ inline WaveLds carve(uint8_t *base, const DeviceIndex &ix, int32_t cap_max) {
+__device__ inline WaveLds carve(uint8_t *base, const DeviceIndex &ix, int32_t cap_max, int32_t vis_hash) {
     WaveLds w;
-    size_t off = 0;
+    w.vt = (uint32_t *)base;
+    size_t off = (size_t)vis_hash * 4;
     int32_t capb = round16((int64_t)cap_max * 4);
     w.pk = (uint32_t *)(base + off);
     off += capb;
@@ -193,36 +195,24 @@ __device__ void encode_query(const DeviceIndex &ix, const WaveLds &w) {
 //   SQ8U IP  helpa::dot_s8_s8  -sum x*y over int8 codes in [-127,127]
 //   SQ4U     helpa::l2a_u4_u4  sum (x-y)^2 over nibbles
 //   SQ2U     helpa::l2a_u2_u2  sum (x-y)^2 over 2-bit fields
-// Unsigned fields are turned into signed byte differences without borrows:
-// ((q | 0x80) - c) ^ 0x80 per byte equals q - c as int8 for q, c <= 127.
+// The squared distances use sum (x-y)^2 = sum x^2 + sum y^2 - 2 sum x*y on
+// unsigned fields: per code dword one dot instruction for sum x*y and one for
+// sum y^2 (v_dot4_u32_u8 on bytes, v_dot8_u32_u4 on nibbles; 2-bit fields are
+// split into two nibble dwords with a 0x33333333 mask), sum x^2 of the lane's
+// slice of the query precomputed.  All terms are integers below 2^31, so the
+// int32 result is the reference's value exactly.
 
-template <int V>
-__device__ __forceinline__ int32_t int_partial_dword(uint32_t q, uint32_t c, int32_t acc) {
-    if (V == V_SQ8U_L2) {
-        uint32_t t = ((q | 0x80808080u) - c) ^ 0x80808080u;
-        return sdot4(t, t, acc);
-    } else if (V == V_SQ8U_IP) {
-        return sdot4(q, c, acc);
-    } else if (V == V_SQ4U) {
-        uint32_t ql = (q & 0x0f0f0f0fu) | 0x80808080u, qh = ((q >> 4) & 0x0f0f0f0fu) | 0x80808080u;
-        uint32_t t1 = (ql - (c & 0x0f0f0f0fu)) ^ 0x80808080u;
-        uint32_t t2 = (qh - ((c >> 4) & 0x0f0f0f0fu)) ^ 0x80808080u;
-        acc = sdot4(t1, t1, acc);
-        return sdot4(t2, t2, acc);
-    } else {  // V_SQ2U
-#pragma unroll
-        for (int s = 0; s < 8; s += 2) {
-            uint32_t qs = ((q >> s) & 0x03030303u) | 0x80808080u;
-            uint32_t t = (qs - ((c >> s) & 0x03030303u)) ^ 0x80808080u;
-            acc = sdot4(t, t, acc);
-        }
-        return acc;
-    }
+__device__ __forceinline__ uint32_t udot4(uint32_t a, uint32_t b, uint32_t c) {
+    return __builtin_amdgcn_udot4(a, b, c, false);
+}
+__device__ __forceinline__ uint32_t udot8(uint32_t a, uint32_t b, uint32_t c) {
+    return __builtin_amdgcn_udot8(a, b, c, false);
 }
 
 template <int V, int DPL>
 struct IntQuery {
     uint32_t q[DPL];
+    int32_t qq;      // sum x^2 over this lane's slice (squared-L2 variants)
     int32_t slot;    // lanes per candidate (power of two)
     int32_t nparts;  // active parts per candidate
     int32_t part;    // this lane's part
@@ -234,11 +224,20 @@ struct IntQuery {
         while (slot < nparts) slot <<= 1;
         part = lane_id() & (slot - 1);
         const uint32_t *qd = (const uint32_t *)w.qcode;
+        uint32_t acc = 0;
 #pragma unroll
         for (int i = 0; i < DPL; ++i) {
             int d = part * DPL + i;
             q[i] = (part < nparts && d < code_dw) ? qd[d] : 0u;
+            if (V == V_SQ8U_L2) acc = udot4(q[i], q[i], acc);
+            if (V == V_SQ4U) acc = udot8(q[i], q[i], acc);
+            if (V == V_SQ2U) {
+                const uint32_t a0 = q[i] & 0x33333333u, a1 = (q[i] >> 2) & 0x33333333u;
+                acc = udot8(a0, a0, acc);
+                acc = udot8(a1, a1, acc);
+            }
         }
+        qq = (int32_t)acc;
     }
 
     // Distances of candidates ci[0..m) -> keys ck[0..m).
@@ -253,19 +252,38 @@ struct IntQuery {
             if (j < m && part < nparts) {
                 const uint32_t id = w.ci[j];
                 const uint4 *src = (const uint4 *)(ix.codes + (int64_t)id * ix.code_size);
-                uint4 c[DPL / 4];
+                uint4 c4[DPL / 4];
 #pragma unroll
                 for (int i = 0; i < DPL / 4; ++i) {
                     int qi = part * (DPL / 4) + i;
-                    c[i] = qi < code_q ? src[qi] : make_uint4(0, 0, 0, 0);
+                    c4[i] = qi < code_q ? src[qi] : make_uint4(0, 0, 0, 0);
                 }
+                uint32_t qc = 0, cc = 0;
+                int32_t ip = 0;
 #pragma unroll
                 for (int i = 0; i < DPL / 4; ++i) {
-                    acc = int_partial_dword<V>(q[4 * i + 0], c[i].x, acc);
-                    acc = int_partial_dword<V>(q[4 * i + 1], c[i].y, acc);
-                    acc = int_partial_dword<V>(q[4 * i + 2], c[i].z, acc);
-                    acc = int_partial_dword<V>(q[4 * i + 3], c[i].w, acc);
+                    const uint32_t cs[4] = {c4[i].x, c4[i].y, c4[i].z, c4[i].w};
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        const uint32_t qv = q[4 * i + r], c = cs[r];
+                        if (V == V_SQ8U_L2) {
+                            qc = udot4(qv, c, qc);
+                            cc = udot4(c, c, cc);
+                        } else if (V == V_SQ8U_IP) {
+                            ip = sdot4(qv, c, ip);
+                        } else if (V == V_SQ4U) {
+                            qc = udot8(qv, c, qc);
+                            cc = udot8(c, c, cc);
+                        } else {  // V_SQ2U
+                            const uint32_t c0 = c & 0x33333333u, c1 = (c >> 2) & 0x33333333u;
+                            qc = udot8(qv & 0x33333333u, c0, qc);
+                            qc = udot8((qv >> 2) & 0x33333333u, c1, qc);
+                            cc = udot8(c0, c0, cc);
+                            cc = udot8(c1, c1, cc);
+                        }
+                    }
                 }
+                acc = (V == V_SQ8U_IP) ? ip : (int32_t)((uint32_t)qq + cc - 2u * qc);
             }
             for (int off = slot >> 1; off >= 1; off >>= 1) acc += __shfl_xor(acc, off);
             if (j < m && part == 0) {
@@ -277,86 +295,112 @@ struct IntQuery {
 };
 
 // ---------------------------------------------------------------------------
-// Float distances: 16 lanes per candidate, lane l accumulates elements
-// j = l, l+16, ... in increasing j (the AVX-512 lane order), then the 16
-// partial sums are reduced in the reference's tree.
+// Float distances in the reference's AVX-512 order.  The compiled reference
+// keeps 16 accumulators, acc[l] summing elements j = l, l+16, ... in
+// increasing j, then reduces the 16 partial sums in a fixed tree.  Here four
+// lanes share one row: lane p owns accumulators 4p..4p+3 and reads elements
+// 16t + 4p .. 16t + 4p + 3 (one dword of codes or one float4) per step t, so
+// every accumulator sees exactly the reference's operand sequence, and 16
+// rows are evaluated per pass.
 //   E5M2  glass::L2SqrE5M2 / IPE5M2   (glass/simd/avx512.hpp:15-38)
 //   SQ8   glass::L2SqrSQ8_ext / IPSQ8_ext (glass/simd/avx512.hpp:40-77)
 //   FP32  helpa::l2_fp32_fp32 / dot_fp32_fp32 (x86/l2_impl.hpp:50-71, x86/dot_impl.hpp:52-71)
 // The fused multiply-adds are where the compiled reference fuses them.
 
-// glass::reduce_add_f32x16 (glass/simd/avx512_utils.hpp:12-22): (c0+c1)+(c2+c3)
-__device__ __forceinline__ float reduce16_glass(float a) {
-    a = a + __shfl_xor(a, 8);
-    a = a + __shfl_xor(a, 4);
-    a = a + __shfl_xor(a, 1);
-    a = a + __shfl_xor(a, 2);
-    return a;
-}
+struct Acc4 {
+    float a[4];
+};
 
-// GCC _mm512_reduce_add_ps as used by helpa (x86/utils.hpp:61-64): (c0+c2)+(c1+c3)
-__device__ __forceinline__ float reduce16_helpa(float a) {
-    a = a + __shfl_xor(a, 8);
-    a = a + __shfl_xor(a, 4);
-    a = a + __shfl_xor(a, 2);
-    a = a + __shfl_xor(a, 1);
-    return a;
+// Tree over the 16 partial sums held 4 per lane by a 4-lane group.
+//   glass::reduce_add_f32x16 (glass/simd/avx512_utils.hpp:12-22):
+//     b_i = a_i + a_{i+8}, c_i = b_i + b_{i+4}, result (c0+c1)+(c2+c3)
+//   GCC _mm512_reduce_add_ps as helpa uses it (x86/utils.hpp:61-64):
+//     same b and c, result (c0+c2)+(c1+c3)
+// Lane p holds a_{4p+r}; a_{i+8} sits two lanes up, b_{i+4} one lane up.
+template <bool kGlassTree>
+__device__ __forceinline__ float reduce16_4lanes(Acc4 v) {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) v.a[r] = v.a[r] + __shfl_xor(v.a[r], 2);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) v.a[r] = v.a[r] + __shfl_xor(v.a[r], 1);
+    return kGlassTree ? (v.a[0] + v.a[1]) + (v.a[2] + v.a[3]) : (v.a[0] + v.a[2]) + (v.a[1] + v.a[3]);
 }
 
 __device__ __forceinline__ float e5m2_to_float(uint32_t byte) {
     return __half2float(__ushort_as_half((unsigned short)(byte << 8)));
 }
 
-// One float candidate row, lanes of a 16-lane segment; returns the reduced sum.
+// One row shared by a 4-lane group (p = lane & 3); returns the reduced sum in
+// every lane of the group.
 template <int V>
-__device__ __forceinline__ float float_row(const DeviceIndex &ix, const WaveLds &w, const uint8_t *row, int l16,
-                                           int32_t mode, int32_t da) {
-    float acc = 0.0f;
+__device__ __forceinline__ float float_row4(const DeviceIndex &ix, const WaveLds &w, const uint8_t *row, int p,
+                                            int32_t mode, int32_t da) {
+    Acc4 acc = {{0.0f, 0.0f, 0.0f, 0.0f}};
     if (V == V_E5M2_L2 || V == V_E5M2_IP) {
-        for (int j = l16; j < da; j += 16) {
-            float y = e5m2_to_float(row[j]);
-            float x = w.qf[j];
-            if (V == V_E5M2_L2) {
-                float t = x - y;
-                acc = fmaf(t, t, acc);
-            } else {
-                acc = fmaf(x, y, acc);
+#pragma unroll 4
+        for (int t = 0; t < da; t += 16) {
+            const int j = t + 4 * p;
+            const uint32_t c = *(const uint32_t *)(row + j);
+            const float4 x = *(const float4 *)(w.qf + j);
+            const float xs[4] = {x.x, x.y, x.z, x.w};
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const float y = e5m2_to_float((c >> (8 * r)) & 0xffu);
+                if (V == V_E5M2_L2) {
+                    const float d = xs[r] - y;
+                    acc.a[r] = fmaf(d, d, acc.a[r]);
+                } else {
+                    acc.a[r] = fmaf(xs[r], y, acc.a[r]);
+                }
             }
         }
-        return reduce16_glass(acc);
+        return reduce16_4lanes<true>(acc);
     } else if (V == V_SQ8_L2 || V == V_SQ8_IP) {
         const int ql = qf_len(ix);
         const float *m256 = w.qm + ql, *dif = w.qm + 2 * ql;
-        for (int j = l16; j < da; j += 16) {
-            float c = (float)row[j] + 0.5f;
-            if (V == V_SQ8_L2) {
-                float t;
-                if (mode == MODE_BATCH) {
-                    // SearchImpl2 context: yy = fma(c, dif, 256min), t = 256x - yy
-                    t = w.qf[j] * 256.0f - fmaf(c, dif[j], m256[j]);
+#pragma unroll 4
+        for (int t = 0; t < da; t += 16) {
+            const int j = t + 4 * p;
+            const uint32_t c4 = *(const uint32_t *)(row + j);
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const float c = (float)((c4 >> (8 * r)) & 0xffu) + 0.5f;
+                if (V == V_SQ8_L2) {
+                    float d;
+                    if (mode == MODE_BATCH) {
+                        // SearchImpl2 context: yy = fma(c, dif, 256min), d = 256x - yy
+                        d = w.qf[j + r] * 256.0f - fmaf(c, dif[j + r], m256[j + r]);
+                    } else {
+                        // Search / bare computer context: d = fma(-c, dif, 256x - 256min)
+                        d = fmaf(-c, dif[j + r], w.qm[j + r]);
+                    }
+                    acc.a[r] = fmaf(d, d, acc.a[r]);
                 } else {
-                    // Search / bare computer context: t = fma(-c, dif, 256x - 256min)
-                    t = fmaf(-c, dif[j], w.qm[j]);
+                    const float yy = fmaf(c, dif[j + r], m256[j + r]);
+                    acc.a[r] = fmaf(w.qf[j + r], yy, acc.a[r]);
                 }
-                acc = fmaf(t, t, acc);
-            } else {
-                float yy = fmaf(c, dif[j], m256[j]);
-                acc = fmaf(w.qf[j], yy, acc);
             }
         }
-        return reduce16_glass(acc);
+        return reduce16_4lanes<true>(acc);
     } else {  // FP32
         const float *y = (const float *)row;
-        for (int j = l16; j < da; j += 16) {
-            float x = w.qf[j];
-            if (V == V_FP32_L2) {
-                float t = x - y[j];
-                acc = fmaf(t, t, acc);
-            } else {
-                acc = fmaf(x, y[j], acc);
+#pragma unroll 4
+        for (int t = 0; t < da; t += 16) {
+            const int j = t + 4 * p;
+            const float4 yv = *(const float4 *)(y + j);
+            const float4 x = *(const float4 *)(w.qf + j);
+            const float xs[4] = {x.x, x.y, x.z, x.w}, ys[4] = {yv.x, yv.y, yv.z, yv.w};
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                if (V == V_FP32_L2) {
+                    const float d = xs[r] - ys[r];
+                    acc.a[r] = fmaf(d, d, acc.a[r]);
+                } else {
+                    acc.a[r] = fmaf(xs[r], ys[r], acc.a[r]);
+                }
             }
         }
-        return reduce16_helpa(acc);
+        return reduce16_4lanes<false>(acc);
     }
 }
 
@@ -364,16 +408,102 @@ template <int V>
 __device__ void float_distances(const DeviceIndex &ix, const WaveLds &w, int m, int32_t mode) {
     if (m <= 0) return;
     const int l = lane_id();
-    const int l16 = l & 15, cj = l >> 4;
-    for (int base = 0; base < m; base += 4) {
+    const int p = l & 3, cj = l >> 2;
+    for (int base = 0; base < m; base += 16) {
         const int j = base + cj;
         const int jj = j < m ? j : (m - 1);
         const uint32_t id = w.ci[jj];
-        float r = float_row<V>(ix, w, ix.codes + (int64_t)id * ix.code_size, l16, mode, ix.dim_align);
+        float r = float_row4<V>(ix, w, ix.codes + (int64_t)id * ix.code_size, p, mode, ix.dim_align);
         bool neg = (V == V_E5M2_IP || V == V_SQ8_IP || V == V_FP32_IP);
-        if (j < m && l16 == 0) w.ck[j] = key_of_float(neg ? -r : r);
+        if (j < m && p == 0) w.ck[j] = key_of_float(neg ? -r : r);
     }
 }
+
+// ---------------------------------------------------------------------------
+// Exact visited set (Bitset, glass/neighbor.hpp:31-79) as an open-addressing
+// hash table in LDS holding id+1.  Ids tested together come from one
+// deduplicated adjacency row, so they are distinct: each lane either finds
+// its id or claims an empty slot with a compare-and-swap.  The table is kept
+// below half full; a query that would exceed that switches to the slot's
+// global bitset for the ids it has not seen yet (the bitset is cleared once
+// at the switch), and keeps consulting the table for the ids inserted before.
+struct VisitedSet {
+    uint32_t *tab;
+    uint32_t mask, shift;
+    int32_t count, thr;
+    bool global;
+    uint32_t *gbits;
+    int64_t gwords;
+
+    __device__ void reset(uint32_t *t, int32_t size, uint32_t *g, int64_t gw) {
+        tab = t;
+        mask = (uint32_t)size - 1u;
+        shift = 32u - (uint32_t)__builtin_ctz((uint32_t)size);
+        count = 0;
+        thr = size / 2;
+        global = false;
+        gbits = g;
+        gwords = gw;
+        uint4 *t4 = (uint4 *)t;
+        for (int i = lane_id(); i < size / 4; i += 64) t4[i] = make_uint4(0, 0, 0, 0);
+        wave_sync();
+    }
+
+    __device__ __forceinline__ uint32_t home(uint32_t v) const { return (v * 0x9E3779B1u) >> shift; }
+
+    // true when v had not been visited; marks it visited.
+    __device__ bool test_set(uint32_t v, bool active) {
+        bool isnew = false;
+        if (active) {
+            const uint32_t key = v + 1u;
+            uint32_t h = home(v);
+            if (!global) {
+                for (;;) {
+                    const uint32_t old = atomicCAS(&tab[h], 0u, key);
+                    if (old == 0u) {
+                        isnew = true;
+                        break;
+                    }
+                    if (old == key) break;
+                    h = (h + 1u) & mask;
+                }
+            } else {
+                bool found = false;
+                for (;;) {
+                    const uint32_t cur = tab[h];
+                    if (cur == key) {
+                        found = true;
+                        break;
+                    }
+                    if (cur == 0u) break;
+                    h = (h + 1u) & mask;
+                }
+                if (!found) {
+                    const uint32_t bit = 1u << (v & 31u);
+                    const uint32_t old =
+                        __hip_atomic_fetch_or(&gbits[v >> 5], bit, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                    isnew = !(old & bit);
+                }
+            }
+        }
+        return isnew;
+    }
+
+    // Account for the ids a row inserted; switch to the global bitset before
+    // the table could pass half full (a row inserts at most 64 ids).
+    __device__ void account(int inserted) {
+        if (global) return;
+        count += inserted;
+        if (count > thr - 64) {
+            global = true;
+            uint4 *g4 = (uint4 *)gbits;
+            for (int64_t i = lane_id(); i < gwords / 4; i += 64) g4[i] = make_uint4(0, 0, 0, 0);
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+            __builtin_amdgcn_s_waitcnt(0);
+            wave_sync();
+        }
+    }
+};
 
 // ---------------------------------------------------------------------------
 // Pool operations.
@@ -422,7 +552,8 @@ __device__ __forceinline__ int upper_bound_lds(const uint32_t *pk, int n, uint32
 
 // Merge candidates ck/ci[0..m) (in insertion order) into the pool.  Equivalent
 // to m sequential LinearPool::insert calls (neighbor.hpp:382-396).
-__device__ void pool_merge(const WaveLds &w, PoolState &ps, int m) {
+// General form for any capacity: binary search and chunked shifts in LDS.
+__device__ void pool_merge_lds(const WaveLds &w, PoolState &ps, int m) {
     const int l = lane_id();
     uint32_t kj = l < m ? w.ck[l] : kKeyMax;
     uint32_t ij = l < m ? w.ci[l] : 0u;
@@ -492,6 +623,107 @@ __device__ void pool_merge(const WaveLds &w, PoolState &ps, int m) {
     ps.cur = new_cur;
 }
 
+// Pools of up to kMaxChunks*64 entries are merged from registers: lane l
+// holds entries l, l+64, ... of the pool.  Each candidate's insertion point
+// (entries with key <= its key) is a sum of ballots, the first moved entry
+// and the first inserted position come from the rank-0 candidate, and all
+// writes happen after all reads.  The node the next pop will take is known
+// at that point, so its adjacency row load is started here (pred_row) and
+// overlaps the rest of the merge and the pop.
+constexpr int kMaxChunks = 4;
+
+__device__ uint32_t pool_merge_reg(const DeviceIndex &ix, const WaveLds &w, PoolState &ps, int m,
+                                   int32_t &pred_row) {
+    const int l = lane_id();
+    const uint32_t kj = l < m ? w.ck[l] : kKeyMax;
+    const uint32_t ij = l < m ? w.ci[l] : 0u;
+    const int nch = (ps.size + 63) >> 6;
+    uint32_t pkc[kMaxChunks], pic[kMaxChunks];
+#pragma unroll
+    for (int c = 0; c < kMaxChunks; ++c) {
+        const int p = 64 * c + l;
+        const bool v = c < nch && p < ps.size;
+        pkc[c] = v ? w.pk[p] : kKeyMax;
+        pic[c] = v ? w.pid[p] : 0u;
+    }
+    bool act = l < m;
+    // insert() rejects d >= data[size-1] when full; such a candidate would
+    // also land past cap in the merge and displaces nothing.
+    if (ps.size == ps.cap) act = act && (kj < w.pk[ps.size - 1]);
+    const uint64_t am = ballot(act);
+    int new_size = ps.size, new_cur = ps.cur, np0 = 0x7fffffff;
+    uint32_t id0 = 0;
+    int r = 0, ub = 0;
+    if (am) {
+        // rank among the active candidates by (key, insertion order) and
+        // upper bound in the pool (new keys go after equal old keys)
+        uint64_t mm = am;
+        while (mm) {
+            const int i = __builtin_ctzll(mm);
+            mm &= mm - 1;
+            const uint32_t ki = (uint32_t)__builtin_amdgcn_readlane((int)kj, i);
+            r += (ki < kj) | ((ki == kj) & (i < l));
+            int cnt = 0;
+#pragma unroll
+            for (int c = 0; c < kMaxChunks; ++c) {
+                if (c < nch) cnt += __builtin_popcountll(ballot(pkc[c] <= ki));
+            }
+            ub = (l == i) ? cnt : ub;
+        }
+        const int i0 = __builtin_ctzll(ballot(act && r == 0));
+        const int ub0 = __builtin_amdgcn_readlane(ub, i0);
+        id0 = (uint32_t)__builtin_amdgcn_readlane((int)ij, i0);
+        np0 = ub0 < ps.cap ? ub0 : 0x7fffffff;
+        const int ma = __builtin_popcountll(am);
+        new_size = ps.size + ma < ps.cap ? ps.size + ma : ps.cap;
+        new_cur = 0x7fffffff;
+        if (ps.cur < ps.size) {
+            const uint32_t kc = w.pk[ps.cur];
+            new_cur = ps.cur + __builtin_popcountll(ballot(act && kj < kc));
+        }
+        new_cur = new_cur < np0 ? new_cur : np0;
+        new_cur = new_cur < new_size ? new_cur : new_size;
+    }
+    // the next pop takes entry new_cur: the rank-0 candidate or the old first
+    // unchecked entry
+    uint32_t next = kIdMask;
+    if (new_cur < new_size && new_cur < ps.ef) next = (new_cur == np0) ? id0 : (w.pid[ps.cur] & kIdMask);
+    if (next != kIdMask) pred_row = l < ix.K ? ix.adj[(int64_t)next * ix.K + l] : -1;
+    if (am) {
+        const int lo = np0 < ps.cap ? np0 : ps.cap;  // entries before the first insertion do not move
+        // shift of each pool entry = number of candidates with a smaller key
+        int sh[kMaxChunks];
+#pragma unroll
+        for (int c = 0; c < kMaxChunks; ++c) sh[c] = 0;
+        uint64_t m2 = am;
+        while (m2) {
+            const int i = __builtin_ctzll(m2);
+            m2 &= m2 - 1;
+            const uint32_t ki = (uint32_t)__builtin_amdgcn_readlane((int)kj, i);
+#pragma unroll
+            for (int c = 0; c < kMaxChunks; ++c) sh[c] += ki < pkc[c];
+        }
+        wave_sync();
+#pragma unroll
+        for (int c = 0; c < kMaxChunks; ++c) {
+            const int p = 64 * c + l;
+            const int dst = p + sh[c];
+            if (c < nch && p >= lo && p < ps.size && dst < ps.cap) {
+                w.pk[dst] = pkc[c];
+                w.pid[dst] = pic[c];
+            }
+        }
+        if (act && r + ub < ps.cap) {
+            w.pk[r + ub] = kj;
+            w.pid[r + ub] = ij;
+        }
+        wave_sync();
+        ps.size = new_size;
+        ps.cur = new_cur;
+    }
+    return next;
+}
+
 // ---------------------------------------------------------------------------
 // The search kernel.
 
@@ -500,8 +732,8 @@ __global__ __launch_bounds__(256) void search_kernel(DeviceIndex ix, QueryBatch 
     extern __shared__ __attribute__((aligned(16))) uint8_t lds_raw[];
     const int l = lane_id();
     const int wv = threadIdx.x >> 6;
-    const size_t wbytes = wave_lds_bytes(ix, lc.cap_max);
-    const WaveLds w = carve(lds_raw + (size_t)wv * wbytes, ix, lc.cap_max);
+    const size_t wbytes = wave_lds_bytes(ix, lc.cap_max, lc.vis_hash);
+    const WaveLds w = carve(lds_raw + (size_t)wv * wbytes, ix, lc.cap_max, lc.vis_hash);
     const int slot = blockIdx.x * lc.waves_per_block + wv;
     uint32_t *vis = ss.visited + (int64_t)slot * ss.vis_stride;
     const int ql = qf_len(ix);
@@ -550,14 +782,9 @@ __global__ __launch_bounds__(256) void search_kernel(DeviceIndex ix, QueryBatch 
             wave_sync();
             iq.init(ix, w);
         }
-        // clear this slot's visited bitset (Bitset::reset, neighbor.hpp:67-74)
-        {
-            uint4 *v4 = (uint4 *)vis;
-            const int64_t n4 = ss.vis_stride / 4;
-            for (int64_t i = l; i < n4; i += 64) v4[i] = make_uint4(0, 0, 0, 0);
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-        }
-        wave_sync();
+        // empty visited set (Bitset::reset, neighbor.hpp:67-74)
+        VisitedSet vs;
+        vs.reset(w.vt, lc.vis_hash, vis, ss.vis_stride);
 
         auto dists = [&](int m) {
             if (is_int_variant(V)) {
@@ -605,8 +832,9 @@ __global__ __launch_bounds__(256) void search_kernel(DeviceIndex ix, QueryBatch 
             if (l == 0) {
                 w.pk[0] = du;
                 w.pid[0] = u;
-                __hip_atomic_fetch_or(&vis[u >> 5], 1u << (u & 31), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
             }
+            vs.test_set(u, l == 0);
+            vs.account(1);
             ps.size = 1;
             ps.cur = 0;
             wave_sync();
@@ -617,35 +845,53 @@ __global__ __launch_bounds__(256) void search_kernel(DeviceIndex ix, QueryBatch 
             wave_sync();
             dists(nep);
             cnt_d += nep;
-            pool_merge(w, ps, nep);
-            if (l < nep) {
-                uint32_t e = (uint32_t)ix.eps[l];
-                __hip_atomic_fetch_or(&vis[e >> 5], 1u << (e & 31), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            if (lc.cap_max <= 64 * kMaxChunks) {
+                int32_t unused = -1;
+                pool_merge_reg(ix, w, ps, nep, unused);
+            } else {
+                pool_merge_lds(w, ps, nep);
             }
+            vs.test_set(l < nep ? (uint32_t)ix.eps[l] : 0u, l < nep);
+            vs.account(nep);
         }
 
-        // best-first expansion (SearchImpl2, graph_searcher.hpp:196-224)
+        // best-first expansion (SearchImpl2, graph_searcher.hpp:196-224).
+        // After each pop the adjacency row of the pool's next unchecked entry
+        // is loaded speculatively, in flight together with the current
+        // candidates' code gathers; the next pop uses it when it pops that
+        // node (the merge did not insert anything closer), else reloads.
+        uint32_t pred = kIdMask;
+        int32_t pred_row = -1;
         while (ps.cur < ps.size && ps.cur < ps.ef) {
             const uint32_t u = pool_pop(w, ps);
             cnt_e += 1;
-            const int32_t v = l < ix.K ? ix.adj[(int64_t)u * ix.K + l] : -1;
+            const int32_t v = (u == pred) ? pred_row : (l < ix.K ? ix.adj[(int64_t)u * ix.K + l] : -1);
             const uint64_t inv = ballot(l < ix.K && v == -1);
             const int n = inv ? __builtin_ctzll(inv) : ix.K;
-            bool isnew = false;
-            if (l < n) {
-                const uint32_t bit = 1u << (v & 31);
-                const uint32_t old = __hip_atomic_fetch_or(&vis[(uint32_t)v >> 5], bit, __ATOMIC_RELAXED,
-                                                           __HIP_MEMORY_SCOPE_WORKGROUP);
-                isnew = !(old & bit);
-            }
+            const bool isnew = vs.test_set((uint32_t)v, l < n);
             const uint64_t nm = ballot(isnew);
-            if (nm == 0) continue;
+            if (nm == 0) {
+                // nothing new: the pool is unchanged and the next pop takes the
+                // first unchecked entry
+                pred = kIdMask;
+                if (ps.cur < ps.size && ps.cur < ps.ef) {
+                    pred = w.pid[ps.cur] & kIdMask;
+                    pred_row = l < ix.K ? ix.adj[(int64_t)pred * ix.K + l] : -1;
+                }
+                continue;
+            }
             const int m = __builtin_popcountll(nm);
+            vs.account(m);
             if (isnew) w.ci[mbcnt(nm)] = (uint32_t)v;
             wave_sync();
             dists(m);
             cnt_d += m;
-            pool_merge(w, ps, m);
+            if (lc.cap_max <= 64 * kMaxChunks) {
+                pred = pool_merge_reg(ix, w, ps, m, pred_row);
+            } else {
+                pool_merge_lds(w, ps, m);
+                pred = kIdMask;
+            }
         }
 
         // outputs
@@ -666,24 +912,16 @@ __global__ __launch_bounds__(256) void search_kernel(DeviceIndex ix, QueryBatch 
             // Refiner::RefineImpl (refiner.hpp:92-110): exact FP32 distances of the
             // first reorder_k pool entries, the k smallest by (dist, id) ascending.
             const int m = kk < ps.size ? kk : ps.size;
-            const int l16 = l & 15, cj = l >> 4;
-            for (int base = 0; base < m; base += 4) {
-                const int j = base + cj;
+            const int p4 = l & 3, rj = l >> 2;
+            for (int base = 0; base < m; base += 16) {
+                const int j = base + rj;
                 const int jj = j < m ? j : m - 1;
                 const uint32_t id = w.pid[jj] & kIdMask;
-                float acc = 0.0f;
-                const float *y = ix.rerank + (int64_t)id * ix.r_dim_align;
-                if (ix.metric == MK_L2) {
-                    for (int t = l16; t < ix.r_dim_align; t += 16) {
-                        float dd = w.qf[t] - y[t];
-                        acc = fmaf(dd, dd, acc);
-                    }
-                } else {
-                    for (int t = l16; t < ix.r_dim_align; t += 16) acc = fmaf(w.qf[t], y[t], acc);
-                }
-                float rsum = reduce16_helpa(acc);
+                const uint8_t *row = (const uint8_t *)(ix.rerank + (int64_t)id * ix.r_dim_align);
+                const float rsum = ix.metric == MK_L2 ? float_row4<V_FP32_L2>(ix, w, row, p4, qb.mode, ix.r_dim_align)
+                                                      : float_row4<V_FP32_IP>(ix, w, row, p4, qb.mode, ix.r_dim_align);
                 wave_sync();
-                if (j < m && l16 == 0) w.pk[j] = key_of_float(ix.metric == MK_L2 ? rsum : -rsum);
+                if (j < m && p4 == 0) w.pk[j] = key_of_float(ix.metric == MK_L2 ? rsum : -rsum);
                 wave_sync();
             }
             wave_sync();
@@ -716,7 +954,7 @@ __global__ __launch_bounds__(256) void search_kernel(DeviceIndex ix, QueryBatch 
 template <int V, int DPL>
 hipError_t launch_one(const DeviceIndex &ix, const QueryBatch &qb, const LaunchConfig &lc, const SlotState &ss,
                       hipStream_t stream) {
-    const size_t lds = wave_lds_bytes(ix, lc.cap_max) * (size_t)lc.waves_per_block;
+    const size_t lds = wave_lds_bytes(ix, lc.cap_max, lc.vis_hash) * (size_t)lc.waves_per_block;
     static bool attr_set = false;
     if (!attr_set) {
         (void)hipFuncSetAttribute((const void *)search_kernel<V, DPL>, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -728,13 +966,21 @@ hipError_t launch_one(const DeviceIndex &ix, const QueryBatch &qb, const LaunchC
     return hipGetLastError();
 }
 
+// Resident workgroups per CU from the kernel's register count and the CDNA
+// limits: 4 SIMDs x 512 VGPRs per lane (granule 8), at most 8 waves per SIMD,
+// 160 KiB of LDS per CU.
 template <int V, int DPL>
 int occupancy_one(const DeviceIndex &ix, const LaunchConfig &lc) {
-    const size_t lds = wave_lds_bytes(ix, lc.cap_max) * (size_t)lc.waves_per_block;
-    int nb = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, (const void *)search_kernel<V, DPL>, 64 * lc.waves_per_block,
-                                                     lds) != hipSuccess)
-        return 0;
+    const size_t lds = wave_lds_bytes(ix, lc.cap_max, lc.vis_hash) * (size_t)lc.waves_per_block;
+    hipFuncAttributes attr;
+    if (hipFuncGetAttributes(&attr, (const void *)search_kernel<V, DPL>) != hipSuccess) return 0;
+    int vgpr = attr.numRegs > 0 ? attr.numRegs : 128;
+    vgpr = (vgpr + 7) / 8 * 8;
+    int waves_simd = 512 / vgpr;
+    if (waves_simd > 8) waves_simd = 8;
+    int by_regs = 4 * waves_simd / lc.waves_per_block;
+    int by_lds = lds ? (int)((160 * 1024) / lds) : by_regs;
+    int nb = by_regs < by_lds ? by_regs : by_lds;
     return nb;
 }
 
@@ -761,7 +1007,39 @@ hipError_t launch_int(const DeviceIndex &ix, const QueryBatch &qb, const LaunchC
 
 }  // namespace
 
-size_t lds_bytes_per_wave(const DeviceIndex &ix, int32_t cap_max) { return wave_lds_bytes(ix, cap_max); }
+size_t lds_bytes_per_wave(const DeviceIndex &ix, int32_t cap_max, int32_t vis_hash) {
+    return wave_lds_bytes(ix, cap_max, vis_hash);
+}
+
+template <int V, int DPL>
+int vgprs_one() {
+    hipFuncAttributes attr;
+    if (hipFuncGetAttributes(&attr, (const void *)search_kernel<V, DPL>) != hipSuccess) return 0;
+    return attr.numRegs;
+}
+
+template <int V>
+int vgprs_int(int team) {
+    switch (team) {
+        case 4: return vgprs_one<V, 4>();
+        case 8: return vgprs_one<V, 8>();
+        case 16: return vgprs_one<V, 16>();
+        default: return 0;
+    }
+}
+
+int kernel_vgprs(const DeviceIndex &ix, const LaunchConfig &lc) {
+    const bool l2 = ix.metric == MK_L2;
+    switch (ix.quant) {
+        case QK_SQ8U: return l2 ? vgprs_int<V_SQ8U_L2>(lc.team) : vgprs_int<V_SQ8U_IP>(lc.team);
+        case QK_SQ4U: return vgprs_int<V_SQ4U>(lc.team);
+        case QK_SQ2U: return vgprs_int<V_SQ2U>(lc.team);
+        case QK_E5M2: return l2 ? vgprs_one<V_E5M2_L2, 4>() : vgprs_one<V_E5M2_IP, 4>();
+        case QK_SQ8: return l2 ? vgprs_one<V_SQ8_L2, 4>() : vgprs_one<V_SQ8_IP, 4>();
+        case QK_FP32: return l2 ? vgprs_one<V_FP32_L2, 4>() : vgprs_one<V_FP32_IP, 4>();
+        default: return 0;
+    }
+}
 
 int max_blocks_per_cu(const DeviceIndex &ix, const LaunchConfig &lc) {
     const bool l2 = ix.metric == MK_L2;
