@@ -173,6 +173,48 @@ def gpu_sweep(s, q_dev, nq, k, gt, ids_dev, d_dev, cnt_dev, args, torch, stream,
     return pts
 
 
+def served_bench(s, Q, gt, k, ef, factor, n_sat=200_000, n_load=400_000, loads=(0.25, 0.5, 0.75, 0.9)):
+    """Served form: single-query requests through the continuous-batching
+    server.  Saturation = completion rate with every request queued at t=0;
+    then Poisson arrivals (seed ARRIVAL_SEED) at fractions of it, with each
+    request's arrival-to-result latency on the host."""
+    import pyglass_amd as pg
+    from pyglass_amd import synth
+
+    s.set_refine_factor(factor)
+    srv = pg.Server(s, max_batch=8192, max_k=k, max_inflight=2)
+    nq = len(Q)
+    rows = np.arange(max(n_sat, n_load)) % nq
+    Qt = Q[rows]
+    srv.replay(Qt[:20000], 1, ef, k, np.zeros(20000, np.int64))  # warm-up
+    lat, ids, _ = srv.replay(Qt[:n_sat], 1, ef, k, np.zeros(n_sat, np.int64))
+    sat = n_sat / (lat.max() / 1e9)
+    rec = recall_at(ids.reshape(-1, k)[:nq], gt[: min(nq, n_sat)], k)
+    log(f"served: saturation {sat:.0f} QPS (ef={ef}, factor={factor}), recall {rec:.4f}, {srv.stats()}")
+    rng = np.random.default_rng(synth.ARRIVAL_SEED)
+    out = {"saturation_qps": sat, "ef": ef, "refine_factor": factor, "recall": rec, "max_batch": 8192,
+           "max_inflight": 2, "loads": []}
+    for f in loads:
+        rate = f * sat
+        arr = np.cumsum(rng.exponential(1e9 / rate, n_load)).astype(np.int64)
+        b0 = srv.stats()
+        lat, _, _ = srv.replay(Qt[:n_load], 1, ef, k, arr)
+        b1 = srv.stats()
+        lt = lat[n_load // 10:] / 1e6  # drop the first 10% (ramp-up)
+        span = (arr[-1] + lat[-1]) / 1e9
+        row = {"offered_fraction": f, "offered_qps": rate, "achieved_qps": n_load / span,
+               "p50_ms": float(np.percentile(lt, 50)), "p99_ms": float(np.percentile(lt, 99)),
+               "mean_batch": (b1["queries"] - b0["queries"]) / max(1, b1["batches"] - b0["batches"])}
+        out["loads"].append(row)
+        log(f"served: offered {rate:.0f} QPS -> p50 {row['p50_ms']:.3f} ms, p99 {row['p99_ms']:.3f} ms, "
+            f"mean batch {row['mean_batch']:.0f}")
+    srv.close()
+    half = next(r for r in out["loads"] if r["offered_fraction"] == 0.5)
+    out["p50_ms_at_half_saturation"] = half["p50_ms"]
+    out["p99_ms_at_half_saturation"] = half["p99_ms"]
+    return out
+
+
 class stdout_to_stderr:
     """Route C-level stdout (the reference's printf progress lines) to stderr
     so stdout carries only the JSON result."""
@@ -237,9 +279,15 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-served", action="store_true", help="skip the served-form (continuous batching) run")
+    ap.add_argument("--served", type=str, default=None,
+                    help="FACTOR:EF - run only the served-form measurement at this operating point")
     ap.add_argument("--small", action="store_true", help="100k-row version for quick checks")
     ap.add_argument("--plain", type=int, default=200, help="queries timed in the plain one-query form")
     ap.add_argument("--calib-copy", action="store_true", help="with --point: also run a 512 MiB device copy")
+    ap.add_argument("--launch", type=str, default=None,
+                    help="TEAM:WAVES_PER_BLOCK:VISITED_ENTRIES - pin the launch shape after optimize() "
+                         "(profiler runs perturb the autotuner's timings)")
     ap.add_argument("--point", type=str, default=None,
                     help="FACTOR:EF - time only this operating point (for profiler runs)")
     args = ap.parse_args()
@@ -272,6 +320,10 @@ def main():
     t = time.time()
     s = pg.Searcher(pg.Graph(gpath), X, "L2", "SQ4U", "FP32")
     s.optimize()
+    if args.launch:
+        lt, lw, lv = (int(v) for v in args.launch.split(":"))
+        s.set_launch(lt, lw, 0)
+        s.set_visited_capacity(lv)
     launch = s.get_launch()
     info = s.info()
     log(f"searcher: set_data+optimize {time.time() - t:.1f}s, launch {launch}, info {info}")
@@ -286,6 +338,10 @@ def main():
     # the reference's default re-rank depth (factor 1.5 for SQ4U) plateaus
     # below recall 0.95 on this workload; factors 4 and 8 are swept for the
     # headline (same knob as Refiner's constructor argument)
+    if args.served:
+        sf, se = args.served.split(":")
+        print(json.dumps(served_bench(s, Q, gt, k, int(se), float(sf))))
+        return
     if args.point:
         pf, pe = args.point.split(":")
         pts = gpu_sweep(s, q_dev, nq, k, gt, ids_dev, d_dev, cnt_dev, args, torch, stream, float(pf), [int(pe)])
@@ -346,6 +402,7 @@ def main():
     plain_qps = npl / tt
     plain_recall = recall_at(plain_ids, gt[:npl], k)
     log(f"plain: {npl} queries, factor={best_f} ef={op['ef']}: {plain_qps:.0f} QPS, recall {plain_recall:.4f}")
+    served = None if args.no_served else served_bench(s, Q, gt, k, op["ef"], best_f)
 
     cpu = None
     parity = None
@@ -395,6 +452,7 @@ def main():
             "plain_one_query_per_launch": {"qps": plain_qps, "recall": plain_recall, "queries": npl,
                                            "ef": op["ef"]},
             "batched_over_plain": (value / plain_qps) if value else None,
+            "served": served,
             "batched_over_cpu": (value / cpu_value) if (value and cpu_value) else None,
             "ids_equal_cpu_reference": parity,
             "gt_fp64_recheck": {"checked": nchk, "mismatches": nbad},

@@ -144,6 +144,41 @@ int glass_hip_searcher_get_calibration(const glass_hip_searcher *s, float *param
 /* Synchronize the searcher's stream. */
 int glass_hip_searcher_sync(glass_hip_searcher *s);
 
+/* ---- Served form: continuous batching ---------------------------------- */
+/* The reference serves one call at a time (python/bindings.cc:100-154: the
+ * GIL is released around Search/SearchBatch).  These entry points queue
+ * requests from any thread; a dispatcher thread packs whatever is queued (up
+ * to max_batch queries) into one launch as soon as one of max_inflight
+ * staging slots is free, so a lone request never waits for a batch to fill.
+ * Each query gets SearchBatch semantics with its request's ef and k; its
+ * result does not depend on what else is in the batch.  The searcher must not
+ * be reconfigured (set_ef/optimize/set_launch) while a server on it runs. */
+typedef struct glass_hip_server glass_hip_server;
+
+int glass_hip_server_create(glass_hip_searcher *s, int32_t max_batch, int32_t max_k, int32_t max_inflight,
+                            glass_hip_server **out);
+/* Queue nq query rows (copied) with their ef and k; *ticket identifies the request. */
+int glass_hip_server_submit(glass_hip_server *sv, const float *qs, int32_t nq, int32_t ef, int32_t k,
+                            int64_t *ticket);
+/* Non-blocking: *done = 1 and nq*k ids/scores written (ticket released) when
+ * the request has completed, else *done = 0. */
+int glass_hip_server_poll(glass_hip_server *sv, int64_t ticket, int32_t *dst, float *scores, int32_t *done);
+/* Blocking variant of poll. */
+int glass_hip_server_wait(glass_hip_server *sv, int64_t ticket, int32_t *dst, float *scores);
+/* Drop a request that has not been dispatched yet (GLASS_HIP_ESTATE once it has). */
+int glass_hip_server_cancel(glass_hip_server *sv, int64_t ticket);
+/* Offered-load driver: the calling thread queues request i (nq[i] rows of qs,
+ * in order, with ef[i], k[i]) arrival_ns[i] after the call starts (sorted),
+ * waits for all, and returns each request's arrival-to-result latency; the
+ * results of request i follow those of requests < i in dst / scores. */
+int glass_hip_server_replay(glass_hip_server *sv, const float *qs, int64_t nreq, const int32_t *nq, const int32_t *ef,
+                            const int32_t *k, const int64_t *arrival_ns, int64_t *latency_ns, int32_t *dst,
+                            float *scores);
+/* Batches launched, queries and requests completed so far. */
+int glass_hip_server_stats(glass_hip_server *sv, int64_t *batches, int64_t *queries, int64_t *requests);
+/* Stops the dispatcher after the batches in flight; queued requests fail. */
+void glass_hip_server_free(glass_hip_server *sv);
+
 #ifdef __cplusplus
 }
 #endif

@@ -509,6 +509,40 @@ static int32_t pool_pop(pool_t *p) {
 
 static int pool_has_next(const pool_t *p) { return p->cur < p->size && p->cur < p->ef; }
 
+int orq_pool_run(int32_t cap, int32_t ef, int32_t nops, const int32_t *ops, const double *dists, int32_t *popped,
+                 int32_t *out_ids, double *out_dists, int32_t *out_size, int32_t *out_cur) {
+    /* Drives one LinearPool through a sequence of operations, for model tests:
+     * ops[i] >= 0 inserts (ops[i], dists[i]); ops[i] < 0 pops when has_next()
+     * holds (popped[i] = id) and records -1 otherwise.  Final entries keep the
+     * checked bit (bit 31). */
+    if (cap < 1 || nops < 0) return -1;
+    pool_t p;
+    p.size = 0;
+    p.cur = 0;
+    p.ef = ef;
+    p.cap = cap;
+    p.dist = (double *)calloc((size_t)cap + 1, sizeof(double));
+    p.id = (int32_t *)calloc((size_t)cap + 1, sizeof(int32_t));
+    p.vis = NULL;
+    for (int32_t i = 0; i < nops; ++i) {
+        if (ops[i] >= 0) {
+            pool_insert(&p, ops[i], dists[i]);
+            popped[i] = -2;
+        } else {
+            popped[i] = pool_has_next(&p) ? pool_pop(&p) : -1;
+        }
+    }
+    for (int32_t i = 0; i < p.size; ++i) {
+        out_ids[i] = p.id[i];
+        out_dists[i] = p.dist[i];
+    }
+    *out_size = p.size;
+    *out_cur = p.cur;
+    free(p.dist);
+    free(p.id);
+    return 0;
+}
+
 /* ------------------------------------------------------------------ */
 /* Search */
 

@@ -85,6 +85,12 @@ void orq_search_many(const orq_graph *g, const orq_quant *q, const uint8_t *code
                      int32_t batch_mode, int32_t *ids, float *dists, int64_t *dist_cmps, int64_t *expansions,
                      int32_t nthreads);
 
+/* LinearPool driver for model tests: ops[i] >= 0 inserts (ops[i], dists[i]),
+ * ops[i] < 0 pops if has_next() (popped[i] = id, else -1; inserts record -2).
+ * Writes the final pool (ids with checked bit 31), its size and cur_. */
+int orq_pool_run(int32_t cap, int32_t ef, int32_t nops, const int32_t *ops, const double *dists, int32_t *popped,
+                 int32_t *out_ids, double *out_dists, int32_t *out_size, int32_t *out_cur);
+
 /* fp32 -> fp16 bits, round to nearest even (F16C semantics). */
 uint16_t orq_f32_to_f16(float f);
 float orq_f16_to_f32(uint16_t h);

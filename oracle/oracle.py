@@ -61,6 +61,8 @@ def lib():
     L.orq_prepare_query.argtypes = [C.POINTER(_Quant), vp, vp]
     L.orq_distance.argtypes = [C.POINTER(_Quant), vp, vp]
     L.orq_distance.restype = C.c_double
+    L.orq_distance_ctx.argtypes = [C.POINTER(_Quant), vp, vp, C.c_int32]
+    L.orq_distance_ctx.restype = C.c_double
     L.orq_graph_load.argtypes = [C.POINTER(_Graph), C.c_char_p]
     L.orq_graph_from_arrays.argtypes = [C.POINTER(_Graph), C.c_int32, C.c_int32, vp, C.c_int32, vp]
     L.orq_graph_free.argtypes = [C.POINTER(_Graph)]
@@ -72,6 +74,7 @@ def lib():
     L.orq_refine_factor.restype = C.c_float
     L.orq_search_many.argtypes = [C.POINTER(_Graph), C.POINTER(_Quant), vp, C.POINTER(_Quant), vp, C.c_float, vp,
                                   C.c_int64, C.c_int32, C.c_int32, C.c_int32, vp, vp, vp, vp, C.c_int32]
+    L.orq_pool_run.argtypes = [C.c_int32, C.c_int32, C.c_int32, vp, vp, vp, vp, vp, vp, vp]
     L.orq_f32_to_f16.argtypes = [C.c_float]
     L.orq_f32_to_f16.restype = C.c_uint16
     L.orq_f16_to_f32.argtypes = [C.c_uint16]
@@ -142,11 +145,17 @@ class Quant:
         self.L.orq_prepare_query(C.byref(self.s), _p(x), _p(buf))
         return buf
 
-    def distances(self, x: np.ndarray, ids) -> np.ndarray:
+    CTX = {"computer": 0, "search": 1, "batch": 2}
+
+    def distances(self, x: np.ndarray, ids, ctx: str = "computer") -> np.ndarray:
+        """Distances as the reference's computer evaluates them; ctx names the
+        calling context whose float contraction is followed ("computer",
+        "search" = Search's inlined loop, "batch" = SearchBatch's)."""
         qb = self.prepare_query(x)
         out = np.empty(len(ids), np.float64)
+        c = self.CTX[ctx]
         for j, i in enumerate(ids):
-            out[j] = self.L.orq_distance(C.byref(self.s), _p(qb), _p(self.codes[int(i)]))
+            out[j] = self.L.orq_distance_ctx(C.byref(self.s), _p(qb), _p(self.codes[int(i)]), c)
         return out
 
 
@@ -221,3 +230,29 @@ class Searcher:
 
     def get_last_search_dist_cmps(self) -> float:
         return self.last_dist_cmps
+
+
+def pool_run(cap: int, ef: int, ops, dists):
+    """Drives one oracle LinearPool: ops[i] >= 0 inserts (ops[i], dists[i]),
+    ops[i] < 0 pops when has_next() holds.  Returns (popped per op, final ids
+    with the checked bit, final dists, cur)."""
+    L = lib()
+    ops = np.ascontiguousarray(ops, np.int32)
+    dists = np.ascontiguousarray(dists, np.float64)
+    popped = np.zeros(len(ops), np.int32)
+    ids = np.zeros(cap, np.int32)
+    dd = np.zeros(cap, np.float64)
+    size = C.c_int32()
+    cur = C.c_int32()
+    if L.orq_pool_run(cap, ef, len(ops), _p(ops), _p(dists), _p(popped), _p(ids), _p(dd), C.byref(size),
+                      C.byref(cur)) != 0:
+        raise ValueError("bad pool arguments")
+    return popped, ids[:size.value], dd[:size.value], cur.value
+
+
+def f32_to_f16(x: float) -> int:
+    return int(lib().orq_f32_to_f16(float(x)))
+
+
+def f16_to_f32(h: int) -> float:
+    return float(lib().orq_f16_to_f32(int(h)))

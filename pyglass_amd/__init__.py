@@ -16,7 +16,7 @@ import numpy as np
 from . import _lib
 from ._lib import GlassError
 
-__all__ = ["Graph", "Searcher", "Index", "build_graph", "GlassError", "set_num_threads", "device_count", "version"]
+__all__ = ["Graph", "Searcher", "Server", "Index", "build_graph", "GlassError", "set_num_threads", "device_count", "version"]
 
 _NUM_THREADS = 0
 
@@ -299,3 +299,83 @@ class Searcher:
 
     def sync(self):
         _lib.check(self._L.glass_hip_searcher_sync(self._h))
+
+
+class Server:
+    """Served form over a Searcher: requests queue from any thread and a
+    dispatcher thread batches whatever is queued into one launch as soon as a
+    staging slot is free (glass_hip_server_* in include/glass_hip.h).  Each
+    query is answered with SearchBatch semantics at its request's ef and k."""
+
+    def __init__(self, searcher: Searcher, max_batch: int = 8192, max_k: int = 100, max_inflight: int = 2):
+        self._L = _lib.load()
+        self.searcher = searcher  # keeps the searcher alive while serving
+        self.dim = searcher.dim
+        h = C.c_void_p()
+        _lib.check(self._L.glass_hip_server_create(searcher._h, int(max_batch), int(max_k), int(max_inflight),
+                                                   C.byref(h)))
+        self._h = h
+        self._k = {}
+
+    def __del__(self):
+        self.close()
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.glass_hip_server_free(self._h)
+            self._h = None
+
+    def submit(self, queries, k: int, ef: int) -> int:
+        q = np.ascontiguousarray(queries, np.float32).reshape(-1, self.dim)
+        t = C.c_int64()
+        _lib.check(self._L.glass_hip_server_submit(self._h, _p(q), q.shape[0], int(ef), int(k), C.byref(t)))
+        self._k[t.value] = (q.shape[0], int(k))
+        return t.value
+
+    def poll(self, ticket: int):
+        """(ids[nq, k], dists[nq, k]) when the request has completed, else None."""
+        nq, k = self._k[ticket]
+        ids = np.empty((nq, k), np.int32)
+        dis = np.empty((nq, k), np.float32)
+        done = C.c_int32()
+        _lib.check(self._L.glass_hip_server_poll(self._h, int(ticket), _p(ids), _p(dis), C.byref(done)))
+        if not done.value:
+            return None
+        del self._k[ticket]
+        return ids, dis
+
+    def wait(self, ticket: int):
+        nq, k = self._k.pop(ticket)
+        ids = np.empty((nq, k), np.int32)
+        dis = np.empty((nq, k), np.float32)
+        _lib.check(self._L.glass_hip_server_wait(self._h, int(ticket), _p(ids), _p(dis)))
+        return ids, dis
+
+    def cancel(self, ticket: int):
+        _lib.check(self._L.glass_hip_server_cancel(self._h, int(ticket)))
+        self._k.pop(ticket, None)
+
+    def replay(self, queries, nq, ef, k, arrival_ns):
+        """Offered-load driver: request i (nq[i] consecutive rows of `queries`)
+        arrives arrival_ns[i] after the call starts.  Returns (latency_ns per
+        request, ids, dists) with request i's results after those of < i."""
+        q = np.ascontiguousarray(queries, np.float32).reshape(-1, self.dim)
+        nreq = len(arrival_ns)
+        nq = np.ascontiguousarray(np.broadcast_to(np.asarray(nq, np.int32), (nreq,)))
+        ef = np.ascontiguousarray(np.broadcast_to(np.asarray(ef, np.int32), (nreq,)))
+        k = np.ascontiguousarray(np.broadcast_to(np.asarray(k, np.int32), (nreq,)))
+        arr = np.ascontiguousarray(arrival_ns, np.int64)
+        if int(nq.sum()) > q.shape[0]:
+            raise GlassError("requests name more query rows than given")
+        total = int((nq.astype(np.int64) * k).sum())
+        lat = np.zeros(nreq, np.int64)
+        ids = np.empty(total, np.int32)
+        dis = np.empty(total, np.float32)
+        _lib.check(self._L.glass_hip_server_replay(self._h, _p(q), nreq, _p(nq), _p(ef), _p(k), _p(arr), _p(lat),
+                                                   _p(ids), _p(dis)))
+        return lat, ids, dis
+
+    def stats(self):
+        b, q, r = C.c_int64(), C.c_int64(), C.c_int64()
+        _lib.check(self._L.glass_hip_server_stats(self._h, C.byref(b), C.byref(q), C.byref(r)))
+        return {"batches": b.value, "queries": q.value, "requests": r.value}

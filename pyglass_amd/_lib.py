@@ -51,6 +51,14 @@ SIGNATURES = {
     "glass_hip_searcher_get_codes": (C.c_int, [_vp, _vp]),
     "glass_hip_searcher_get_calibration": (C.c_int, [_vp, _vp, _pi32]),
     "glass_hip_searcher_sync": (C.c_int, [_vp]),
+    "glass_hip_server_create": (C.c_int, [_vp, _i32, _i32, _i32, C.POINTER(_vp)]),
+    "glass_hip_server_submit": (C.c_int, [_vp, _vp, _i32, _i32, _i32, C.POINTER(C.c_int64)]),
+    "glass_hip_server_poll": (C.c_int, [_vp, C.c_int64, _vp, _vp, _pi32]),
+    "glass_hip_server_wait": (C.c_int, [_vp, C.c_int64, _vp, _vp]),
+    "glass_hip_server_cancel": (C.c_int, [_vp, C.c_int64]),
+    "glass_hip_server_replay": (C.c_int, [_vp, _vp, C.c_int64, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "glass_hip_server_stats": (C.c_int, [_vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
+    "glass_hip_server_free": (None, [_vp]),
 }
 
 _lib = None
@@ -68,6 +76,15 @@ def load():
     if not os.path.exists(LIB_PATH):
         raise ImportError(f"{LIB_PATH} is missing: build it with `python -m pyglass_amd.build` "
                           "(the HIP path has no CPU fallback)")
+    # PyTorch-ROCm ships its own libamdhip64.  Loading torch first lets this
+    # library bind to that same HIP runtime, so device pointers and streams
+    # from torch tensors are valid here and torch still sees the GPUs; loaded
+    # the other way round, torch would bind to the system runtime and find no
+    # device.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     lib = C.CDLL(LIB_PATH)
     for name, (res, args) in SIGNATURES.items():
         fn = getattr(lib, name)

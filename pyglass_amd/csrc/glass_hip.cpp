@@ -11,9 +11,11 @@
 #include <atomic>
 #include <chrono>
 #include <cmath>
+#include <condition_variable>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <deque>
 #include <fstream>
 #include <memory>
 #include <mutex>
@@ -21,6 +23,7 @@
 #include <random>
 #include <string>
 #include <thread>
+#include <unordered_map>
 #include <vector>
 
 #include "../../include/glass_hip.h"
@@ -168,6 +171,7 @@ struct glass_hip_searcher {
     int32_t vis_override = 0;     // visited-table entries forced by the caller (0 = tuned/auto)
     int32_t force_team = 0, force_wpb = 0;
     int32_t cu_count = 0;
+    int32_t device = 0;           // HIP device the stream and buffers live on
     LaunchConfig last_launch{};
 
     static int bucket(int32_t ef) {
@@ -460,7 +464,10 @@ void gen_random(std::mt19937 &rng, int32_t *addr, int32_t size, int32_t N) {
 }
 
 int ensure_stream(glass_hip_searcher &s) {
-    if (!s.stream) HIP_TRY(hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking));
+    if (!s.stream) {
+        HIP_TRY(hipGetDevice(&s.device));
+        HIP_TRY(hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking));
+    }
     if (!s.cu_count) {
         int dev = 0;
         HIP_TRY(hipGetDevice(&dev));
@@ -1062,6 +1069,446 @@ int glass_hip_searcher_sync(glass_hip_searcher *s) {
     if (!s) return fail(GLASS_HIP_EINVAL, "null searcher");
     if (s->stream) HIP_TRY(hipStreamSynchronize(s->stream));
     return GLASS_HIP_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Served form: continuous batching.
+//
+// Requests (one or more queries with their own ef and k) queue on the host.
+// A dispatcher thread packs whatever is queued, up to max_batch queries, into
+// a pinned staging slot, copies it to the device in one transfer, launches the
+// persistent search kernel with per-query ef/k (SearchBatch semantics per
+// query) and copies the results back, all on the searcher's stream, with up
+// to max_inflight batches enqueued at once.  A batch is formed as soon as a
+// slot is free and anything is queued, so a lone request never waits for a
+// batch to fill.  Each query's result depends only on (query, ef, k, index),
+// so a request gets the same ids and distances alone or in any mix.
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+int64_t now_ns() {
+    return std::chrono::duration_cast<std::chrono::nanoseconds>(Clock::now().time_since_epoch()).count();
+}
+
+enum ReqState : int32_t { RQ_QUEUED = 0, RQ_RUNNING = 1, RQ_DONE = 2, RQ_FAILED = 3 };
+
+struct ServerReq {
+    int64_t ticket = 0;
+    int32_t nq = 0, ef = 0, k = 0;
+    const float *q = nullptr;        // nq x dim rows (owned copy or caller's replay array)
+    int32_t *ids = nullptr;          // nq x k results
+    float *dists = nullptr;
+    std::vector<float> own_q;        // submit(): copies owned by the request
+    std::vector<int32_t> own_ids;
+    std::vector<float> own_dists;
+    int64_t arrival_ns = 0, done_ns = 0;
+    int32_t state = RQ_QUEUED;
+};
+
+struct ServerSlot {
+    // pinned staging: [ef | k | off | queries] in, [ids | dists] out
+    uint8_t *h_in = nullptr, *h_out = nullptr;
+    DevBuf<uint8_t> d_in, d_out;
+    hipEvent_t done = nullptr;
+    std::vector<ServerReq *> reqs;
+    int32_t nq = 0;
+    int64_t nres = 0;
+};
+
+}  // namespace
+
+struct glass_hip_server {
+    glass_hip_searcher *s = nullptr;
+    int32_t max_batch = 0, max_k = 0, max_inflight = 0, dim = 0;
+    size_t in_meta = 0, in_bytes = 0, out_ids_bytes = 0, out_bytes = 0;
+    std::vector<std::unique_ptr<ServerSlot>> slots;
+    std::thread worker;
+
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    std::deque<ServerReq *> pending;
+    std::atomic<int64_t> pending_n{0};  // pending.size(), readable without the lock
+    std::unordered_map<int64_t, std::unique_ptr<ServerReq>> owned;  // submit() requests by ticket
+    int64_t next_ticket = 1;
+    bool stop = false;
+    int err_code = GLASS_HIP_OK;
+    std::string err_msg;
+    int64_t n_batches = 0, n_queries = 0, n_requests = 0;
+
+    ~glass_hip_server() {
+        for (auto &sl : slots) {
+            if (sl->h_in) (void)hipHostFree(sl->h_in);
+            if (sl->h_out) (void)hipHostFree(sl->h_out);
+            if (sl->done) (void)hipEventDestroy(sl->done);
+        }
+    }
+};
+
+namespace {
+
+void server_fail_all(glass_hip_server &sv, int code, const std::string &msg) {
+    std::lock_guard<std::mutex> lk(sv.mu);
+    sv.err_code = code;
+    sv.err_msg = msg;
+    for (ServerReq *r : sv.pending) r->state = RQ_FAILED;
+    sv.pending.clear();
+    sv.pending_n.store(0);
+    for (auto &sl : sv.slots)
+        for (ServerReq *r : sl->reqs) r->state = RQ_FAILED;
+    sv.cv_done.notify_all();
+}
+
+// Packs the requests of `sl` and enqueues copy-in, search and copy-out.
+int server_launch(glass_hip_server &sv, ServerSlot &sl) {
+    glass_hip_searcher &s = *sv.s;
+    const int32_t mb = sv.max_batch;
+    int32_t *h_ef = (int32_t *)sl.h_in;
+    int32_t *h_k = h_ef + mb;
+    int64_t *h_off = (int64_t *)(sl.h_in + 8 * (size_t)mb);
+    float *h_q = (float *)(sl.h_in + sv.in_meta);
+    int32_t pos = 0, max_cap = 1, max_ef = 1;
+    int64_t res = 0;
+    for (ServerReq *r : sl.reqs) {
+        std::memcpy(h_q + (size_t)pos * sv.dim, r->q, sizeof(float) * (size_t)r->nq * sv.dim);
+        for (int32_t j = 0; j < r->nq; ++j) {
+            h_ef[pos + j] = r->ef;
+            h_k[pos + j] = r->k;
+            h_off[pos + j] = res + (int64_t)j * r->k;
+        }
+        pos += r->nq;
+        res += (int64_t)r->nq * r->k;
+        max_cap = std::max(max_cap, std::max(inner_k(s, r->k), r->ef));
+        max_ef = std::max(max_ef, r->ef);
+    }
+    sl.nq = pos;
+    sl.nres = res;
+    std::lock_guard<std::mutex> lk(s.mu);
+    const size_t in_used = sv.in_meta + sizeof(float) * (size_t)pos * sv.dim;
+    HIP_TRY(hipMemcpyAsync(sl.d_in.p, sl.h_in, in_used, hipMemcpyHostToDevice, s.stream));
+    const int32_t *d_ef = (const int32_t *)sl.d_in.p;
+    const int32_t *d_k = d_ef + mb;
+    const int64_t *d_off = (const int64_t *)(sl.d_in.p + 8 * (size_t)mb);
+    const float *d_q = (const float *)(sl.d_in.p + sv.in_meta);
+    int32_t *d_ids = (int32_t *)sl.d_out.p;
+    float *d_dists = (float *)(sl.d_out.p + sv.out_ids_bytes);
+    if (int r = run_batch(s, d_q, pos, d_ef, d_k, 0, 0, max_cap, max_ef, MODE_BATCH, d_off, d_ids, d_dists, nullptr,
+                          s.stream))
+        return r;
+    HIP_TRY(hipMemcpyAsync(sl.h_out, d_ids, 4 * (size_t)res, hipMemcpyDeviceToHost, s.stream));
+    HIP_TRY(hipMemcpyAsync(sl.h_out + sv.out_ids_bytes, d_dists, 4 * (size_t)res, hipMemcpyDeviceToHost, s.stream));
+    HIP_TRY(hipEventRecord(sl.done, s.stream));
+    return GLASS_HIP_OK;
+}
+
+void server_finish(glass_hip_server &sv, ServerSlot &sl) {
+    const int64_t t = now_ns();
+    const int32_t *ids = (const int32_t *)sl.h_out;
+    const float *dists = (const float *)(sl.h_out + sv.out_ids_bytes);
+    std::lock_guard<std::mutex> lk(sv.mu);
+    int64_t off = 0;
+    for (ServerReq *r : sl.reqs) {
+        const size_t n = (size_t)r->nq * r->k;
+        std::memcpy(r->ids, ids + off, 4 * n);
+        std::memcpy(r->dists, dists + off, 4 * n);
+        off += (int64_t)n;
+        r->done_ns = t;
+        r->state = RQ_DONE;
+    }
+    sv.n_batches += 1;
+    sv.n_queries += sl.nq;
+    sv.n_requests += (int64_t)sl.reqs.size();
+    sl.reqs.clear();
+    sv.cv_done.notify_all();
+}
+
+void server_loop(glass_hip_server *psv) {
+    glass_hip_server &sv = *psv;
+    if (hipSetDevice(sv.s->device) != hipSuccess) {
+        server_fail_all(sv, GLASS_HIP_EDEVICE, "server thread: hipSetDevice failed");
+        return;
+    }
+    std::deque<int> inflight;  // slot indices in launch order
+    std::vector<int> free_slots;
+    for (int i = (int)sv.slots.size() - 1; i >= 0; --i) free_slots.push_back(i);
+    for (;;) {
+        bool progressed = false;
+        while (!inflight.empty()) {
+            ServerSlot &sl = *sv.slots[inflight.front()];
+            hipError_t e = hipEventQuery(sl.done);
+            if (e == hipErrorNotReady) break;
+            if (e != hipSuccess) {
+                server_fail_all(sv, GLASS_HIP_EDEVICE, std::string("search batch failed: ") + hipGetErrorString(e));
+                (void)hipStreamSynchronize(sv.s->stream);
+                return;
+            }
+            server_finish(sv, sl);
+            free_slots.push_back(inflight.front());
+            inflight.pop_front();
+            progressed = true;
+        }
+        if (!free_slots.empty()) {
+            ServerSlot &sl = *sv.slots[free_slots.back()];
+            if (inflight.empty() || sv.pending_n.load(std::memory_order_relaxed) > 0) {
+                std::unique_lock<std::mutex> lk(sv.mu);
+                if (sv.pending.empty() && inflight.empty()) {
+                    if (sv.stop) return;
+                    sv.cv_work.wait_for(lk, std::chrono::milliseconds(2));
+                }
+                int32_t total = 0;
+                while (!sv.pending.empty() && total + sv.pending.front()->nq <= sv.max_batch) {
+                    ServerReq *r = sv.pending.front();
+                    sv.pending.pop_front();
+                    r->state = RQ_RUNNING;
+                    total += r->nq;
+                    sl.reqs.push_back(r);
+                }
+                sv.pending_n.store((int64_t)sv.pending.size());
+            }
+            if (!sl.reqs.empty()) {
+                if (int rc = server_launch(sv, sl)) {
+                    server_fail_all(sv, rc, g_last_error);
+                    (void)hipStreamSynchronize(sv.s->stream);
+                    return;
+                }
+                inflight.push_back(free_slots.back());
+                free_slots.pop_back();
+                progressed = true;
+            }
+        }
+        if (!progressed && !inflight.empty()) std::this_thread::yield();
+        if (!progressed && inflight.empty()) {
+            std::lock_guard<std::mutex> lk(sv.mu);
+            if (sv.stop && sv.pending.empty()) return;
+        }
+    }
+}
+
+int server_enqueue(glass_hip_server &sv, ServerReq *r) {
+    if (r->nq <= 0 || r->nq > sv.max_batch) return fail(GLASS_HIP_EINVAL, "request of %d queries (max_batch %d)", r->nq, sv.max_batch);
+    if (r->k <= 0 || r->k > sv.max_k || r->ef <= 0) return fail(GLASS_HIP_EINVAL, "bad ef %d / k %d (max_k %d)", r->ef, r->k, sv.max_k);
+    if (round_cap(std::max(inner_k(*sv.s, r->k), r->ef)) > 4096)
+        return fail(GLASS_HIP_EINVAL, "max(k, ef) = %d exceeds 4096", std::max(inner_k(*sv.s, r->k), r->ef));
+    return GLASS_HIP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int glass_hip_server_create(glass_hip_searcher *s, int32_t max_batch, int32_t max_k, int32_t max_inflight,
+                            glass_hip_server **out) {
+    if (!s || !out || max_batch <= 0 || max_k <= 0 || max_inflight <= 0 || max_inflight > 8)
+        return fail(GLASS_HIP_EINVAL, "bad server arguments");
+    if (!s->has_data) return fail(GLASS_HIP_ESTATE, "server before set_data");
+    auto sv = std::make_unique<glass_hip_server>();
+    sv->s = s;
+    sv->max_batch = max_batch;
+    sv->max_k = max_k;
+    sv->max_inflight = max_inflight;
+    sv->dim = s->dim;
+    sv->in_meta = 16 * (size_t)max_batch;
+    sv->in_bytes = sv->in_meta + sizeof(float) * (size_t)max_batch * s->dim;
+    sv->out_ids_bytes = 4 * (size_t)max_batch * max_k;
+    sv->out_bytes = 2 * sv->out_ids_bytes;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        if (int r = ensure_stream(*s)) return r;
+        // size the per-slot state for a full grid now, so no batch reallocates
+        // memory that an earlier batch in flight still uses
+        LaunchConfig lc{};
+        int32_t most = 0;
+        for (int32_t team : {4, 8, 16}) {
+            if (!team_valid(s->index(), team)) continue;
+            lc.team = team;
+            for (int32_t w = 1; w <= 4; ++w) {
+                lc.waves_per_block = w;
+                lc.cap_max = 64;
+                lc.vis_hash = 256;
+                most = std::max(most, std::max(1, max_blocks_per_cu(s->index(), lc)) * std::max(1, s->cu_count) * w);
+            }
+        }
+        if (most == 0) most = 32 * std::max(1, s->cu_count);  // 8 waves per SIMD, 4 SIMDs per CU
+        if (int r = ensure_slots(*s, most)) return r;
+        if (!s->d_counter.p) HIP_TRY(s->d_counter.alloc(1));
+    }
+    for (int32_t i = 0; i < max_inflight; ++i) {
+        sv->slots.push_back(std::make_unique<ServerSlot>());
+        ServerSlot &sl = *sv->slots.back();
+        HIP_TRY(hipHostMalloc((void **)&sl.h_in, sv->in_bytes, hipHostMallocDefault));
+        HIP_TRY(hipHostMalloc((void **)&sl.h_out, sv->out_bytes, hipHostMallocDefault));
+        HIP_TRY(sl.d_in.alloc(sv->in_bytes));
+        HIP_TRY(sl.d_out.alloc(sv->out_bytes));
+        HIP_TRY(hipEventCreateWithFlags(&sl.done, hipEventDisableTiming));
+    }
+    glass_hip_server *p = sv.get();
+    sv->worker = std::thread(server_loop, p);
+    *out = sv.release();
+    return GLASS_HIP_OK;
+}
+
+int glass_hip_server_submit(glass_hip_server *sv, const float *qs, int32_t nq, int32_t ef, int32_t k,
+                            int64_t *ticket) {
+    if (!sv || !qs || !ticket) return fail(GLASS_HIP_EINVAL, "null argument");
+    auto r = std::make_unique<ServerReq>();
+    r->nq = nq;
+    r->ef = ef;
+    r->k = k;
+    if (int rc = server_enqueue(*sv, r.get())) return rc;
+    r->own_q.assign(qs, qs + (size_t)nq * sv->dim);
+    r->own_ids.assign((size_t)nq * k, -1);
+    r->own_dists.assign((size_t)nq * k, 0.0f);
+    r->q = r->own_q.data();
+    r->ids = r->own_ids.data();
+    r->dists = r->own_dists.data();
+    r->arrival_ns = now_ns();
+    {
+        std::lock_guard<std::mutex> lk(sv->mu);
+        if (sv->err_code) return fail(sv->err_code, "%s", sv->err_msg.c_str());
+        if (sv->stop) return fail(GLASS_HIP_ESTATE, "server is stopping");
+        r->ticket = sv->next_ticket++;
+        *ticket = r->ticket;
+        sv->pending.push_back(r.get());
+        sv->pending_n.store((int64_t)sv->pending.size());
+        sv->owned.emplace(r->ticket, std::move(r));
+    }
+    sv->cv_work.notify_one();
+    return GLASS_HIP_OK;
+}
+
+static int server_take(glass_hip_server *sv, std::unique_lock<std::mutex> &lk, int64_t ticket, int32_t *dst,
+                       float *scores, bool wait, int32_t *done) {
+    auto it = sv->owned.find(ticket);
+    if (it == sv->owned.end()) return fail(GLASS_HIP_EINVAL, "unknown ticket %lld", (long long)ticket);
+    ServerReq *r = it->second.get();
+    if (wait) sv->cv_done.wait(lk, [&] { return r->state == RQ_DONE || r->state == RQ_FAILED; });
+    if (r->state == RQ_FAILED) {
+        sv->owned.erase(it);
+        return fail(sv->err_code ? sv->err_code : GLASS_HIP_EDEVICE, "%s", sv->err_msg.c_str());
+    }
+    if (r->state != RQ_DONE) {
+        if (done) *done = 0;
+        return GLASS_HIP_OK;
+    }
+    if (dst) std::copy(r->own_ids.begin(), r->own_ids.end(), dst);
+    if (scores) std::copy(r->own_dists.begin(), r->own_dists.end(), scores);
+    if (done) *done = 1;
+    sv->owned.erase(it);
+    return GLASS_HIP_OK;
+}
+
+int glass_hip_server_poll(glass_hip_server *sv, int64_t ticket, int32_t *dst, float *scores, int32_t *done) {
+    if (!sv || !done) return fail(GLASS_HIP_EINVAL, "null argument");
+    std::unique_lock<std::mutex> lk(sv->mu);
+    return server_take(sv, lk, ticket, dst, scores, false, done);
+}
+
+int glass_hip_server_wait(glass_hip_server *sv, int64_t ticket, int32_t *dst, float *scores) {
+    if (!sv) return fail(GLASS_HIP_EINVAL, "null argument");
+    std::unique_lock<std::mutex> lk(sv->mu);
+    return server_take(sv, lk, ticket, dst, scores, true, nullptr);
+}
+
+int glass_hip_server_cancel(glass_hip_server *sv, int64_t ticket) {
+    if (!sv) return fail(GLASS_HIP_EINVAL, "null argument");
+    std::lock_guard<std::mutex> lk(sv->mu);
+    auto it = sv->owned.find(ticket);
+    if (it == sv->owned.end()) return fail(GLASS_HIP_EINVAL, "unknown ticket %lld", (long long)ticket);
+    ServerReq *r = it->second.get();
+    if (r->state != RQ_QUEUED) return fail(GLASS_HIP_ESTATE, "request %lld already dispatched", (long long)ticket);
+    sv->pending.erase(std::find(sv->pending.begin(), sv->pending.end(), r));
+    sv->pending_n.store((int64_t)sv->pending.size());
+    sv->owned.erase(it);
+    return GLASS_HIP_OK;
+}
+
+int glass_hip_server_replay(glass_hip_server *sv, const float *qs, int64_t nreq, const int32_t *nq, const int32_t *ef,
+                            const int32_t *k, const int64_t *arrival_ns, int64_t *latency_ns, int32_t *dst,
+                            float *scores) {
+    if (!sv || nreq < 0 || (nreq && (!qs || !nq || !ef || !k || !arrival_ns || !latency_ns || !dst || !scores)))
+        return fail(GLASS_HIP_EINVAL, "bad replay arguments");
+    std::vector<ServerReq> reqs((size_t)nreq);
+    int64_t qoff = 0, roff = 0;
+    for (int64_t i = 0; i < nreq; ++i) {
+        ServerReq &r = reqs[(size_t)i];
+        r.nq = nq[i];
+        r.ef = ef[i];
+        r.k = k[i];
+        if (int rc = server_enqueue(*sv, &r)) return rc;
+        if (i && arrival_ns[i] < arrival_ns[i - 1]) return fail(GLASS_HIP_EINVAL, "arrivals must be sorted");
+        r.q = qs + qoff * sv->dim;
+        r.ids = dst + roff;
+        r.dists = scores + roff;
+        qoff += r.nq;
+        roff += (int64_t)r.nq * r.k;
+    }
+    const int64_t t0 = now_ns();
+    int64_t next = 0;
+    while (next < nreq) {
+        const int64_t t = now_ns() - t0;
+        if (arrival_ns[next] > t) {
+            if (arrival_ns[next] - t > 200000) std::this_thread::sleep_for(std::chrono::microseconds(50));
+            continue;
+        }
+        {
+            std::lock_guard<std::mutex> lk(sv->mu);
+            if (sv->err_code) break;
+            while (next < nreq && arrival_ns[next] <= t) {
+                ServerReq &r = reqs[(size_t)next];
+                r.arrival_ns = t0 + arrival_ns[next];
+                sv->pending.push_back(&r);
+                ++next;
+            }
+            sv->pending_n.store((int64_t)sv->pending.size());
+        }
+        sv->cv_work.notify_one();
+    }
+    std::unique_lock<std::mutex> lk(sv->mu);
+    sv->cv_done.wait(lk, [&] {
+        if (sv->err_code) return true;
+        for (const ServerReq &r : reqs)
+            if (r.state != RQ_DONE) return false;
+        return true;
+    });
+    if (sv->err_code) {
+        // requests of this call may still sit in the queue: drop them
+        for (auto it = sv->pending.begin(); it != sv->pending.end();) {
+            ServerReq *p = *it;
+            it = (p >= reqs.data() && p < reqs.data() + reqs.size()) ? sv->pending.erase(it) : it + 1;
+        }
+        sv->pending_n.store((int64_t)sv->pending.size());
+        return fail(sv->err_code, "%s", sv->err_msg.c_str());
+    }
+    for (int64_t i = 0; i < nreq; ++i) latency_ns[i] = reqs[(size_t)i].done_ns - reqs[(size_t)i].arrival_ns;
+    return GLASS_HIP_OK;
+}
+
+int glass_hip_server_stats(glass_hip_server *sv, int64_t *batches, int64_t *queries, int64_t *requests) {
+    if (!sv) return fail(GLASS_HIP_EINVAL, "null argument");
+    std::lock_guard<std::mutex> lk(sv->mu);
+    if (batches) *batches = sv->n_batches;
+    if (queries) *queries = sv->n_queries;
+    if (requests) *requests = sv->n_requests;
+    return GLASS_HIP_OK;
+}
+
+void glass_hip_server_free(glass_hip_server *sv) {
+    if (!sv) return;
+    {
+        std::lock_guard<std::mutex> lk(sv->mu);
+        sv->stop = true;
+        for (ServerReq *r : sv->pending) r->state = RQ_FAILED;
+        sv->pending.clear();
+        sv->pending_n.store(0);
+        sv->cv_done.notify_all();
+    }
+    sv->cv_work.notify_all();
+    if (sv->worker.joinable()) sv->worker.join();
+    if (sv->s && sv->s->stream) (void)hipStreamSynchronize(sv->s->stream);
+    delete sv;
 }
 
 }  // extern "C"

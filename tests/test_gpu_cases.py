@@ -1,0 +1,196 @@
+"""HIP path vs oracle on the cases the golden graphs do not reach: the
+hand-worked HNSW toy graph, ef >= N brute force (both pool-merge paths),
+per-query ef/k, the visited-set overflow path, every launch shape, short
+pools (k larger than the reachable set) and the device-pointer entry point.
+All through the C ABI; ids and distances bit-exact.
+"""
+
+import numpy as np
+import pytest
+
+import pyglass_amd as pg
+from oracle import oracle as O
+from oracle import toy_graphs as T
+from pyglass_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module", autouse=True)
+def need_gpu():
+    if pg.device_count() < 1:
+        pytest.skip("no HIP device")
+
+
+def ring_graph(n, k, seed):
+    rng = np.random.default_rng(seed)
+    adj = np.full((n, k), -1, np.int32)
+    for i in range(n):
+        nb = [(i + 1) % n] + list(rng.choice(n, k - 1, replace=False))
+        nb = [v for v in dict.fromkeys(nb) if v != i][:k]
+        adj[i, :len(nb)] = nb
+    return adj
+
+
+def mixture(n, d, seed):
+    rng = np.random.default_rng(seed)
+    c = rng.uniform(-3, 3, (8, d)).astype(np.float32)
+    return (c[rng.integers(0, 8, n)] + rng.standard_normal((n, d)).astype(np.float32)).astype(np.float32)
+
+
+@pytest.mark.parametrize("batch", [False, True])
+def test_toy_hnsw_descent(tmp_path, batch):
+    path = str(tmp_path / "toy.glass")
+    X, q = T.toy_hnsw(path)
+    print(f"toy hnsw batch={batch}", flush=True)
+    s = pg.Searcher(pg.Graph(path), X, "L2", "FP32")
+    s.set_ef(3)
+    if batch:
+        ids, dis = s.batch_search(q[None], 3)
+        ids, dis = ids[0], dis[0]
+        assert s.get_last_search_dist_cmps() == T.TOY_DIST_CMPS
+    else:
+        ids, dis = s.search(q, 3)
+    assert list(ids) == T.TOY_RESULT_IDS
+    assert list(dis) == T.TOY_RESULT_DISTS
+
+
+def test_short_pool_pads_like_oracle(tmp_path):
+    # k = 5 but only 4 nodes are reachable from the entry: the tail is -1/+inf
+    path = str(tmp_path / "toy.glass")
+    X, q = T.toy_hnsw(path)
+    s = pg.Searcher(pg.Graph(path), X, "L2", "FP32")
+    o = O.Searcher(O.Graph(path), X, "L2", "FP32")
+    s.set_ef(3)
+    o.set_ef(3)
+    I, D = s.batch_search(q[None], 5)
+    Io, Do = o.batch_search(q[None], 5)
+    np.testing.assert_array_equal(I, Io)
+    np.testing.assert_array_equal(D, Do)
+    assert I[0, -1] == -1 and np.isinf(D[0, -1])
+
+
+@pytest.mark.parametrize("n", [240, 600])
+@pytest.mark.parametrize("quant,metric,refine", [("SQ4U", "L2", "FP32"), ("SQ8U", "IP", ""), ("SQ8", "L2", "")])
+def test_ef_at_least_n_matches_oracle(n, quant, metric, refine):
+    # n=240 keeps the pool in registers (capacity <= 256); n=600 takes the LDS merge
+    X = mixture(n, 64, 31)
+    Q = mixture(40, 64, 32)
+    adj = ring_graph(n, 12, 33)
+    s = pg.Searcher(pg.Graph.from_arrays(adj, [3]), X, metric, quant, refine)
+    o = O.Searcher(O.Graph(adj=adj, eps=[3]), X, metric, quant, refine)
+    print(f"ef>=N n={n} {quant} {metric} {refine or 'none'}", flush=True)
+    for ef, k in [(n, 10), (n + 7, 1), (n // 2, 25)]:
+        s.set_ef(ef)
+        o.set_ef(ef)
+        I, D = s.batch_search(Q, k)
+        Io, Do = o.batch_search(Q, k)
+        np.testing.assert_array_equal(I, Io)
+        np.testing.assert_array_equal(D, Do)
+        assert s.get_last_search_dist_cmps() == o.get_last_search_dist_cmps()
+        for i in range(0, 40, 13):
+            a, b = s.search(Q[i], k)
+            c, d = o.search(Q[i], k)
+            np.testing.assert_array_equal(a, c)
+            np.testing.assert_array_equal(b, d)
+
+
+@pytest.fixture(scope="module")
+def c1(golden_dir):
+    import os
+
+    X = synth.gaussian(10000, 128, synth.BASE_SEED)
+    Q = synth.gaussian(1000, 128, synth.QUERY_SEED)
+    path = os.path.join(golden_dir, "c1_hnsw_l2.glass")
+    return X, Q, path
+
+
+@pytest.mark.parametrize("quant,refine", [("SQ8U", ""), ("SQ4U", "FP32"), ("E5M2", "")])
+def test_per_query_ef_k_matches_oracle(c1, quant, refine):
+    X, Q, path = c1
+    nq = 300
+    rng = np.random.default_rng(synth.EFK_SEED)
+    ef = np.rint(np.exp(rng.uniform(np.log(16), np.log(512), nq))).astype(np.int32)
+    k = rng.integers(1, 101, nq).astype(np.int32)
+    s = pg.Searcher(pg.Graph(path), X, "L2", quant, refine)
+    o = O.Searcher(O.Graph(path), X, "L2", quant, refine)
+    print(f"per-query ef/k {quant} {refine or 'none'}", flush=True)
+    ids, dis, cnt = s.batch_search_efk(Q[:nq], ef, k, with_counters=True)
+    for i in range(nq):
+        o.set_ef(int(ef[i]))
+        Io, Do = o.batch_search(Q[i:i + 1], int(k[i]))
+        np.testing.assert_array_equal(ids[i], Io[0], err_msg=f"query {i} ef={ef[i]} k={k[i]}")
+        np.testing.assert_array_equal(dis[i], Do[0], err_msg=f"query {i}")
+        assert cnt[i, 0] == o.last_counts[0][0]
+        assert cnt[i, 1] == o.last_counts[1][0]
+
+
+@pytest.mark.parametrize("entries", [256, 512])
+def test_visited_overflow_to_global_bitset(c1, entries):
+    # a small LDS table forces most queries past half occupancy onto the
+    # slot's global bitset part-way through the search
+    X, Q, path = c1
+    s = pg.Searcher(pg.Graph(path), X, "L2", "SQ8U")
+    o = O.Searcher(O.Graph(path), X, "L2", "SQ8U")
+    s.set_visited_capacity(entries)
+    print(f"visited overflow entries={entries}", flush=True)
+    for ef in [64, 200]:
+        s.set_ef(ef)
+        o.set_ef(ef)
+        I, D = s.batch_search(Q[:200], 10)
+        Io, Do = o.batch_search(Q[:200], 10)
+        np.testing.assert_array_equal(I, Io)
+        np.testing.assert_array_equal(D, Do)
+        assert s.get_last_search_dist_cmps() == o.get_last_search_dist_cmps()
+
+
+@pytest.mark.parametrize("team", [4, 8, 16])
+@pytest.mark.parametrize("wpb", [1, 2, 4])
+def test_every_launch_shape_gives_the_same_result(c1, team, wpb):
+    X, Q, path = c1
+    s = pg.Searcher(pg.Graph(path), X, "L2", "SQ8U", "FP32")
+    o = O.Searcher(O.Graph(path), X, "L2", "SQ8U", "FP32")
+    s.set_launch(team, wpb, 0)
+    s.set_ef(48)
+    o.set_ef(48)
+    print(f"launch team={team} wpb={wpb}", flush=True)
+    I, D = s.batch_search(Q[:256], 10)
+    Io, Do = o.batch_search(Q[:256], 10)
+    np.testing.assert_array_equal(I, Io)
+    np.testing.assert_array_equal(D, Do)
+    # a grid smaller than the batch: the persistent waves loop over the queue
+    s.set_launch(team, wpb, 3)
+    I2, D2 = s.batch_search(Q[:256], 10)
+    np.testing.assert_array_equal(I2, Io)
+
+
+def test_device_pointer_entry_point_matches_host_batch(c1):
+    import torch
+
+    X, Q, path = c1
+    s = pg.Searcher(pg.Graph(path), X, "L2", "SQ4U", "FP32")
+    s.set_ef(64)
+    I, D = s.batch_search(Q[:500], 10)
+    dev = torch.device("cuda:0")
+    q = torch.from_numpy(Q[:500].copy()).to(dev)
+    ids = torch.full((500, 10), -7, dtype=torch.int32, device=dev)
+    dis = torch.zeros((500, 10), dtype=torch.float32, device=dev)
+    cnt = torch.zeros((500, 3), dtype=torch.int32, device=dev)
+    st = torch.cuda.Stream(device=dev)
+    print("device-pointer batch", flush=True)
+    s.search_batch_dev(q.data_ptr(), 500, 10, ids.data_ptr(), dis.data_ptr(), cnt.data_ptr(), stream=st.cuda_stream)
+    st.synchronize()
+    np.testing.assert_array_equal(ids.cpu().numpy(), I)
+    np.testing.assert_array_equal(dis.cpu().numpy(), D)
+    assert cnt.cpu().numpy()[:, 0].mean() == s.get_last_search_dist_cmps()
+
+
+def test_empty_batch_and_bad_arguments(c1):
+    X, Q, path = c1
+    s = pg.Searcher(pg.Graph(path), X, "L2", "SQ8U")
+    I, D = s.batch_search(Q[:0], 10)
+    assert I.shape == (0, 10)
+    with pytest.raises(pg.GlassError):
+        s.batch_search(Q[:4], 0)
+    with pytest.raises(pg.GlassError):
+        s.set_visited_capacity(300)

@@ -1,0 +1,116 @@
+"""Served form (continuous batching) through the C ABI: a request's result is
+the same alone or in any mix of other requests, equals the batch path and
+the oracle, and cancel / poll / replay behave as declared."""
+
+import os
+
+import numpy as np
+import pytest
+
+import pyglass_amd as pg
+from oracle import oracle as O
+from pyglass_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module", autouse=True)
+def need_gpu():
+    if pg.device_count() < 1:
+        pytest.skip("no HIP device")
+
+
+@pytest.fixture(scope="module")
+def setup(golden_dir):
+    X = synth.gaussian(10000, 128, synth.BASE_SEED)
+    Q = synth.gaussian(1000, 128, synth.QUERY_SEED)
+    path = os.path.join(golden_dir, "c1_hnsw_l2.glass")
+    s = pg.Searcher(pg.Graph(path), X, "L2", "SQ4U", "FP32")
+    return X, Q, path, s
+
+
+def test_alone_equals_in_mix_equals_batch_and_oracle(setup):
+    X, Q, path, s = setup
+    srv = pg.Server(s, max_batch=1024, max_k=64, max_inflight=2)
+    print("server alone vs mix", flush=True)
+    a_ids, a_d = srv.wait(srv.submit(Q[7], k=10, ef=64))
+    rng = np.random.default_rng(synth.EFK_SEED)
+    tickets = []
+    row = 0
+    for i in range(60):
+        nq = int(rng.integers(1, 9))
+        ef = int(rng.choice([16, 32, 64, 128, 256]))
+        k = int(rng.integers(1, 51))
+        rows = np.arange(row, row + nq) % len(Q)
+        tickets.append((srv.submit(Q[rows], k=k, ef=ef), rows, ef, k))
+        row += nq
+        if i == 30:
+            tickets.append((srv.submit(Q[7], k=10, ef=64), np.array([7]), 64, 10))
+    o = O.Searcher(O.Graph(path), X, "L2", "SQ4U", "FP32")
+    for t, rows, ef, k in tickets:
+        ids, dis = srv.wait(t)
+        bi, bd = s.batch_search_efk(Q[rows], ef, k)
+        np.testing.assert_array_equal(ids, np.stack(bi))
+        np.testing.assert_array_equal(dis, np.stack(bd))
+        if len(rows) == 1 and rows[0] == 7 and ef == 64 and k == 10:
+            np.testing.assert_array_equal(ids, a_ids)
+            np.testing.assert_array_equal(dis, a_d)
+        o.set_ef(ef)
+        oi, od = o.batch_search(Q[rows[:1]], k)
+        np.testing.assert_array_equal(ids[:1], oi)
+        np.testing.assert_array_equal(dis[:1], od)
+    st = srv.stats()
+    assert st["requests"] == len(tickets) + 1
+    srv.close()
+
+
+def test_poll_cancel_and_errors(setup):
+    X, Q, path, s = setup
+    srv = pg.Server(s, max_batch=1024, max_k=16, max_inflight=1)
+    big = srv.submit(Q[:1000], k=10, ef=256)
+    small = srv.submit(Q[3], k=5, ef=32)
+    try:
+        srv.cancel(small)
+        cancelled = True
+    except pg.GlassError:
+        cancelled = False  # the dispatcher had already taken it
+    ids, _ = srv.wait(big)
+    assert ids.shape == (1000, 10)
+    if cancelled:
+        with pytest.raises(pg.GlassError):
+            srv.cancel(small)  # the ticket is gone
+    else:
+        r = srv.wait(small)
+        assert r[0].shape == (1, 5)
+    t = srv.submit(Q[4], k=5, ef=32)
+    res = None
+    for _ in range(200000):
+        res = srv.poll(t)
+        if res is not None:
+            break
+    assert res is not None
+    with pytest.raises(pg.GlassError):
+        srv.submit(Q[:2], k=17, ef=32)  # k above max_k
+    with pytest.raises(pg.GlassError):
+        srv.submit(Q[:2], k=5, ef=5000)  # pool above the LDS limit
+    srv.close()
+
+
+def test_replay_poisson_matches_batch(setup):
+    X, Q, path, s = setup
+    srv = pg.Server(s, max_batch=2048, max_k=10, max_inflight=2)
+    rng = np.random.default_rng(synth.ARRIVAL_SEED)
+    n = 2000
+    gaps = rng.exponential(1e9 / 200_000, n)  # 200k requests/s offered
+    arr = np.cumsum(gaps).astype(np.int64)
+    rows = rng.integers(0, len(Q), n)
+    print("server replay", flush=True)
+    lat, ids, dis = srv.replay(Q[rows], 1, 48, 10, arr)
+    s.set_ef(48)
+    bi, bd = s.batch_search(Q[rows], 10)
+    np.testing.assert_array_equal(ids.reshape(n, 10), bi)
+    np.testing.assert_array_equal(dis.reshape(n, 10), bd)
+    assert np.all(lat > 0)
+    st = srv.stats()
+    assert st["queries"] == n and st["batches"] > 1
+    srv.close()
