@@ -215,6 +215,74 @@ def served_bench(s, Q, gt, k, ef, factor, n_sat=200_000, n_load=400_000, loads=(
     return out
 
 
+def _pmc_values(db_path, last=3):
+    """Mean per-dispatch counter value of the last `last` search_kernel
+    dispatches and of the 512 MiB copyBuffer dispatches in a rocpd database."""
+    import sqlite3
+    from collections import OrderedDict
+
+    con = sqlite3.connect(db_path)
+    try:
+        cols = [c[0] for c in con.execute("select * from counters_collection limit 1").description]
+        key = "dispatch_id" if "dispatch_id" in cols else "start"
+        rows = list(con.execute(f"select kernel_name, grid_size, value, {key} from counters_collection order by start"))
+    finally:
+        con.close()
+    disp = OrderedDict()
+    for name, grid, val, d in rows:
+        e = disp.setdefault(d, [name, grid, 0.0])
+        e[2] += val
+    sk = [v[2] for v in disp.values() if "search_kernel" in v[0]][-last:]
+    cp = [v for v in disp.values() if "copyBuffer" in v[0]]
+    if not sk or not cp:
+        return None, None
+    big = max(v[1] for v in cp)
+    cpv = [v[2] for v in cp if v[1] == big]
+    return sum(sk) / len(sk), sum(cpv) / len(cpv)
+
+
+def pmc_traffic(point: str, launch: str, timeout_s: int = 240):
+    """HBM bytes per search launch at an operating point: one rocprofv3 --pmc
+    run per counter (FETCH_SIZE, WRITE_SIZE) of `bench.py --point`, each
+    counter scaled by what it reads for a 512 MiB device copy in the same run
+    (bytes = KiB * 1024 * 512 MiB / copy bytes counted)."""
+    import shutil
+    import subprocess
+    import tempfile
+
+    prof = shutil.which("rocprofv3") or "/opt/rocm/bin/rocprofv3"
+    if not os.path.exists(prof):
+        return None, {"error": "rocprofv3 not found"}
+    tmp = tempfile.mkdtemp(prefix="glass_pmc_", dir="/tmp")
+    env = dict(os.environ, TMPDIR="/tmp")
+    detail = {"point": point, "launch": launch}
+    total = 0.0
+    try:
+        for ctr in ("FETCH_SIZE", "WRITE_SIZE"):
+            d = os.path.join(tmp, ctr.lower())
+            cmd = [prof, "--pmc", ctr, "-d", d, "-o", "run", "--", sys.executable, os.path.abspath(__file__),
+                   "--point", point, "--launch", launch, "--steps", "2", "--warmup", "1", "--no-cpu", "--calib-copy"]
+            log(f"pmc: {ctr} pass")
+            try:
+                subprocess.run(cmd, cwd="/tmp", env=env, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL,
+                               timeout=timeout_s)
+            except subprocess.TimeoutExpired:
+                return None, dict(detail, error=f"{ctr} pass timed out")
+            dbs = [os.path.join(r, f) for r, _, fs in os.walk(d) for f in fs if f.endswith("_results.db")]
+            if not dbs:
+                return None, dict(detail, error=f"{ctr} pass wrote no database")
+            kern, copy = _pmc_values(dbs[0])
+            if kern is None:
+                return None, dict(detail, error=f"{ctr}: dispatches not found")
+            scale = (512 << 20) / (copy * 1024.0)
+            detail[ctr] = {"kib_per_launch": kern, "copy_512MiB_kib": copy, "scale": scale,
+                           "bytes_per_launch": kern * 1024.0 * scale}
+            total += kern * 1024.0 * scale
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+    return total, detail
+
+
 class stdout_to_stderr:
     """Route C-level stdout (the reference's printf progress lines) to stderr
     so stdout carries only the JSON result."""
@@ -280,6 +348,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-served", action="store_true", help="skip the served-form (continuous batching) run")
+    ap.add_argument("--no-pmc", action="store_true", help="skip the two rocprofv3 --pmc traffic passes")
     ap.add_argument("--served", type=str, default=None,
                     help="FACTOR:EF - run only the served-form measurement at this operating point")
     ap.add_argument("--small", action="store_true", help="100k-row version for quick checks")
@@ -319,7 +388,8 @@ def main():
 
     t = time.time()
     s = pg.Searcher(pg.Graph(gpath), X, "L2", "SQ4U", "FP32")
-    s.optimize()
+    if not (args.point and args.launch):
+        s.optimize()
     if args.launch:
         lt, lw, lv = (int(v) for v in args.launch.split(":"))
         s.set_launch(lt, lw, 0)
@@ -403,6 +473,13 @@ def main():
     plain_recall = recall_at(plain_ids, gt[:npl], k)
     log(f"plain: {npl} queries, factor={best_f} ef={op['ef']}: {plain_qps:.0f} QPS, recall {plain_recall:.4f}")
     served = None if args.no_served else served_bench(s, Q, gt, k, op["ef"], best_f)
+    traffic, traffic_detail = None, None
+    if not args.no_pmc:
+        s.set_ef(op["ef"])
+        lz = s.get_launch()
+        traffic, traffic_detail = pmc_traffic(f"{best_f}:{op['ef']}",
+                                              f"{lz['team']}:{lz['waves_per_block']}:{lz['vis_entries']}")
+        log(f"pmc: traffic {traffic} bytes per launch ({traffic_detail})")
 
     cpu = None
     parity = None
@@ -463,7 +540,8 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": traffic,
+            "traffic_detail": traffic_detail,
             "algorithmic_bytes_per_launch": alg_bytes,
             "kernel_ms": op["kernel_ms"],
         },
