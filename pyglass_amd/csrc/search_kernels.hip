@@ -51,6 +51,23 @@ __device__ __forceinline__ float float_of_key(uint32_t k) {
 
 __device__ __forceinline__ uint64_t ballot(bool p) { return __ballot(p); }
 
+// Marks a wave-uniform value as such (moves it to a scalar register), so that
+// branches on it are scalar branches and arithmetic on it runs on the SALU.
+__device__ __forceinline__ int32_t uni(int32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ uint32_t uni(uint32_t v) { return (uint32_t)__builtin_amdgcn_readfirstlane((int)v); }
+
+// Cross-lane moves inside groups of lanes with DPP (no LDS round trip, unlike
+// ds_bpermute).  dpp_xor1/dpp_xor2: partner lane l^1 / l^2 (quad_perm);
+// dpp_down4/dpp_down8: lane l+4 / l+8 within its row of 16 (row_ror:12 /
+// row_ror:8; row_ror:n moves data n lanes up), the partner of the group's
+// first lane for groups of 8 / 16 lanes.
+__device__ __forceinline__ int32_t dpp_xor1(int32_t v) { return __builtin_amdgcn_mov_dpp(v, 0xB1, 0xf, 0xf, false); }
+__device__ __forceinline__ int32_t dpp_xor2(int32_t v) { return __builtin_amdgcn_mov_dpp(v, 0x4E, 0xf, 0xf, false); }
+__device__ __forceinline__ int32_t dpp_down4(int32_t v) { return __builtin_amdgcn_mov_dpp(v, 0x12C, 0xf, 0xf, false); }
+__device__ __forceinline__ int32_t dpp_down8(int32_t v) { return __builtin_amdgcn_mov_dpp(v, 0x128, 0xf, 0xf, false); }
+__device__ __forceinline__ float dpp_xor1(float v) { return __int_as_float(dpp_xor1(__float_as_int(v))); }
+__device__ __forceinline__ float dpp_xor2(float v) { return __int_as_float(dpp_xor2(__float_as_int(v))); }
+
 __device__ __forceinline__ int mbcnt(uint64_t m) {
     return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
 }
@@ -285,7 +302,13 @@ struct IntQuery {
                 }
                 acc = (V == V_SQ8U_IP) ? ip : (int32_t)((uint32_t)qq + cc - 2u * qc);
             }
-            for (int off = slot >> 1; off >= 1; off >>= 1) acc += __shfl_xor(acc, off);
+            if (slot >= 2) acc += dpp_xor1(acc);
+            if (slot >= 4) acc += dpp_xor2(acc);
+            if (slot >= 8) acc += dpp_down4(acc);
+            if (slot >= 16) acc += dpp_down8(acc);
+            // across rows of 16: the first lane of each 16-lane group holds its partial sum
+            if (slot >= 32) acc += __shfl_xor(acc, 16);
+            if (slot >= 64) acc += __shfl_xor(acc, 32);
             if (j < m && part == 0) {
                 int32_t d = (V == V_SQ8U_IP) ? -acc : acc;
                 w.ck[j] = key_of_int(d);
@@ -320,9 +343,9 @@ struct Acc4 {
 template <bool kGlassTree>
 __device__ __forceinline__ float reduce16_4lanes(Acc4 v) {
 #pragma unroll
-    for (int r = 0; r < 4; ++r) v.a[r] = v.a[r] + __shfl_xor(v.a[r], 2);
+    for (int r = 0; r < 4; ++r) v.a[r] = v.a[r] + dpp_xor2(v.a[r]);
 #pragma unroll
-    for (int r = 0; r < 4; ++r) v.a[r] = v.a[r] + __shfl_xor(v.a[r], 1);
+    for (int r = 0; r < 4; ++r) v.a[r] = v.a[r] + dpp_xor1(v.a[r]);
     return kGlassTree ? (v.a[0] + v.a[1]) + (v.a[2] + v.a[3]) : (v.a[0] + v.a[2]) + (v.a[1] + v.a[3]);
 }
 
@@ -516,7 +539,7 @@ struct PoolState {
 // cur to the next unchecked entry.
 __device__ __forceinline__ uint32_t pool_pop(const WaveLds &w, PoolState &ps) {
     const int l = lane_id();
-    const uint32_t uid = w.pid[ps.cur];
+    const uint32_t uid = uni(w.pid[ps.cur]);
     wave_sync();
     if (l == 0) w.pid[ps.cur] = uid | kChecked;
     int c = ps.cur + 1;
@@ -530,7 +553,7 @@ __device__ __forceinline__ uint32_t pool_pop(const WaveLds &w, PoolState &ps) {
         }
         c += 64;
     }
-    ps.cur = c < ps.size ? c : ps.size;
+    ps.cur = uni(c < ps.size ? c : ps.size);
     wave_sync();
     return uid & kIdMask;
 }
@@ -619,8 +642,8 @@ __device__ void pool_merge_lds(const WaveLds &w, PoolState &ps, int m) {
         w.pid[np] = ij;
     }
     wave_sync();
-    ps.size = new_size;
-    ps.cur = new_cur;
+    ps.size = uni(new_size);
+    ps.cur = uni(new_cur);
 }
 
 // Pools of up to kMaxChunks*64 entries are merged from registers: lane l
@@ -687,7 +710,9 @@ __device__ uint32_t pool_merge_reg(const DeviceIndex &ix, const WaveLds &w, Pool
     // the next pop takes entry new_cur: the rank-0 candidate or the old first
     // unchecked entry
     uint32_t next = kIdMask;
-    if (new_cur < new_size && new_cur < ps.ef) next = (new_cur == np0) ? id0 : (w.pid[ps.cur] & kIdMask);
+    new_size = uni(new_size);
+    new_cur = uni(new_cur);
+    if (new_cur < new_size && new_cur < ps.ef) next = uni((new_cur == np0) ? id0 : (w.pid[ps.cur] & kIdMask));
     if (next != kIdMask) pred_row = l < ix.K ? ix.adj[(int64_t)next * ix.K + l] : -1;
     if (am) {
         const int lo = np0 < ps.cap ? np0 : ps.cap;  // entries before the first insertion do not move
@@ -741,11 +766,11 @@ __global__ __launch_bounds__(256) void search_kernel(DeviceIndex ix, QueryBatch 
     for (;;) {
         int qi = 0;
         if (l == 0) qi = atomicAdd(ss.work_counter, 1);
-        qi = __shfl(qi, 0);
+        qi = uni(qi);
         if (qi >= qb.nq) break;
 
-        const int32_t ef = qb.ef ? qb.ef[qi] : qb.ef_uniform;
-        const int32_t k = qb.k ? qb.k[qi] : qb.k_uniform;
+        const int32_t ef = uni(qb.ef ? qb.ef[qi] : qb.ef_uniform);
+        const int32_t k = uni(qb.k ? qb.k[qi] : qb.k_uniform);
         // Refiner::Search/SearchBatch: reorder_k = int(k * factor) (refiner.hpp:46,73)
         int32_t kk = k;
         bool refine = false;
@@ -802,11 +827,13 @@ __global__ __launch_bounds__(256) void search_kernel(DeviceIndex ix, QueryBatch 
             if (l == 0) w.ci[0] = u;
             wave_sync();
             dists(1);
-            uint32_t du = w.ck[0];
+            uint32_t du = uni(w.ck[0]);
             cnt_d += 1;
-            for (int level = ix.levels[ix.ep]; level > 0; --level) {
+            for (int level = uni(ix.levels[ix.ep]); level > 0; --level) {
                 for (;;) {
-                    const int32_t *list = ix.up_lists + ix.up_off[u] + (int64_t)(level - 1) * ix.up_K;
+                    const int64_t uo = ((int64_t)uni((int32_t)(ix.up_off[u] >> 32)) << 32) |
+                                       (uint32_t)uni((int32_t)ix.up_off[u]);
+                    const int32_t *list = ix.up_lists + uo + (int64_t)(level - 1) * ix.up_K;
                     const int32_t v = l < ix.up_K ? list[l] : -1;
                     const uint64_t inv = ballot(l < ix.up_K && v == -1);
                     const int n = inv ? __builtin_ctzll(inv) : ix.up_K;
@@ -818,8 +845,8 @@ __global__ __launch_bounds__(256) void search_kernel(DeviceIndex ix, QueryBatch 
                     cnt_u += n;
                     const uint32_t mk = l < n ? w.ck[l] : kKeyMax;
                     const uint64_t best = wave_min_u64(((uint64_t)mk << 32) | (uint32_t)l);
-                    const uint32_t bk = (uint32_t)(best >> 32);
-                    const int bi = (int)(best & 0xffffffffu);
+                    const uint32_t bk = uni((uint32_t)(best >> 32));
+                    const int bi = uni((int)(best & 0xffffffffu));
                     wave_sync();
                     if (bk < du) {  // strict <: the first minimum of the pass wins
                         du = bk;
@@ -875,7 +902,7 @@ __global__ __launch_bounds__(256) void search_kernel(DeviceIndex ix, QueryBatch 
                 // first unchecked entry
                 pred = kIdMask;
                 if (ps.cur < ps.size && ps.cur < ps.ef) {
-                    pred = w.pid[ps.cur] & kIdMask;
+                    pred = uni(w.pid[ps.cur]) & kIdMask;
                     pred_row = l < ix.K ? ix.adj[(int64_t)pred * ix.K + l] : -1;
                 }
                 continue;
