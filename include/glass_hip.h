@@ -116,6 +116,11 @@ int glass_hip_searcher_set_refine_factor(glass_hip_searcher *s, float factor);
  * set_launch forces team / waves_per_block for every bucket (0 = tuned) and
  * caps the persistent grid at `blocks` workgroups (0 = fill the device). */
 int glass_hip_searcher_set_launch(glass_hip_searcher *s, int32_t team, int32_t waves_per_block, int32_t blocks);
+/* Integer codes: 1 = gather every neighbour's code row as soon as the
+ * adjacency row is known and run the visited test while the loads are in
+ * flight (used when one lane group per neighbour fits a wave), 0 = gather
+ * only the unvisited neighbours, -1 = tuned by optimize (default). */
+int glass_hip_searcher_set_gather(glass_hip_searcher *s, int32_t mode);
 /* LDS visited-table entries per query (power of two in [256, 16384]; 0 =
  * tuned / sized from ef).  A query that visits more than half as many nodes
  * continues on the global bitset of its slot. */

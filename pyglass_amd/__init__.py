@@ -261,6 +261,11 @@ class Searcher:
         cap the persistent grid (0 = fill the device)."""
         _lib.check(self._L.glass_hip_searcher_set_launch(self._h, int(team), int(waves_per_block), int(blocks)))
 
+    def set_gather(self, mode: int = -1):
+        """Integer codes: 1 gathers every neighbour's code row before the
+        visited test, 0 only the unvisited ones, -1 lets optimize() choose."""
+        _lib.check(self._L.glass_hip_searcher_set_gather(self._h, int(mode)))
+
     def set_visited_capacity(self, entries: int = 0):
         """LDS visited-table entries per query (0 = tuned / sized from ef)."""
         _lib.check(self._L.glass_hip_searcher_set_visited_capacity(self._h, int(entries)))

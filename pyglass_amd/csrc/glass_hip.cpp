@@ -164,12 +164,14 @@ struct glass_hip_searcher {
     // launch configuration, tuned per ef bucket by Optimize
     struct Tune {
         int32_t team = 8, waves_per_block = 4, vis_hash = 0;  // vis_hash 0: size from ef
+        int32_t spec = 1;  // integer codes: gather all neighbours' rows before the visited test
     };
     static constexpr int kBuckets = 7;  // ef <= 16, 32, 64, 128, 256, 512, larger
     Tune tune[kBuckets];
     int32_t blocks = 0;           // persistent grid cap (0 = fill the device)
     int32_t vis_override = 0;     // visited-table entries forced by the caller (0 = tuned/auto)
     int32_t force_team = 0, force_wpb = 0;
+    int32_t force_spec = -1;      // -1 tuned, 0 off, 1 on
     int32_t cu_count = 0;
     int32_t device = 0;           // HIP device the stream and buffers live on
     LaunchConfig last_launch{};
@@ -497,6 +499,15 @@ int ensure_slots(glass_hip_searcher &s, int32_t slots) {
     return GLASS_HIP_OK;
 }
 
+// The all-neighbour gather needs one lane group per adjacency entry in a wave.
+bool spec_fits(const glass_hip_searcher &s, int32_t team) {
+    const int32_t code_dw = s.code_size / 4;
+    const int32_t parts = (code_dw + team - 1) / team;
+    int32_t slot = 1;
+    while (slot < parts) slot <<= 1;
+    return slot * s.graph.K <= 64;
+}
+
 int32_t round_cap(int32_t cap) {
     int32_t c = 64;
     while (c < cap) c <<= 1;
@@ -518,6 +529,7 @@ int run_batch(glass_hip_searcher &s, const float *qs_dev, int32_t nq, const int3
     lc.waves_per_block = s.force_wpb ? s.force_wpb : tn.waves_per_block;
     lc.cap_max = round_cap(max_cap);
     lc.vis_hash = s.vis_override ? s.vis_override : (tn.vis_hash ? tn.vis_hash : glass_hip_searcher::auto_vis(max_ef));
+    lc.spec = s.force_spec >= 0 ? s.force_spec : tn.spec;
     const size_t lds = lds_bytes_per_wave(s.index(), lc.cap_max, lc.vis_hash) * (size_t)lc.waves_per_block;
     if (lds > 160 * 1024) return fail(GLASS_HIP_EINVAL, "pool capacity %d needs %zu bytes of LDS", lc.cap_max, lds);
     const int32_t blocks = grid_blocks(s, lc, nq);
@@ -910,7 +922,9 @@ int glass_hip_searcher_optimize(glass_hip_searcher *s, int32_t num_threads) {
     const int32_t vhs[] = {1024, 2048, 4096};
     std::vector<int32_t> ids((size_t)nq * kTryK);
     const int32_t saved_ef = s->ef;
-    const int32_t saved_vo = s->vis_override, saved_ft = s->force_team, saved_fw = s->force_wpb;
+    const int32_t saved_vo = s->vis_override, saved_ft = s->force_team, saved_fw = s->force_wpb,
+                  saved_fs = s->force_spec;
+    const bool int_codes = s->quant == QK_SQ8U || s->quant == QK_SQ4U || s->quant == QK_SQ2U;
     auto timeit = [&]() -> double {
         auto t0 = std::chrono::high_resolution_clock::now();
         int r = host_batch(*s, qs.data(), nq, nullptr, nullptr, kTryK, MODE_BATCH, ids.data(), nullptr, nullptr,
@@ -925,19 +939,25 @@ int glass_hip_searcher_optimize(glass_hip_searcher *s, int32_t num_threads) {
         for (int32_t t : teams) {
             for (int32_t w : wpbs) {
                 for (int32_t v : vhs) {
-                    s->force_team = t;
-                    s->force_wpb = w;
-                    s->vis_override = v;
-                    const size_t lds = lds_bytes_per_wave(s->index(), round_cap(std::max(inner_k(*s, kTryK), s->ef)),
-                                                          v) * (size_t)w;
-                    if (lds > 160 * 1024) continue;
-                    timeit();
-                    double e = std::min(timeit(), timeit());
-                    if (e < best) {
-                        best = e;
-                        bt.team = t;
-                        bt.waves_per_block = w;
-                        bt.vis_hash = v;
+                    for (int32_t sp : {0, 1}) {
+                        if (sp && !(int_codes && spec_fits(*s, t))) continue;
+                        s->force_team = t;
+                        s->force_wpb = w;
+                        s->vis_override = v;
+                        s->force_spec = sp;
+                        const size_t lds = lds_bytes_per_wave(
+                                               s->index(), round_cap(std::max(inner_k(*s, kTryK), s->ef)), v) *
+                                           (size_t)w;
+                        if (lds > 160 * 1024) continue;
+                        timeit();
+                        double e = std::min(timeit(), timeit());
+                        if (e < best) {
+                            best = e;
+                            bt.team = t;
+                            bt.waves_per_block = w;
+                            bt.vis_hash = v;
+                            bt.spec = sp;
+                        }
                     }
                 }
             }
@@ -949,6 +969,7 @@ int glass_hip_searcher_optimize(glass_hip_searcher *s, int32_t num_threads) {
     s->vis_override = saved_vo;
     s->force_team = saved_ft;
     s->force_wpb = saved_fw;
+    s->force_spec = saved_fs;
     return GLASS_HIP_OK;
 }
 
@@ -972,6 +993,13 @@ int glass_hip_searcher_set_launch(glass_hip_searcher *s, int32_t team, int32_t w
     s->force_team = team;
     s->force_wpb = waves_per_block;
     s->blocks = blocks;
+    return GLASS_HIP_OK;
+}
+
+int glass_hip_searcher_set_gather(glass_hip_searcher *s, int32_t mode) {
+    if (!s || mode < -1 || mode > 1) return fail(GLASS_HIP_EINVAL, "gather mode must be -1, 0 or 1");
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->force_spec = mode;
     return GLASS_HIP_OK;
 }
 

@@ -60,6 +60,7 @@ struct LaunchConfig {
     int32_t blocks;          // persistent grid size
     int32_t cap_max;         // LDS pool capacity per wave (>= every query's max(k',ef))
     int32_t vis_hash;        // LDS visited-table entries per wave (power of two >= 256)
+    int32_t spec;            // integer codes: gather every neighbour's row before the visited test
 };
 
 struct SlotState {

@@ -257,62 +257,70 @@ struct IntQuery {
         qq = (int32_t)acc;
     }
 
+    // This lane's slice of code row `id` (zeros for slices past the row).
+    __device__ __forceinline__ void load_row(const DeviceIndex &ix, uint32_t id, bool on, uint4 (&c4)[DPL / 4]) const {
+        const int code_q = ix.code_size / 16;  // uint4 per row
+        const uint4 *src = (const uint4 *)(ix.codes + (int64_t)id * ix.code_size);
+#pragma unroll
+        for (int i = 0; i < DPL / 4; ++i) {
+            const int qi = part * (DPL / 4) + i;
+            c4[i] = (on && part < nparts && qi < code_q) ? src[qi] : make_uint4(0, 0, 0, 0);
+        }
+    }
+
+    // Distance of a row whose slices c4 the lanes of one group hold; valid in
+    // the group's first lane (part 0).
+    __device__ __forceinline__ int32_t row_dist(const uint4 (&c4)[DPL / 4], bool on) const {
+        int32_t acc = 0;
+        if (on && part < nparts) {
+            uint32_t qc = 0, cc = 0;
+            int32_t ip = 0;
+#pragma unroll
+            for (int i = 0; i < DPL / 4; ++i) {
+                const uint32_t cs[4] = {c4[i].x, c4[i].y, c4[i].z, c4[i].w};
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const uint32_t qv = q[4 * i + r], c = cs[r];
+                    if (V == V_SQ8U_L2) {
+                        qc = udot4(qv, c, qc);
+                        cc = udot4(c, c, cc);
+                    } else if (V == V_SQ8U_IP) {
+                        ip = sdot4(qv, c, ip);
+                    } else if (V == V_SQ4U) {
+                        qc = udot8(qv, c, qc);
+                        cc = udot8(c, c, cc);
+                    } else {  // V_SQ2U
+                        const uint32_t c0 = c & 0x33333333u, c1 = (c >> 2) & 0x33333333u;
+                        qc = udot8(qv & 0x33333333u, c0, qc);
+                        qc = udot8((qv >> 2) & 0x33333333u, c1, qc);
+                        cc = udot8(c0, c0, cc);
+                        cc = udot8(c1, c1, cc);
+                    }
+                }
+            }
+            acc = (V == V_SQ8U_IP) ? ip : (int32_t)((uint32_t)qq + cc - 2u * qc);
+        }
+        if (slot >= 2) acc += dpp_xor1(acc);
+        if (slot >= 4) acc += dpp_xor2(acc);
+        if (slot >= 8) acc += dpp_down4(acc);
+        if (slot >= 16) acc += dpp_down8(acc);
+        // across rows of 16: the first lane of each 16-lane group holds its partial sum
+        if (slot >= 32) acc += __shfl_xor(acc, 16);
+        if (slot >= 64) acc += __shfl_xor(acc, 32);
+        return (V == V_SQ8U_IP) ? -acc : acc;
+    }
+
     // Distances of candidates ci[0..m) -> keys ck[0..m).
     __device__ void distances(const DeviceIndex &ix, const WaveLds &w, int m) const {
         const int l = lane_id();
         const int cpr = 64 / slot;
         const int cj = l / slot;
-        const int code_q = ix.code_size / 16;  // uint4 per row
         for (int base = 0; base < m; base += cpr) {
             const int j = base + cj;
-            int32_t acc = 0;
-            if (j < m && part < nparts) {
-                const uint32_t id = w.ci[j];
-                const uint4 *src = (const uint4 *)(ix.codes + (int64_t)id * ix.code_size);
-                uint4 c4[DPL / 4];
-#pragma unroll
-                for (int i = 0; i < DPL / 4; ++i) {
-                    int qi = part * (DPL / 4) + i;
-                    c4[i] = qi < code_q ? src[qi] : make_uint4(0, 0, 0, 0);
-                }
-                uint32_t qc = 0, cc = 0;
-                int32_t ip = 0;
-#pragma unroll
-                for (int i = 0; i < DPL / 4; ++i) {
-                    const uint32_t cs[4] = {c4[i].x, c4[i].y, c4[i].z, c4[i].w};
-#pragma unroll
-                    for (int r = 0; r < 4; ++r) {
-                        const uint32_t qv = q[4 * i + r], c = cs[r];
-                        if (V == V_SQ8U_L2) {
-                            qc = udot4(qv, c, qc);
-                            cc = udot4(c, c, cc);
-                        } else if (V == V_SQ8U_IP) {
-                            ip = sdot4(qv, c, ip);
-                        } else if (V == V_SQ4U) {
-                            qc = udot8(qv, c, qc);
-                            cc = udot8(c, c, cc);
-                        } else {  // V_SQ2U
-                            const uint32_t c0 = c & 0x33333333u, c1 = (c >> 2) & 0x33333333u;
-                            qc = udot8(qv & 0x33333333u, c0, qc);
-                            qc = udot8((qv >> 2) & 0x33333333u, c1, qc);
-                            cc = udot8(c0, c0, cc);
-                            cc = udot8(c1, c1, cc);
-                        }
-                    }
-                }
-                acc = (V == V_SQ8U_IP) ? ip : (int32_t)((uint32_t)qq + cc - 2u * qc);
-            }
-            if (slot >= 2) acc += dpp_xor1(acc);
-            if (slot >= 4) acc += dpp_xor2(acc);
-            if (slot >= 8) acc += dpp_down4(acc);
-            if (slot >= 16) acc += dpp_down8(acc);
-            // across rows of 16: the first lane of each 16-lane group holds its partial sum
-            if (slot >= 32) acc += __shfl_xor(acc, 16);
-            if (slot >= 64) acc += __shfl_xor(acc, 32);
-            if (j < m && part == 0) {
-                int32_t d = (V == V_SQ8U_IP) ? -acc : acc;
-                w.ck[j] = key_of_int(d);
-            }
+            uint4 c4[DPL / 4];
+            load_row(ix, j < m ? w.ci[j] : 0u, j < m, c4);
+            const int32_t d = row_dist(c4, j < m);
+            if (j < m && part == 0) w.ck[j] = key_of_int(d);
         }
     }
 };
@@ -655,11 +663,11 @@ __device__ void pool_merge_lds(const WaveLds &w, PoolState &ps, int m) {
 // overlaps the rest of the merge and the pop.
 constexpr int kMaxChunks = 4;
 
-__device__ uint32_t pool_merge_reg(const DeviceIndex &ix, const WaveLds &w, PoolState &ps, int m,
-                                   int32_t &pred_row) {
+// Candidates: lanes with `cand` set, keys kj / ids ij, insertion order = lane
+// order.  nj is the adjacency-row entry this lane loads for the prefetch.
+__device__ uint32_t pool_merge_reg(const DeviceIndex &ix, const WaveLds &w, PoolState &ps, uint32_t kj,
+                                   uint32_t ij, bool cand, int nj, int32_t &pred_row) {
     const int l = lane_id();
-    const uint32_t kj = l < m ? w.ck[l] : kKeyMax;
-    const uint32_t ij = l < m ? w.ci[l] : 0u;
     const int nch = (ps.size + 63) >> 6;
     uint32_t pkc[kMaxChunks], pic[kMaxChunks];
 #pragma unroll
@@ -669,7 +677,7 @@ __device__ uint32_t pool_merge_reg(const DeviceIndex &ix, const WaveLds &w, Pool
         pkc[c] = v ? w.pk[p] : kKeyMax;
         pic[c] = v ? w.pid[p] : 0u;
     }
-    bool act = l < m;
+    bool act = cand;
     // insert() rejects d >= data[size-1] when full; such a candidate would
     // also land past cap in the merge and displaces nothing.
     if (ps.size == ps.cap) act = act && (kj < w.pk[ps.size - 1]);
@@ -713,7 +721,7 @@ __device__ uint32_t pool_merge_reg(const DeviceIndex &ix, const WaveLds &w, Pool
     new_size = uni(new_size);
     new_cur = uni(new_cur);
     if (new_cur < new_size && new_cur < ps.ef) next = uni((new_cur == np0) ? id0 : (w.pid[ps.cur] & kIdMask));
-    if (next != kIdMask) pred_row = l < ix.K ? ix.adj[(int64_t)next * ix.K + l] : -1;
+    if (next != kIdMask) pred_row = nj < ix.K ? ix.adj[(int64_t)next * ix.K + nj] : -1;
     if (am) {
         const int lo = np0 < ps.cap ? np0 : ps.cap;  // entries before the first insertion do not move
         // shift of each pool entry = number of candidates with a smaller key
@@ -747,6 +755,15 @@ __device__ uint32_t pool_merge_reg(const DeviceIndex &ix, const WaveLds &w, Pool
         ps.cur = new_cur;
     }
     return next;
+}
+
+// Candidates 0..m-1 staged in LDS (ck / ci).
+__device__ __forceinline__ uint32_t pool_merge_reg_lds(const DeviceIndex &ix, const WaveLds &w, PoolState &ps, int m,
+                                                       int32_t &pred_row) {
+    const int l = lane_id();
+    const uint32_t kj = l < m ? w.ck[l] : kKeyMax;
+    const uint32_t ij = l < m ? w.ci[l] : 0u;
+    return pool_merge_reg(ix, w, ps, kj, ij, l < m, l, pred_row);
 }
 
 // ---------------------------------------------------------------------------
@@ -874,7 +891,7 @@ __global__ __launch_bounds__(256) void search_kernel(DeviceIndex ix, QueryBatch 
             cnt_d += nep;
             if (lc.cap_max <= 64 * kMaxChunks) {
                 int32_t unused = -1;
-                pool_merge_reg(ix, w, ps, nep, unused);
+                pool_merge_reg_lds(ix, w, ps, nep, unused);
             } else {
                 pool_merge_lds(w, ps, nep);
             }
@@ -887,12 +904,54 @@ __global__ __launch_bounds__(256) void search_kernel(DeviceIndex ix, QueryBatch 
         // is loaded speculatively, in flight together with the current
         // candidates' code gathers; the next pop uses it when it pops that
         // node (the merge did not insert anything closer), else reloads.
+        //
+        // Integer codes, when one lane group per neighbour fits the wave
+        // (lc.spec): every group loads its neighbour's id straight from the
+        // adjacency row and issues that neighbour's code gather at once; the
+        // visited test runs while the gathers are in flight, and the rows
+        // that turn out visited are discarded.  Candidates stay in their
+        // lanes, so lane order is adjacency order for the merge.
+        const bool spec = is_int_variant(V) && lc.spec && iq.slot * ix.K <= 64;
+        const int nj = spec ? l / iq.slot : l;  // adjacency entry this lane reads
         uint32_t pred = kIdMask;
         int32_t pred_row = -1;
         while (ps.cur < ps.size && ps.cur < ps.ef) {
             const uint32_t u = pool_pop(w, ps);
             cnt_e += 1;
-            const int32_t v = (u == pred) ? pred_row : (l < ix.K ? ix.adj[(int64_t)u * ix.K + l] : -1);
+            const int32_t v = (u == pred) ? pred_row : (nj < ix.K ? ix.adj[(int64_t)u * ix.K + nj] : -1);
+            if (spec) {
+                const bool lead = iq.part == 0;
+                const uint64_t inv = ballot(lead && nj < ix.K && v == -1);
+                const int n = inv ? __builtin_ctzll(inv) / iq.slot : ix.K;
+                const bool valid = nj < n;
+                uint4 c4[DPL / 4];
+                iq.load_row(ix, (uint32_t)v, valid, c4);
+                const bool isnew = vs.test_set((uint32_t)v, lead && valid);
+                const uint64_t nm = ballot(isnew);
+                pred = kIdMask;
+                if (nm == 0) {
+                    if (ps.cur < ps.size && ps.cur < ps.ef) {
+                        pred = uni(w.pid[ps.cur]) & kIdMask;
+                        pred_row = nj < ix.K ? ix.adj[(int64_t)pred * ix.K + nj] : -1;
+                    }
+                    continue;
+                }
+                const int m = __builtin_popcountll(nm);
+                vs.account(m);
+                const uint32_t kj = key_of_int(iq.row_dist(c4, valid));
+                cnt_d += m;
+                if (lc.cap_max <= 64 * kMaxChunks) {
+                    pred = pool_merge_reg(ix, w, ps, kj, (uint32_t)v, isnew, nj, pred_row);
+                } else {
+                    if (isnew) {
+                        w.ck[mbcnt(nm)] = kj;
+                        w.ci[mbcnt(nm)] = (uint32_t)v;
+                    }
+                    wave_sync();
+                    pool_merge_lds(w, ps, m);
+                }
+                continue;
+            }
             const uint64_t inv = ballot(l < ix.K && v == -1);
             const int n = inv ? __builtin_ctzll(inv) : ix.K;
             const bool isnew = vs.test_set((uint32_t)v, l < n);
@@ -903,7 +962,7 @@ __global__ __launch_bounds__(256) void search_kernel(DeviceIndex ix, QueryBatch 
                 pred = kIdMask;
                 if (ps.cur < ps.size && ps.cur < ps.ef) {
                     pred = uni(w.pid[ps.cur]) & kIdMask;
-                    pred_row = l < ix.K ? ix.adj[(int64_t)pred * ix.K + l] : -1;
+                    pred_row = nj < ix.K ? ix.adj[(int64_t)pred * ix.K + nj] : -1;
                 }
                 continue;
             }
@@ -914,7 +973,7 @@ __global__ __launch_bounds__(256) void search_kernel(DeviceIndex ix, QueryBatch 
             dists(m);
             cnt_d += m;
             if (lc.cap_max <= 64 * kMaxChunks) {
-                pred = pool_merge_reg(ix, w, ps, m, pred_row);
+                pred = pool_merge_reg_lds(ix, w, ps, m, pred_row);
             } else {
                 pool_merge_lds(w, ps, m);
                 pred = kIdMask;

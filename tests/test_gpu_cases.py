@@ -194,3 +194,25 @@ def test_empty_batch_and_bad_arguments(c1):
         s.batch_search(Q[:4], 0)
     with pytest.raises(pg.GlassError):
         s.set_visited_capacity(300)
+
+
+@pytest.mark.parametrize("quant,metric", [("SQ4U", "L2"), ("SQ2U", "L2"), ("SQ8U", "L2"), ("SQ8U", "IP")])
+@pytest.mark.parametrize("team", [8, 16])
+@pytest.mark.parametrize("gather", [0, 1])
+def test_neighbour_gather_modes_match_oracle(c1, quant, metric, team, gather):
+    # gather=1 loads every neighbour's code row before the visited test (when
+    # one lane group per neighbour fits the wave); results must not change
+    X, Q, path = c1
+    s = pg.Searcher(pg.Graph(path), X, metric, quant, "FP32" if quant != "SQ8U" else "")
+    o = O.Searcher(O.Graph(path), X, metric, quant, "FP32" if quant != "SQ8U" else "")
+    s.set_launch(team, 1, 0)
+    s.set_gather(gather)
+    print(f"gather={gather} team={team} {quant} {metric}", flush=True)
+    for ef, k in [(24, 10), (96, 10), (300, 50)]:
+        s.set_ef(ef)
+        o.set_ef(ef)
+        I, D = s.batch_search(Q[:200], k)
+        Io, Do = o.batch_search(Q[:200], k)
+        np.testing.assert_array_equal(I, Io)
+        np.testing.assert_array_equal(D, Do)
+        assert s.get_last_search_dist_cmps() == o.get_last_search_dist_cmps()
