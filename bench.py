@@ -173,7 +173,8 @@ def gpu_sweep(s, q_dev, nq, k, gt, ids_dev, d_dev, cnt_dev, args, torch, stream,
     return pts
 
 
-def served_bench(s, Q, gt, k, ef, factor, n_sat=200_000, n_load=400_000, loads=(0.25, 0.5, 0.75, 0.9)):
+def served_bench(s, Q, gt, k, ef, factor, n_sat=200_000, n_load=400_000, loads=(0.25, 0.5, 0.75, 0.9),
+                 max_batch=8192, max_inflight=3):
     """Served form: single-query requests through the continuous-batching
     server.  Saturation = completion rate with every request queued at t=0;
     then Poisson arrivals (seed ARRIVAL_SEED) at fractions of it, with each
@@ -182,7 +183,7 @@ def served_bench(s, Q, gt, k, ef, factor, n_sat=200_000, n_load=400_000, loads=(
     from pyglass_amd import synth
 
     s.set_refine_factor(factor)
-    srv = pg.Server(s, max_batch=8192, max_k=k, max_inflight=2)
+    srv = pg.Server(s, max_batch=max_batch, max_k=k, max_inflight=max_inflight)
     nq = len(Q)
     rows = np.arange(max(n_sat, n_load)) % nq
     Qt = Q[rows]
@@ -192,8 +193,8 @@ def served_bench(s, Q, gt, k, ef, factor, n_sat=200_000, n_load=400_000, loads=(
     rec = recall_at(ids.reshape(-1, k)[:nq], gt[: min(nq, n_sat)], k)
     log(f"served: saturation {sat:.0f} QPS (ef={ef}, factor={factor}), recall {rec:.4f}, {srv.stats()}")
     rng = np.random.default_rng(synth.ARRIVAL_SEED)
-    out = {"saturation_qps": sat, "ef": ef, "refine_factor": factor, "recall": rec, "max_batch": 8192,
-           "max_inflight": 2, "loads": []}
+    out = {"saturation_qps": sat, "ef": ef, "refine_factor": factor, "recall": rec, "max_batch": max_batch,
+           "max_inflight": max_inflight, "loads": []}
     for f in loads:
         rate = f * sat
         arr = np.cumsum(rng.exponential(1e9 / rate, n_load)).astype(np.int64)
@@ -350,7 +351,7 @@ def main():
     ap.add_argument("--no-served", action="store_true", help="skip the served-form (continuous batching) run")
     ap.add_argument("--no-pmc", action="store_true", help="skip the two rocprofv3 --pmc traffic passes")
     ap.add_argument("--served", type=str, default=None,
-                    help="FACTOR:EF - run only the served-form measurement at this operating point")
+                    help="FACTOR:EF[:MAX_BATCH[:MAX_INFLIGHT]] - run only the served-form measurement")
     ap.add_argument("--small", action="store_true", help="100k-row version for quick checks")
     ap.add_argument("--plain", type=int, default=200, help="queries timed in the plain one-query form")
     ap.add_argument("--calib-copy", action="store_true", help="with --point: also run a 512 MiB device copy")
@@ -409,8 +410,10 @@ def main():
     # below recall 0.95 on this workload; factors 4 and 8 are swept for the
     # headline (same knob as Refiner's constructor argument)
     if args.served:
-        sf, se = args.served.split(":")
-        print(json.dumps(served_bench(s, Q, gt, k, int(se), float(sf))))
+        parts = args.served.split(":")
+        mb = int(parts[2]) if len(parts) > 2 else 8192
+        mi = int(parts[3]) if len(parts) > 3 else 3
+        print(json.dumps(served_bench(s, Q, gt, k, int(parts[1]), float(parts[0]), max_batch=mb, max_inflight=mi)))
         return
     if args.point:
         pf, pe = args.point.split(":")

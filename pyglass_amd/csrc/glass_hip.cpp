@@ -514,10 +514,21 @@ int32_t round_cap(int32_t cap) {
     return c;
 }
 
+// Per-launch state for batches that run concurrently on several streams:
+// their own work counter and visited bitsets (max_slots x vis_stride words),
+// and optionally (id, dist) pair outputs.
+struct BatchState {
+    uint32_t *visited = nullptr;
+    int32_t max_slots = 0;
+    int32_t *counter = nullptr;
+    uint64_t *out_pairs = nullptr;
+};
+
 // Enqueue one batch: queries/outputs are device pointers.
 int run_batch(glass_hip_searcher &s, const float *qs_dev, int32_t nq, const int32_t *ef_dev, const int32_t *k_dev,
               int32_t ef_uniform, int32_t k_uniform, int32_t max_cap, int32_t max_ef, int32_t mode,
-              const int64_t *off_dev, int32_t *ids_dev, float *dists_dev, int32_t *counters_dev, hipStream_t stream) {
+              const int64_t *off_dev, int32_t *ids_dev, float *dists_dev, int32_t *counters_dev, hipStream_t stream,
+              const BatchState *bs = nullptr) {
     if (!s.has_data) return fail(GLASS_HIP_ESTATE, "search before set_data");
     if (nq <= 0) return GLASS_HIP_OK;
     if (int r = ensure_stream(s)) return r;
@@ -532,11 +543,15 @@ int run_batch(glass_hip_searcher &s, const float *qs_dev, int32_t nq, const int3
     lc.spec = s.force_spec >= 0 ? s.force_spec : tn.spec;
     const size_t lds = lds_bytes_per_wave(s.index(), lc.cap_max, lc.vis_hash) * (size_t)lc.waves_per_block;
     if (lds > 160 * 1024) return fail(GLASS_HIP_EINVAL, "pool capacity %d needs %zu bytes of LDS", lc.cap_max, lds);
-    const int32_t blocks = grid_blocks(s, lc, nq);
+    int32_t blocks = grid_blocks(s, lc, nq);
+    if (bs) blocks = std::max(1, std::min(blocks, bs->max_slots / lc.waves_per_block));
     lc.blocks = blocks;
-    if (int r = ensure_slots(s, blocks * lc.waves_per_block)) return r;
-    if (!s.d_counter.p) HIP_TRY(s.d_counter.alloc(1));
-    HIP_TRY(launch_fill_i32(s.d_counter.p, 1, 0, stream));
+    if (!bs) {
+        if (int r = ensure_slots(s, blocks * lc.waves_per_block)) return r;
+        if (!s.d_counter.p) HIP_TRY(s.d_counter.alloc(1));
+    }
+    int32_t *counter = bs ? bs->counter : s.d_counter.p;
+    HIP_TRY(launch_fill_i32(counter, 1, 0, stream));
     QueryBatch qb{};
     qb.queries = qs_dev;
     qb.nq = nq;
@@ -550,11 +565,12 @@ int run_batch(glass_hip_searcher &s, const float *qs_dev, int32_t nq, const int3
     qb.out_ids = ids_dev;
     qb.out_dists = dists_dev;
     qb.out_counters = counters_dev;
+    qb.out_pairs = bs ? bs->out_pairs : nullptr;
     SlotState ss{};
-    ss.visited = s.d_visited.p;
+    ss.visited = bs ? bs->visited : s.d_visited.p;
     ss.vis_stride = s.vis_stride;
-    ss.nslots = s.nslots;
-    ss.work_counter = s.d_counter.p;
+    ss.nslots = bs ? bs->max_slots : s.nslots;
+    ss.work_counter = counter;
     HIP_TRY(launch_search(s.index(), qb, lc, ss, stream));
     s.last_launch = lc;
     return GLASS_HIP_OK;
@@ -1106,13 +1122,19 @@ int glass_hip_searcher_sync(glass_hip_searcher *s) {
 //
 // Requests (one or more queries with their own ef and k) queue on the host.
 // A dispatcher thread packs whatever is queued, up to max_batch queries, into
-// a pinned staging slot, copies it to the device in one transfer, launches the
-// persistent search kernel with per-query ef/k (SearchBatch semantics per
-// query) and copies the results back, all on the searcher's stream, with up
-// to max_inflight batches enqueued at once.  A batch is formed as soon as a
-// slot is free and anything is queued, so a lone request never waits for a
-// batch to fill.  Each query's result depends only on (query, ef, k, index),
-// so a request gets the same ids and distances alone or in any mix.
+// a free staging slot, copies it to the device in one transfer and launches
+// the persistent search kernel with per-query ef/k (SearchBatch semantics per
+// query).  Each slot has its own stream, work counter and per-wave visited
+// bitsets, so up to max_inflight batches run on the GPU at once and a new
+// batch fills the compute units a finishing batch leaves.  The kernel writes
+// every query's results straight into host-mapped memory as (id, distance)
+// pairs, one 64-bit store each, over a sentinel the host wrote before the
+// launch; a request completes as soon as none of its results holds the
+// sentinel, not when the slowest query of its batch does, and no fence is
+// needed on the device.  A batch is formed as soon as
+// a slot is free and anything is queued.  Each query's result depends only on
+// (query, ef, k, index), so a request gets the same ids and distances alone
+// or in any mix.
 
 namespace {
 
@@ -1135,25 +1157,36 @@ struct ServerReq {
     std::vector<float> own_dists;
     int64_t arrival_ns = 0, done_ns = 0;
     int32_t state = RQ_QUEUED;
+    int64_t *group_done = nullptr;   // replay(): completed-request count of its call (under the server lock)
 };
 
 struct ServerSlot {
-    // pinned staging: [ef | k | off | queries] in, [ids | dists] out
-    uint8_t *h_in = nullptr, *h_out = nullptr;
-    DevBuf<uint8_t> d_in, d_out;
+    // pinned staging in: [ef | k | off | queries]; host-mapped coherent out:
+    // (id, dist) pairs the kernel writes with one 64-bit store each
+    uint8_t *h_in = nullptr;
+    uint64_t *h_out = nullptr;
+    uint64_t *d_out = nullptr;       // device address of h_out
+    DevBuf<uint8_t> d_in;
+    DevBuf<int32_t> counter;
+    uint32_t *visited = nullptr;     // this slot's share of the server's visited bitsets
+    hipStream_t stream = nullptr;
     hipEvent_t done = nullptr;
     std::vector<ServerReq *> reqs;
+    std::vector<int32_t> req_pos;    // first query of each request in the batch
+    std::vector<int64_t> req_res;    // first result of each request
+    std::vector<uint8_t> req_done;
+    int32_t pending = 0;             // requests of this batch not yet completed
     int32_t nq = 0;
-    int64_t nres = 0;
 };
 
 }  // namespace
 
 struct glass_hip_server {
     glass_hip_searcher *s = nullptr;
-    int32_t max_batch = 0, max_k = 0, max_inflight = 0, dim = 0;
-    size_t in_meta = 0, in_bytes = 0, out_ids_bytes = 0, out_bytes = 0;
+    int32_t max_batch = 0, max_k = 0, max_inflight = 0, dim = 0, slots_per_batch = 0;
+    size_t in_meta = 0, in_bytes = 0, out_bytes = 0;
     std::vector<std::unique_ptr<ServerSlot>> slots;
+    DevBuf<uint32_t> visited;
     std::thread worker;
 
     std::mutex mu;
@@ -1172,6 +1205,7 @@ struct glass_hip_server {
             if (sl->h_in) (void)hipHostFree(sl->h_in);
             if (sl->h_out) (void)hipHostFree(sl->h_out);
             if (sl->done) (void)hipEventDestroy(sl->done);
+            if (sl->stream) (void)hipStreamDestroy(sl->stream);
         }
     }
 };
@@ -1186,11 +1220,12 @@ void server_fail_all(glass_hip_server &sv, int code, const std::string &msg) {
     sv.pending.clear();
     sv.pending_n.store(0);
     for (auto &sl : sv.slots)
-        for (ServerReq *r : sl->reqs) r->state = RQ_FAILED;
+        for (ServerReq *r : sl->reqs)
+            if (r->state != RQ_DONE) r->state = RQ_FAILED;
     sv.cv_done.notify_all();
 }
 
-// Packs the requests of `sl` and enqueues copy-in, search and copy-out.
+// Packs the requests of `sl` and enqueues copy-in and search on the slot's stream.
 int server_launch(glass_hip_server &sv, ServerSlot &sl) {
     glass_hip_searcher &s = *sv.s;
     const int32_t mb = sv.max_batch;
@@ -1200,7 +1235,11 @@ int server_launch(glass_hip_server &sv, ServerSlot &sl) {
     float *h_q = (float *)(sl.h_in + sv.in_meta);
     int32_t pos = 0, max_cap = 1, max_ef = 1;
     int64_t res = 0;
+    sl.req_pos.clear();
+    sl.req_res.clear();
     for (ServerReq *r : sl.reqs) {
+        sl.req_pos.push_back(pos);
+        sl.req_res.push_back(res);
         std::memcpy(h_q + (size_t)pos * sv.dim, r->q, sizeof(float) * (size_t)r->nq * sv.dim);
         for (int32_t j = 0; j < r->nq; ++j) {
             h_ef[pos + j] = r->ef;
@@ -1212,45 +1251,72 @@ int server_launch(glass_hip_server &sv, ServerSlot &sl) {
         max_cap = std::max(max_cap, std::max(inner_k(s, r->k), r->ef));
         max_ef = std::max(max_ef, r->ef);
     }
+    sl.req_done.assign(sl.reqs.size(), 0);
+    sl.pending = (int32_t)sl.reqs.size();
     sl.nq = pos;
-    sl.nres = res;
+    // sentinel id 0xFEFEFEFE in every result this batch writes (ids are -1 or
+    // in [0, N)); no kernel of this slot is running
+    std::memset(sl.h_out, 0xFE, 8 * (size_t)res);
     std::lock_guard<std::mutex> lk(s.mu);
     const size_t in_used = sv.in_meta + sizeof(float) * (size_t)pos * sv.dim;
-    HIP_TRY(hipMemcpyAsync(sl.d_in.p, sl.h_in, in_used, hipMemcpyHostToDevice, s.stream));
+    HIP_TRY(hipMemcpyAsync(sl.d_in.p, sl.h_in, in_used, hipMemcpyHostToDevice, sl.stream));
     const int32_t *d_ef = (const int32_t *)sl.d_in.p;
     const int32_t *d_k = d_ef + mb;
     const int64_t *d_off = (const int64_t *)(sl.d_in.p + 8 * (size_t)mb);
     const float *d_q = (const float *)(sl.d_in.p + sv.in_meta);
-    int32_t *d_ids = (int32_t *)sl.d_out.p;
-    float *d_dists = (float *)(sl.d_out.p + sv.out_ids_bytes);
-    if (int r = run_batch(s, d_q, pos, d_ef, d_k, 0, 0, max_cap, max_ef, MODE_BATCH, d_off, d_ids, d_dists, nullptr,
-                          s.stream))
+    BatchState bs;
+    bs.visited = sl.visited;
+    bs.max_slots = sv.slots_per_batch;
+    bs.counter = sl.counter.p;
+    bs.out_pairs = sl.d_out;
+    if (int r = run_batch(s, d_q, pos, d_ef, d_k, 0, 0, max_cap, max_ef, MODE_BATCH, d_off, nullptr, nullptr, nullptr,
+                          sl.stream, &bs))
         return r;
-    HIP_TRY(hipMemcpyAsync(sl.h_out, d_ids, 4 * (size_t)res, hipMemcpyDeviceToHost, s.stream));
-    HIP_TRY(hipMemcpyAsync(sl.h_out + sv.out_ids_bytes, d_dists, 4 * (size_t)res, hipMemcpyDeviceToHost, s.stream));
-    HIP_TRY(hipEventRecord(sl.done, s.stream));
+    HIP_TRY(hipEventRecord(sl.done, sl.stream));
     return GLASS_HIP_OK;
 }
 
-void server_finish(glass_hip_server &sv, ServerSlot &sl) {
-    const int64_t t = now_ns();
-    const int32_t *ids = (const int32_t *)sl.h_out;
-    const float *dists = (const float *)(sl.h_out + sv.out_ids_bytes);
-    std::lock_guard<std::mutex> lk(sv.mu);
-    int64_t off = 0;
-    for (ServerReq *r : sl.reqs) {
+constexpr uint32_t kSentinelId = 0xFEFEFEFEu;
+
+// Completes the requests of a running slot whose results have all arrived
+// (no pair still holds the sentinel id; every request when `all` is set: the
+// batch has finished).  Returns true when every request of the slot is complete.
+bool server_collect(glass_hip_server &sv, ServerSlot &sl, bool all) {
+    int64_t t = 0;
+    bool completed = false;
+    for (size_t i = 0; i < sl.reqs.size(); ++i) {
+        if (sl.req_done[i]) continue;
+        ServerReq *r = sl.reqs[i];
         const size_t n = (size_t)r->nq * r->k;
-        std::memcpy(r->ids, ids + off, 4 * n);
-        std::memcpy(r->dists, dists + off, 4 * n);
-        off += (int64_t)n;
+        const uint64_t *res = sl.h_out + sl.req_res[i];
+        bool ready = true;
+        if (!all) {
+            for (size_t j = 0; j < n && ready; ++j)
+                ready = (uint32_t)__atomic_load_n(&res[j], __ATOMIC_ACQUIRE) != kSentinelId;
+        }
+        if (!ready) continue;
+        if (!t) t = now_ns();
+        std::lock_guard<std::mutex> lk(sv.mu);
+        completed = true;
+        for (size_t j = 0; j < n; ++j) {
+            const uint64_t pr = __atomic_load_n(&res[j], __ATOMIC_ACQUIRE);
+            r->ids[j] = (int32_t)(uint32_t)pr;
+            uint32_t db = (uint32_t)(pr >> 32);
+            std::memcpy(&r->dists[j], &db, 4);
+        }
         r->done_ns = t;
         r->state = RQ_DONE;
+        if (r->group_done) *r->group_done += 1;
+        sl.req_done[i] = 1;
+        sl.pending -= 1;
+        sv.n_requests += 1;
+        sv.n_queries += r->nq;
     }
-    sv.n_batches += 1;
-    sv.n_queries += sl.nq;
-    sv.n_requests += (int64_t)sl.reqs.size();
-    sl.reqs.clear();
-    sv.cv_done.notify_all();
+    if (completed) {
+        std::lock_guard<std::mutex> lk(sv.mu);
+        sv.cv_done.notify_all();
+    }
+    return sl.pending == 0;
 }
 
 void server_loop(glass_hip_server *psv) {
@@ -1259,30 +1325,43 @@ void server_loop(glass_hip_server *psv) {
         server_fail_all(sv, GLASS_HIP_EDEVICE, "server thread: hipSetDevice failed");
         return;
     }
-    std::deque<int> inflight;  // slot indices in launch order
+    std::vector<int> running;  // slot indices with a batch in flight
     std::vector<int> free_slots;
     for (int i = (int)sv.slots.size() - 1; i >= 0; --i) free_slots.push_back(i);
+    auto drain = [&]() {
+        for (auto &sl : sv.slots) (void)hipStreamSynchronize(sl->stream);
+    };
     for (;;) {
         bool progressed = false;
-        while (!inflight.empty()) {
-            ServerSlot &sl = *sv.slots[inflight.front()];
-            hipError_t e = hipEventQuery(sl.done);
-            if (e == hipErrorNotReady) break;
-            if (e != hipSuccess) {
+        for (size_t i = 0; i < running.size();) {
+            ServerSlot &sl = *sv.slots[running[i]];
+            const hipError_t e = hipEventQuery(sl.done);
+            if (e != hipSuccess && e != hipErrorNotReady) {
                 server_fail_all(sv, GLASS_HIP_EDEVICE, std::string("search batch failed: ") + hipGetErrorString(e));
-                (void)hipStreamSynchronize(sv.s->stream);
+                drain();
                 return;
             }
-            server_finish(sv, sl);
-            free_slots.push_back(inflight.front());
-            inflight.pop_front();
-            progressed = true;
+            const bool finished = e == hipSuccess;
+            if (sl.pending && server_collect(sv, sl, finished)) progressed = true;
+            if (finished) {
+                if (sl.pending) server_collect(sv, sl, true);
+                {
+                    std::lock_guard<std::mutex> lk(sv.mu);
+                    sv.n_batches += 1;
+                }
+                sl.reqs.clear();
+                free_slots.push_back(running[i]);
+                running.erase(running.begin() + (ptrdiff_t)i);
+                progressed = true;
+                continue;
+            }
+            ++i;
         }
-        if (!free_slots.empty()) {
+        if (!free_slots.empty() && (running.empty() || sv.pending_n.load(std::memory_order_relaxed) > 0)) {
             ServerSlot &sl = *sv.slots[free_slots.back()];
-            if (inflight.empty() || sv.pending_n.load(std::memory_order_relaxed) > 0) {
+            {
                 std::unique_lock<std::mutex> lk(sv.mu);
-                if (sv.pending.empty() && inflight.empty()) {
+                if (sv.pending.empty() && running.empty()) {
                     if (sv.stop) return;
                     sv.cv_work.wait_for(lk, std::chrono::milliseconds(2));
                 }
@@ -1299,16 +1378,16 @@ void server_loop(glass_hip_server *psv) {
             if (!sl.reqs.empty()) {
                 if (int rc = server_launch(sv, sl)) {
                     server_fail_all(sv, rc, g_last_error);
-                    (void)hipStreamSynchronize(sv.s->stream);
+                    drain();
                     return;
                 }
-                inflight.push_back(free_slots.back());
+                running.push_back(free_slots.back());
                 free_slots.pop_back();
                 progressed = true;
             }
         }
-        if (!progressed && !inflight.empty()) std::this_thread::yield();
-        if (!progressed && inflight.empty()) {
+        if (!progressed && !running.empty()) std::this_thread::yield();
+        if (!progressed && running.empty()) {
             std::lock_guard<std::mutex> lk(sv.mu);
             if (sv.stop && sv.pending.empty()) return;
         }
@@ -1316,8 +1395,10 @@ void server_loop(glass_hip_server *psv) {
 }
 
 int server_enqueue(glass_hip_server &sv, ServerReq *r) {
-    if (r->nq <= 0 || r->nq > sv.max_batch) return fail(GLASS_HIP_EINVAL, "request of %d queries (max_batch %d)", r->nq, sv.max_batch);
-    if (r->k <= 0 || r->k > sv.max_k || r->ef <= 0) return fail(GLASS_HIP_EINVAL, "bad ef %d / k %d (max_k %d)", r->ef, r->k, sv.max_k);
+    if (r->nq <= 0 || r->nq > sv.max_batch)
+        return fail(GLASS_HIP_EINVAL, "request of %d queries (max_batch %d)", r->nq, sv.max_batch);
+    if (r->k <= 0 || r->k > sv.max_k || r->ef <= 0)
+        return fail(GLASS_HIP_EINVAL, "bad ef %d / k %d (max_k %d)", r->ef, r->k, sv.max_k);
     if (round_cap(std::max(inner_k(*sv.s, r->k), r->ef)) > 4096)
         return fail(GLASS_HIP_EINVAL, "max(k, ef) = %d exceeds 4096", std::max(inner_k(*sv.s, r->k), r->ef));
     return GLASS_HIP_OK;
@@ -1340,13 +1421,11 @@ int glass_hip_server_create(glass_hip_searcher *s, int32_t max_batch, int32_t ma
     sv->dim = s->dim;
     sv->in_meta = 16 * (size_t)max_batch;
     sv->in_bytes = sv->in_meta + sizeof(float) * (size_t)max_batch * s->dim;
-    sv->out_ids_bytes = 4 * (size_t)max_batch * max_k;
-    sv->out_bytes = 2 * sv->out_ids_bytes;
+    sv->out_bytes = 8 * (size_t)max_batch * max_k;
     {
         std::lock_guard<std::mutex> lk(s->mu);
         if (int r = ensure_stream(*s)) return r;
-        // size the per-slot state for a full grid now, so no batch reallocates
-        // memory that an earlier batch in flight still uses
+        // waves one batch can have resident: the most any launch shape allows
         LaunchConfig lc{};
         int32_t most = 0;
         for (int32_t team : {4, 8, 16}) {
@@ -1360,16 +1439,21 @@ int glass_hip_server_create(glass_hip_searcher *s, int32_t max_batch, int32_t ma
             }
         }
         if (most == 0) most = 32 * std::max(1, s->cu_count);  // 8 waves per SIMD, 4 SIMDs per CU
-        if (int r = ensure_slots(*s, most)) return r;
-        if (!s->d_counter.p) HIP_TRY(s->d_counter.alloc(1));
+        sv->slots_per_batch = most;
+        if (int r = ensure_slots(*s, 1)) return r;  // sets the bitset stride
+        // a query clears its wave's bitset when it first needs it
+        HIP_TRY(sv->visited.alloc((size_t)max_inflight * (size_t)most * (size_t)s->vis_stride));
     }
     for (int32_t i = 0; i < max_inflight; ++i) {
         sv->slots.push_back(std::make_unique<ServerSlot>());
         ServerSlot &sl = *sv->slots.back();
         HIP_TRY(hipHostMalloc((void **)&sl.h_in, sv->in_bytes, hipHostMallocDefault));
-        HIP_TRY(hipHostMalloc((void **)&sl.h_out, sv->out_bytes, hipHostMallocDefault));
+        HIP_TRY(hipHostMalloc((void **)&sl.h_out, sv->out_bytes, hipHostMallocMapped | hipHostMallocCoherent));
+        HIP_TRY(hipHostGetDevicePointer((void **)&sl.d_out, sl.h_out, 0));
         HIP_TRY(sl.d_in.alloc(sv->in_bytes));
-        HIP_TRY(sl.d_out.alloc(sv->out_bytes));
+        HIP_TRY(sl.counter.alloc(1));
+        sl.visited = sv->visited.p + (size_t)i * (size_t)sv->slots_per_batch * (size_t)s->vis_stride;
+        HIP_TRY(hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking));
         HIP_TRY(hipEventCreateWithFlags(&sl.done, hipEventDisableTiming));
     }
     glass_hip_server *p = sv.get();
@@ -1459,6 +1543,7 @@ int glass_hip_server_replay(glass_hip_server *sv, const float *qs, int64_t nreq,
     if (!sv || nreq < 0 || (nreq && (!qs || !nq || !ef || !k || !arrival_ns || !latency_ns || !dst || !scores)))
         return fail(GLASS_HIP_EINVAL, "bad replay arguments");
     std::vector<ServerReq> reqs((size_t)nreq);
+    int64_t done_count = 0;  // guarded by sv->mu
     int64_t qoff = 0, roff = 0;
     for (int64_t i = 0; i < nreq; ++i) {
         ServerReq &r = reqs[(size_t)i];
@@ -1468,6 +1553,7 @@ int glass_hip_server_replay(glass_hip_server *sv, const float *qs, int64_t nreq,
         if (int rc = server_enqueue(*sv, &r)) return rc;
         if (i && arrival_ns[i] < arrival_ns[i - 1]) return fail(GLASS_HIP_EINVAL, "arrivals must be sorted");
         r.q = qs + qoff * sv->dim;
+        r.group_done = &done_count;
         r.ids = dst + roff;
         r.dists = scores + roff;
         qoff += r.nq;
@@ -1495,12 +1581,7 @@ int glass_hip_server_replay(glass_hip_server *sv, const float *qs, int64_t nreq,
         sv->cv_work.notify_one();
     }
     std::unique_lock<std::mutex> lk(sv->mu);
-    sv->cv_done.wait(lk, [&] {
-        if (sv->err_code) return true;
-        for (const ServerReq &r : reqs)
-            if (r.state != RQ_DONE) return false;
-        return true;
-    });
+    sv->cv_done.wait(lk, [&] { return sv->err_code || done_count == nreq; });
     if (sv->err_code) {
         // requests of this call may still sit in the queue: drop them
         for (auto it = sv->pending.begin(); it != sv->pending.end();) {
@@ -1535,7 +1616,8 @@ void glass_hip_server_free(glass_hip_server *sv) {
     }
     sv->cv_work.notify_all();
     if (sv->worker.joinable()) sv->worker.join();
-    if (sv->s && sv->s->stream) (void)hipStreamSynchronize(sv->s->stream);
+    for (auto &sl : sv->slots)
+        if (sl->stream) (void)hipStreamSynchronize(sl->stream);
     delete sv;
 }
 

@@ -52,6 +52,8 @@ struct QueryBatch {
     int32_t *out_ids;
     float *out_dists;        // may be null
     int32_t *out_counters;   // nq x 3: dist_cmps, expansions, upper evals; may be null
+    uint64_t *out_pairs;     // if set: results as (id | dist bits << 32), one 64-bit store each, instead of
+                             // out_ids/out_dists (lets a host poll host-mapped results without fences)
 };
 
 struct LaunchConfig {

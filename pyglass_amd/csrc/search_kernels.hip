@@ -766,6 +766,17 @@ __device__ __forceinline__ uint32_t pool_merge_reg_lds(const DeviceIndex &ix, co
     return pool_merge_reg(ix, w, ps, kj, ij, l < m, l, pred_row);
 }
 
+// One result: separate id / distance arrays, or one 64-bit (id, dist) pair
+// store, which a host polling host-mapped memory sees whole.
+__device__ __forceinline__ void put_result(const QueryBatch &qb, int64_t pos, int32_t id, float d) {
+    if (qb.out_pairs) {
+        qb.out_pairs[pos] = (uint64_t)(uint32_t)id | ((uint64_t)__float_as_uint(d) << 32);
+    } else {
+        qb.out_ids[pos] = id;
+        if (qb.out_dists) qb.out_dists[pos] = d;
+    }
+}
+
 // ---------------------------------------------------------------------------
 // The search kernel.
 
@@ -991,8 +1002,7 @@ __global__ __launch_bounds__(256) void search_kernel(DeviceIndex ix, QueryBatch 
                     id = (int32_t)(w.pid[i] & kIdMask);
                     d = is_int_variant(V) ? (float)int_of_key(w.pk[i]) : float_of_key(w.pk[i]);
                 }
-                qb.out_ids[ob + i] = id;
-                if (qb.out_dists) qb.out_dists[ob + i] = d;
+                put_result(qb, ob + i, id, d);
             }
         } else {
             // Refiner::RefineImpl (refiner.hpp:92-110): exact FP32 distances of the
@@ -1018,15 +1028,9 @@ __global__ __launch_bounds__(256) void search_kernel(DeviceIndex ix, QueryBatch 
                     const uint32_t ki = w.pk[i], ii = w.pid[i] & kIdMask;
                     rank += (ki < kj) | ((ki == kj) & (ii < ij));
                 }
-                if (rank < k) {
-                    qb.out_ids[ob + rank] = (int32_t)ij;
-                    if (qb.out_dists) qb.out_dists[ob + rank] = float_of_key(kj);
-                }
+                if (rank < k) put_result(qb, ob + rank, (int32_t)ij, float_of_key(kj));
             }
-            for (int i = m + l; i < k; i += 64) {
-                qb.out_ids[ob + i] = -1;
-                if (qb.out_dists) qb.out_dists[ob + i] = __builtin_inff();
-            }
+            for (int i = m + l; i < k; i += 64) put_result(qb, ob + i, -1, __builtin_inff());
         }
         if (qb.out_counters && l == 0) {
             qb.out_counters[(int64_t)qi * 3 + 0] = cnt_d;
