@@ -57,10 +57,12 @@ int glass_hip_graph_info(const glass_hip_graph *g, int32_t *n, int32_t *k, int32
 int glass_hip_graph_copy(const glass_hip_graph *g, int32_t *adj, int32_t *eps);
 void glass_hip_graph_free(glass_hip_graph *g);
 /* Index(index_type, metric, R, L).build(data) / build_graph
- * (python/bindings.cc:60-88,231-237): CPU graph construction.  "HNSW"
- * restates glass/hnsw/hnsw.hpp:27-69 (M = R/2, efConstruction = L) with
- * fp32 distances; num_threads <= 0 uses every core.  *seconds (nullable)
- * receives the construction time. */
+ * (python/bindings.cc:60-88,231-237): CPU graph construction with fp32
+ * distances.  "HNSW" restates glass/hnsw/hnsw.hpp:27-69 (M = R/2,
+ * efConstruction = L); "NSG" / "SSG" restate glass/nsg/nsg.hpp:22-434 over
+ * an NN-Descent kNN graph (L2 whatever `metric` says, as create_nsg).
+ * num_threads <= 0 uses every core.  *seconds (nullable) receives the
+ * construction time. */
 int glass_hip_build_graph(const char *index_type, const float *data, int32_t n, int32_t dim, const char *metric,
                           int32_t R, int32_t L, int32_t num_threads, glass_hip_graph **out, double *seconds);
 

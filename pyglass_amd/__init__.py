@@ -122,8 +122,9 @@ class Graph:
 
 class Index:
     """glass.Index(index_type, metric, quant="FP32", R=32, L=200)
-    (python/bindings.cc:60-88): CPU graph construction.  Only "HNSW" is
-    built here; distances are fp32 on the raw rows whatever `quant` says."""
+    (python/bindings.cc:60-88): CPU graph construction, "HNSW", "NSG" or
+    "SSG" (NSG/SSG ignore the metric and build on L2, as create_nsg does).
+    Distances are fp32 on the raw rows whatever `quant` says."""
 
     def __init__(self, index_type: str, metric: str, quant: str = "FP32", R: int = 32, L: int = 200):
         self.index_type, self.metric, self.quant, self.R, self.L = index_type, metric, quant, int(R), int(L)

@@ -42,6 +42,8 @@ namespace glass_build {
 int build_hnsw(const float *X, int32_t n, int32_t d, bool ip, int32_t R, int32_t L, int32_t num_threads,
                std::vector<int32_t> &adj, int32_t &ep, std::vector<int32_t> &levels, std::vector<int32_t> &lists,
                double *seconds);
+int build_nsg(const float *X, int32_t n, int32_t d, int32_t R, int32_t L, bool ssg, int32_t num_threads,
+              std::vector<int32_t> &adj, int32_t &ep, double *seconds);
 }  // namespace glass_build
 
 using namespace glass_hip;
@@ -753,6 +755,18 @@ int glass_hip_build_graph(const char *index_type, const float *data, int32_t n, 
         return fail(GLASS_HIP_EINVAL, "bad build arguments");
     std::string t(index_type), m(metric);
     if (m != "L2" && m != "IP") return fail(GLASS_HIP_EINVAL, "unknown metric '%s'", metric);
+    if (t == "NSG" || t == "SSG") {
+        // create_nsg (glass/nsg/nsg.hpp:411-434): the metric is ignored, graphs are L2
+        auto g = std::make_unique<glass_hip_graph>();
+        int32_t ep = 0;
+        if (glass_build::build_nsg(data, n, dim, R, L, t == "SSG", num_threads, g->adj, ep, seconds))
+            return fail(GLASS_HIP_EINVAL, "%s build failed", index_type);
+        g->N = n;
+        g->K = R;
+        g->eps = {ep};
+        *out = g.release();
+        return GLASS_HIP_OK;
+    }
     if (t != "HNSW") return fail(GLASS_HIP_EINVAL, "unsupported index type '%s'", index_type);
     auto g = std::make_unique<glass_hip_graph>();
     std::vector<int32_t> levels, lists;
