@@ -107,6 +107,11 @@ int glass_hip_searcher_search_batch_efk(glass_hip_searcher *s, const float *qs, 
 int glass_hip_searcher_search_batch_dev(glass_hip_searcher *s, const float *qs_dev, int32_t nq, int32_t k,
                                         int32_t mode, int32_t *dst_dev, float *scores_dev, int32_t *counters_dev,
                                         void *stream);
+/* search_batch_dev replays a captured hipGraph (counter-reset kernel +
+ * search kernel) when every kernel argument and the stream repeat the
+ * previous call's; the first call with new arguments runs eagerly and is
+ * then captured.  on = 0 always launches eagerly.  Default on. */
+int glass_hip_searcher_set_graph_capture(glass_hip_searcher *s, int32_t on);
 /* Re-rank depth: reorder_k = int(k * factor).  0 restores the reference's
  * get_refine_factor(quantizer) (graph_searcher.hpp:227-238: 1.5 for
  * SQ8U/SQ8/SQ4U, 3.0 for SQ2U), which Refiner takes as its constructor

@@ -262,6 +262,10 @@ class Searcher:
         cap the persistent grid (0 = fill the device)."""
         _lib.check(self._L.glass_hip_searcher_set_launch(self._h, int(team), int(waves_per_block), int(blocks)))
 
+    def set_graph_capture(self, on: bool = True):
+        """search_batch_dev replays a captured hipGraph when its arguments repeat."""
+        _lib.check(self._L.glass_hip_searcher_set_graph_capture(self._h, int(bool(on))))
+
     def set_gather(self, mode: int = -1):
         """Integer codes: 1 gathers every neighbour's code row before the
         visited test, 0 only the unvisited ones, -1 lets optimize() choose."""

@@ -152,6 +152,16 @@ struct glass_hip_graph {
     std::vector<int32_t> up_lists;
 };
 
+// Everything one batch launch passes to its kernels, plus the stream: two
+// launches with equal keys are identical, so a captured graph can replay one.
+struct GraphKey {
+    glass_hip::DeviceIndex ix;
+    glass_hip::QueryBatch qb;
+    glass_hip::LaunchConfig lc;
+    glass_hip::SlotState ss;
+    hipStream_t stream;
+};
+
 struct glass_hip_searcher {
     glass_hip_graph graph;
     int32_t metric = MK_L2, quant = QK_SQ8U, refine = QK_NONE;
@@ -213,7 +223,13 @@ struct glass_hip_searcher {
     std::vector<float> optimize_queries;
     int32_t sample_points_num = 0;
 
+    // device-pointer batches replay a captured graph when their arguments repeat
+    bool graph_capture = true;
+    hipGraphExec_t gexec = nullptr;
+    GraphKey gkey{};
+
     ~glass_hip_searcher() {
+        if (gexec) (void)hipGraphExecDestroy(gexec);
         if (stream) (void)hipStreamDestroy(stream);
     }
 
@@ -530,7 +546,7 @@ struct BatchState {
 int run_batch(glass_hip_searcher &s, const float *qs_dev, int32_t nq, const int32_t *ef_dev, const int32_t *k_dev,
               int32_t ef_uniform, int32_t k_uniform, int32_t max_cap, int32_t max_ef, int32_t mode,
               const int64_t *off_dev, int32_t *ids_dev, float *dists_dev, int32_t *counters_dev, hipStream_t stream,
-              const BatchState *bs = nullptr) {
+              const BatchState *bs = nullptr, bool graph = false) {
     if (!s.has_data) return fail(GLASS_HIP_ESTATE, "search before set_data");
     if (nq <= 0) return GLASS_HIP_OK;
     if (int r = ensure_stream(s)) return r;
@@ -553,7 +569,6 @@ int run_batch(glass_hip_searcher &s, const float *qs_dev, int32_t nq, const int3
         if (!s.d_counter.p) HIP_TRY(s.d_counter.alloc(1));
     }
     int32_t *counter = bs ? bs->counter : s.d_counter.p;
-    HIP_TRY(launch_fill_i32(counter, 1, 0, stream));
     QueryBatch qb{};
     qb.queries = qs_dev;
     qb.nq = nq;
@@ -573,8 +588,51 @@ int run_batch(glass_hip_searcher &s, const float *qs_dev, int32_t nq, const int3
     ss.vis_stride = s.vis_stride;
     ss.nslots = bs ? bs->max_slots : s.nslots;
     ss.work_counter = counter;
-    HIP_TRY(launch_search(s.index(), qb, lc, ss, stream));
+    const DeviceIndex ix = s.index();
+    // counter reset (a fill kernel, not hipMemsetAsync, so it can sit in a graph) + search
+    auto enqueue = [&]() -> hipError_t {
+        hipError_t e = launch_fill_i32(counter, 1, 0, stream);
+        return e == hipSuccess ? launch_search(ix, qb, lc, ss, stream) : e;
+    };
     s.last_launch = lc;
+    if (!graph) {
+        HIP_TRY(enqueue());
+        return GLASS_HIP_OK;
+    }
+    // Graph replay: the key is every kernel argument plus the stream, so a
+    // replay launches exactly what an eager call with these arguments would.
+    GraphKey key{};
+    key.ix = ix;
+    key.qb = qb;
+    key.lc = lc;
+    key.ss = ss;
+    key.stream = stream;
+    if (s.gexec && std::memcmp(&key, &s.gkey, sizeof(key)) == 0) {
+        HIP_TRY(hipGraphLaunch(s.gexec, stream));
+        return GLASS_HIP_OK;
+    }
+    // first call with these arguments: run eagerly (this also does any
+    // one-time set-up), then capture the same two launches for next time
+    HIP_TRY(enqueue());
+    if (s.gexec) {
+        (void)hipGraphExecDestroy(s.gexec);
+        s.gexec = nullptr;
+    }
+    hipGraph_t gr = nullptr;
+    HIP_TRY(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+    const hipError_t ce = enqueue();
+    const hipError_t ee = hipStreamEndCapture(stream, &gr);
+    if (ce != hipSuccess || ee != hipSuccess) {
+        if (gr) (void)hipGraphDestroy(gr);
+        return fail(GLASS_HIP_EDEVICE, "graph capture failed: %s", hipGetErrorString(ce != hipSuccess ? ce : ee));
+    }
+    const hipError_t ie = hipGraphInstantiate(&s.gexec, gr, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(gr);
+    if (ie != hipSuccess) {
+        s.gexec = nullptr;
+        return fail(GLASS_HIP_EDEVICE, "graph instantiate failed: %s", hipGetErrorString(ie));
+    }
+    s.gkey = key;
     return GLASS_HIP_OK;
 }
 
@@ -923,7 +981,14 @@ int glass_hip_searcher_search_batch_dev(glass_hip_searcher *s, const float *qs_d
     const int32_t max_cap = std::max(inner_k(*s, k), s->ef);
     if (round_cap(max_cap) > 4096) return fail(GLASS_HIP_EINVAL, "max(k, ef) = %d exceeds 4096", max_cap);
     return run_batch(*s, qs_dev, nq, nullptr, nullptr, s->ef, k, max_cap, s->ef, mode, nullptr, dst_dev, scores_dev,
-                     counters_dev, (hipStream_t)stream);
+                     counters_dev, (hipStream_t)stream, nullptr, s->graph_capture);
+}
+
+int glass_hip_searcher_set_graph_capture(glass_hip_searcher *s, int32_t on) {
+    if (!s) return fail(GLASS_HIP_EINVAL, "null searcher");
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->graph_capture = on != 0;
+    return GLASS_HIP_OK;
 }
 
 int glass_hip_searcher_optimize(glass_hip_searcher *s, int32_t num_threads) {

@@ -216,3 +216,40 @@ def test_neighbour_gather_modes_match_oracle(c1, quant, metric, team, gather):
         np.testing.assert_array_equal(I, Io)
         np.testing.assert_array_equal(D, Do)
         assert s.get_last_search_dist_cmps() == o.get_last_search_dist_cmps()
+
+
+def test_graph_replay_matches_eager(c1):
+    # search_batch_dev captures (counter-reset kernel + search kernel) after
+    # the first eager call and replays the graph while the arguments repeat;
+    # every replay is checked against the eager result, with the outputs
+    # cleared before it and eager searches + synchronisations in between
+    import torch
+
+    X, Q, path = c1
+    s = pg.Searcher(pg.Graph(path), X, "L2", "SQ4U", "FP32")
+    dev = torch.device("cuda:0")
+    st = torch.cuda.Stream(device=dev)
+    q = torch.from_numpy(Q[:500].copy()).to(dev)
+    ids = torch.empty((500, 10), dtype=torch.int32, device=dev)
+    dis = torch.empty((500, 10), dtype=torch.float32, device=dev)
+    for ef in [64, 32]:
+        s.set_ef(ef)
+        I_ref, D_ref = s.batch_search(Q[:500], 10)
+        for rep in range(5):
+            print(f"graph replay ef={ef} rep={rep}", flush=True)
+            with torch.cuda.stream(st):
+                ids.fill_(-7)
+                dis.fill_(0.0)
+            s.search_batch_dev(q.data_ptr(), 500, 10, ids.data_ptr(), dis.data_ptr(), stream=st.cuda_stream)
+            st.synchronize()
+            np.testing.assert_array_equal(ids.cpu().numpy(), I_ref)
+            np.testing.assert_array_equal(dis.cpu().numpy(), D_ref)
+            # the program's own eager work between replays
+            Ie, _ = s.batch_search(Q[500:600], 10)
+            torch.cuda.synchronize()
+    s.set_graph_capture(False)
+    with torch.cuda.stream(st):
+        ids.fill_(-7)
+    s.search_batch_dev(q.data_ptr(), 500, 10, ids.data_ptr(), dis.data_ptr(), stream=st.cuda_stream)
+    st.synchronize()
+    np.testing.assert_array_equal(ids.cpu().numpy(), I_ref)
