@@ -217,70 +217,88 @@ def served_bench(s, Q, gt, k, ef, factor, n_sat=200_000, n_load=400_000, loads=(
 
 
 def _pmc_values(db_path, last=3):
-    """Mean per-dispatch counter value of the last `last` search_kernel
-    dispatches and of the 512 MiB copyBuffer dispatches in a rocpd database."""
+    """Per counter: mean per-dispatch value over the last `last` search_kernel
+    dispatches, and over the 512 MiB copyBuffer dispatches, of a rocpd database."""
     import sqlite3
-    from collections import OrderedDict
+    from collections import OrderedDict, defaultdict
 
     con = sqlite3.connect(db_path)
     try:
         cols = [c[0] for c in con.execute("select * from counters_collection limit 1").description]
         key = "dispatch_id" if "dispatch_id" in cols else "start"
-        rows = list(con.execute(f"select kernel_name, grid_size, value, {key} from counters_collection order by start"))
+        rows = list(con.execute(f"select kernel_name, grid_size, counter_name, value, {key} "
+                                f"from counters_collection order by start"))
     finally:
         con.close()
     disp = OrderedDict()
-    for name, grid, val, d in rows:
-        e = disp.setdefault(d, [name, grid, 0.0])
-        e[2] += val
+    for name, grid, ctr, val, d in rows:
+        e = disp.setdefault(d, [name, grid, defaultdict(float)])
+        e[2][ctr] += val
     sk = [v[2] for v in disp.values() if "search_kernel" in v[0]][-last:]
     cp = [v for v in disp.values() if "copyBuffer" in v[0]]
-    if not sk or not cp:
-        return None, None
-    big = max(v[1] for v in cp)
-    cpv = [v[2] for v in cp if v[1] == big]
-    return sum(sk) / len(sk), sum(cpv) / len(cpv)
+    kern = {c: sum(x[c] for x in sk) / len(sk) for c in sk[0]} if sk else {}
+    copy = {}
+    if cp:
+        big = max(v[1] for v in cp)
+        cpv = [v[2] for v in cp if v[1] == big]
+        copy = {c: sum(x[c] for x in cpv) / len(cpv) for c in cpv[0]}
+    return kern, copy
 
 
-def pmc_traffic(point: str, launch: str, timeout_s: int = 240):
-    """HBM bytes per search launch at an operating point: one rocprofv3 --pmc
-    run per counter (FETCH_SIZE, WRITE_SIZE) of `bench.py --point`, each
-    counter scaled by what it reads for a 512 MiB device copy in the same run
-    (bytes = KiB * 1024 * 512 MiB / copy bytes counted)."""
+def pmc_run(counters, point: str, launch: str, timeout_s: int = 240):
+    """One rocprofv3 --pmc child run of `bench.py --point` with a 512 MiB
+    device copy; returns (search-kernel values, copy values) per counter, or
+    (None, error)."""
     import shutil
     import subprocess
     import tempfile
 
     prof = shutil.which("rocprofv3") or "/opt/rocm/bin/rocprofv3"
     if not os.path.exists(prof):
-        return None, {"error": "rocprofv3 not found"}
+        return None, "rocprofv3 not found"
     tmp = tempfile.mkdtemp(prefix="glass_pmc_", dir="/tmp")
     env = dict(os.environ, TMPDIR="/tmp")
-    detail = {"point": point, "launch": launch}
-    total = 0.0
     try:
-        for ctr in ("FETCH_SIZE", "WRITE_SIZE"):
-            d = os.path.join(tmp, ctr.lower())
-            cmd = [prof, "--pmc", ctr, "-d", d, "-o", "run", "--", sys.executable, os.path.abspath(__file__),
-                   "--point", point, "--launch", launch, "--steps", "2", "--warmup", "1", "--no-cpu", "--calib-copy"]
-            log(f"pmc: {ctr} pass")
-            try:
-                subprocess.run(cmd, cwd="/tmp", env=env, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL,
-                               timeout=timeout_s)
-            except subprocess.TimeoutExpired:
-                return None, dict(detail, error=f"{ctr} pass timed out")
-            dbs = [os.path.join(r, f) for r, _, fs in os.walk(d) for f in fs if f.endswith("_results.db")]
-            if not dbs:
-                return None, dict(detail, error=f"{ctr} pass wrote no database")
-            kern, copy = _pmc_values(dbs[0])
-            if kern is None:
-                return None, dict(detail, error=f"{ctr}: dispatches not found")
-            scale = (512 << 20) / (copy * 1024.0)
-            detail[ctr] = {"kib_per_launch": kern, "copy_512MiB_kib": copy, "scale": scale,
-                           "bytes_per_launch": kern * 1024.0 * scale}
-            total += kern * 1024.0 * scale
+        cmd = [prof, "--pmc", *counters, "-d", tmp, "-o", "run", "--", sys.executable, os.path.abspath(__file__),
+               "--point", point, "--launch", launch, "--steps", "2", "--warmup", "1", "--no-cpu", "--calib-copy"]
+        log(f"pmc: {' '.join(counters)} pass")
+        try:
+            subprocess.run(cmd, cwd="/tmp", env=env, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL,
+                           timeout=timeout_s)
+        except subprocess.TimeoutExpired:
+            return None, "pass timed out"
+        dbs = [os.path.join(r, f) for r, _, fs in os.walk(tmp) for f in fs if f.endswith("_results.db")]
+        if not dbs:
+            return None, "pass wrote no database"
+        kern, copy = _pmc_values(dbs[0])
+        if not kern:
+            return None, "search_kernel dispatches not found"
+        return kern, copy
     finally:
         shutil.rmtree(tmp, ignore_errors=True)
+
+
+def info_launch_waves(s):
+    """Waves per CU the persistent grid of the current shape was sized for."""
+    ki, lz = s.kernel_info(), s.get_launch()
+    return ki["blocks_per_cu"] * lz["waves_per_block"]
+
+
+def pmc_traffic(point: str, launch: str):
+    """HBM bytes per search launch at an operating point: one rocprofv3 --pmc
+    run per counter (FETCH_SIZE, WRITE_SIZE), each counter scaled by what it
+    reads for a 512 MiB device copy in the same run (bytes = KiB * 1024 *
+    512 MiB / copy bytes counted)."""
+    detail = {"point": point, "launch": launch}
+    total = 0.0
+    for ctr in ("FETCH_SIZE", "WRITE_SIZE"):
+        kern, copy = pmc_run([ctr], point, launch)
+        if kern is None or ctr not in copy:
+            return None, dict(detail, error=f"{ctr}: {copy if kern is None else 'copy dispatch not found'}")
+        scale = (512 << 20) / (copy[ctr] * 1024.0)
+        detail[ctr] = {"kib_per_launch": kern[ctr], "copy_512MiB_kib": copy[ctr], "scale": scale,
+                       "bytes_per_launch": kern[ctr] * 1024.0 * scale}
+        total += kern[ctr] * 1024.0 * scale
     return total, detail
 
 
@@ -476,13 +494,22 @@ def main():
     plain_recall = recall_at(plain_ids, gt[:npl], k)
     log(f"plain: {npl} queries, factor={best_f} ef={op['ef']}: {plain_qps:.0f} QPS, recall {plain_recall:.4f}")
     served = None if args.no_served else served_bench(s, Q, gt, k, op["ef"], best_f)
-    traffic, traffic_detail = None, None
+    traffic, traffic_detail, occ = None, None, None
     if not args.no_pmc:
         s.set_ef(op["ef"])
         lz = s.get_launch()
-        traffic, traffic_detail = pmc_traffic(f"{best_f}:{op['ef']}",
-                                              f"{lz['team']}:{lz['waves_per_block']}:{lz['vis_entries']}")
+        pt, ln = f"{best_f}:{op['ef']}", f"{lz['team']}:{lz['waves_per_block']}:{lz['vis_entries']}"
+        traffic, traffic_detail = pmc_traffic(pt, ln)
         log(f"pmc: traffic {traffic} bytes per launch ({traffic_detail})")
+        # occupancy (time-averaged resident waves / 32 per CU) and LDS bank
+        # conflicts, as rocprofv3's derived metrics define them
+        kern, _ = pmc_run(["OccupancyPercent", "LdsBankConflict", "LDSBankConflict"], pt, ln)
+        if kern is not None:
+            occ = {"occupancy_percent": kern.get("OccupancyPercent"),
+                   "lds_bank_conflicts_per_access": kern.get("LdsBankConflict"),
+                   "lds_bank_conflict_time_percent": kern.get("LDSBankConflict"),
+                   "resident_waves_per_cu_allocated": info_launch_waves(s)}
+        log(f"pmc: occupancy / LDS {occ}")
 
     cpu = None
     parity = None
@@ -545,6 +572,7 @@ def main():
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": traffic,
             "traffic_detail": traffic_detail,
+            "occupancy_lds": occ,
             "algorithmic_bytes_per_launch": alg_bytes,
             "kernel_ms": op["kernel_ms"],
         },

@@ -41,7 +41,8 @@ int glass_hip_device_count(void);
 /* ---- Graph (glass::Graph<int32_t>) ------------------------------------ */
 
 /* Graph(filename, format) / Graph::load: format "glass" (graph.hpp:136-153),
- * "nsg" (graph.hpp:220-245) or "diskann" (graph.hpp:155-182). */
+ * "nsg" (graph.hpp:220-245), "diskann" (graph.hpp:155-182) or "hnswlib"
+ * (graph.hpp:184-218, level 0 of a graph with at most one upper level). */
 int glass_hip_graph_load(const char *path, const char *format, glass_hip_graph **out);
 /* Graph::save (graph.hpp:106-119), .glass layout incl. HNSW upper layers. */
 int glass_hip_graph_save(const glass_hip_graph *g, const char *path);

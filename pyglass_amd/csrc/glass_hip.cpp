@@ -400,6 +400,44 @@ int load_nsg(const char *path, glass_hip_graph &g) {
     return GLASS_HIP_OK;
 }
 
+// Graph::load_hnswlib (graph.hpp:184-218): an hnswlib index file.  Header:
+// u64 offset_level0, u64 max_elements (= N), u64 cur_count, u64
+// size_per_element, u64 label_offset, u64 offset_data, i32 max_level, i32
+// entry point, u64 maxM, u64 maxM0 (= K), u64 M, f64 mult, u64 ef_construction;
+// then N elements of size_per_element bytes starting with the level-0 link
+// list (count, maxM0 ids).  Only level 0 is read; files with more than one
+// upper level are refused, as in the reference.  The count is the low 16
+// bits of the list header (hnswlib keeps its delete mark in the third byte).
+int load_hnswlib(const char *path, glass_hip_graph &g) {
+    std::ifstream f(path, std::ios::binary);
+    if (!f) return fail(GLASS_HIP_EIO, "cannot open %s", path);
+    uint64_t off0 = 0, max_el = 0, cur = 0, spe = 0, lab = 0, offd = 0, maxM = 0, maxM0 = 0, M = 0, efc = 0;
+    int32_t max_level = 0, ep = 0;
+    double mult = 0.0;
+    if (!read_all(f, &off0, 8) || !read_all(f, &max_el, 8) || !read_all(f, &cur, 8) || !read_all(f, &spe, 8) ||
+        !read_all(f, &lab, 8) || !read_all(f, &offd, 8) || !read_all(f, &max_level, 4) || !read_all(f, &ep, 4) ||
+        !read_all(f, &maxM, 8) || !read_all(f, &maxM0, 8) || !read_all(f, &M, 8) || !read_all(f, &mult, 8) ||
+        !read_all(f, &efc, 8))
+        return fail(GLASS_HIP_EIO, "%s: bad hnswlib header", path);
+    if (max_level > 1) return fail(GLASS_HIP_EINVAL, "%s: hnswlib graphs with %d levels are not supported", path, max_level);
+    if (max_el == 0 || max_el >= (uint64_t)INT32_MAX || maxM0 == 0 || maxM0 > 4096 || spe < 4 * (1 + maxM0))
+        return fail(GLASS_HIP_EIO, "%s: bad hnswlib sizes", path);
+    g.N = (int32_t)max_el;
+    g.K = (int32_t)maxM0;
+    g.eps = {ep};
+    g.adj.assign((size_t)g.N * g.K, -1);
+    std::vector<uint8_t> buf(spe);
+    for (int32_t i = 0; i < g.N; ++i) {
+        if (!read_all(f, buf.data(), spe)) return fail(GLASS_HIP_EIO, "%s: truncated at element %d", path, i);
+        uint32_t hdr;
+        std::memcpy(&hdr, buf.data(), 4);
+        const uint32_t c = hdr & 0xffffu;
+        if (c > maxM0) return fail(GLASS_HIP_EIO, "%s: element %d has %u links > maxM0", path, i, c);
+        std::memcpy(g.adj.data() + (size_t)i * g.K, buf.data() + 4, 4 * (size_t)c);
+    }
+    return GLASS_HIP_OK;
+}
+
 // Graph::load_diskann (graph.hpp:155-182): u64 size, K, ep, u64, rows (cnt, ids).
 int load_diskann(const char *path, glass_hip_graph &g) {
     std::ifstream f(path, std::ios::binary);
@@ -728,6 +766,7 @@ int glass_hip_graph_load(const char *path, const char *format, glass_hip_graph *
     if (fmt == "glass") r = load_glass(path, *g);
     else if (fmt == "nsg") r = load_nsg(path, *g);
     else if (fmt == "diskann") r = load_diskann(path, *g);
+    else if (fmt == "hnswlib") r = load_hnswlib(path, *g);
     else return fail(GLASS_HIP_EINVAL, "unknown graph format '%s'", fmt.c_str());
     if (r) return r;
     *out = g.release();

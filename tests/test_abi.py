@@ -67,6 +67,43 @@ def test_graph_from_arrays_and_nsg_format(tmp_path):
     np.testing.assert_array_equal(e3, [3])
 
 
+def _write_hnswlib(path, adj, ep, dim=4, max_level=0, delete_mark_row=None):
+    """hnswlib's saveIndex layout as glass/graph.hpp:184-218 reads it."""
+    n, m0 = adj.shape
+    spe = 4 * (1 + m0) + 4 * dim + 8
+    with open(path, "wb") as f:
+        np.array([0, n, n, spe, 4 * (1 + m0) + 4 * dim, 4 * (1 + m0)], np.uint64).tofile(f)
+        np.array([max_level, ep], np.int32).tofile(f)
+        np.array([m0 // 2, m0, m0 // 2], np.uint64).tofile(f)
+        np.array([1.0 / np.log(m0 // 2)], np.float64).tofile(f)
+        np.array([100], np.uint64).tofile(f)
+        for i in range(n):
+            row = adj[i][adj[i] >= 0]
+            hdr = len(row) | ((1 << 16) if i == delete_mark_row else 0)
+            np.array([hdr], np.uint32).tofile(f)
+            np.concatenate([row, np.zeros(m0 - len(row), np.int32)]).astype(np.int32).tofile(f)
+            np.zeros(dim, np.float32).tofile(f)
+            np.array([i], np.uint64).tofile(f)
+        np.zeros(n, np.int32).tofile(f)  # per-element upper-level list sizes
+
+
+def test_hnswlib_format(tmp_path):
+    rng = np.random.default_rng(1)
+    n, m0 = 40, 6
+    adj = np.full((n, m0), -1, np.int32)
+    for i in range(n):
+        c = int(rng.integers(1, m0 + 1))
+        adj[i, :c] = rng.choice([j for j in range(n) if j != i], c, replace=False)
+    p = tmp_path / "g.hnswlib"
+    _write_hnswlib(p, adj, 7, delete_mark_row=5)
+    a, e = pg.Graph(str(p), "hnswlib").arrays()
+    np.testing.assert_array_equal(a, adj)
+    np.testing.assert_array_equal(e, [7])
+    _write_hnswlib(p, adj, 7, max_level=2)
+    with pytest.raises(pg.GlassError):
+        pg.Graph(str(p), "hnswlib")
+
+
 def test_errors_are_raised_not_crashes(tmp_path):
     with pytest.raises(pg.GlassError):
         pg.Graph(str(tmp_path / "missing.glass"))
