@@ -4,13 +4,16 @@
 // reference's best-first expansion exactly:
 //   * the candidate pool is glass::LinearPool (glass/neighbor.hpp:320-433)
 //     kept sorted in LDS as (key, id|checked) pairs;
-//   * the R neighbours of a popped node are loaded by lanes 0..R-1 from the
-//     adjacency row (one coalesced 4R-byte read), tested-and-set in an exact
-//     per-slot visited bitset (global, cleared per query) with atomic OR, and
-//     compacted in adjacency order (glass/searcher/graph_searcher.hpp:196-224);
+//   * the R neighbours of a popped node are loaded from the adjacency row
+//     (one coalesced 4R-byte read), tested-and-set in an exact visited set
+//     (a per-wave LDS hash table that moves to the slot's global bitset if
+//     it fills past half) and kept in adjacency order
+//     (glass/searcher/graph_searcher.hpp:196-224);
 //   * their distances are computed by "teams" of lanes from coalesced code
-//     gathers (integer quantizers: v_dot4 on byte-expanded differences, exact;
-//     float quantizers: 16-lane accumulators in the reference's order);
+//     gathers (integer quantizers: |q|^2 + |c|^2 - 2 q.c with packed dot
+//     products, exact; float quantizers: the reference's 16 accumulators and
+//     reduction tree, bit-exact); integer codes may be gathered for every
+//     neighbour before the visited test to overlap the two;
 //   * the whole batch of new candidates is merged into the pool at once.  The
 //     merge reproduces sequential LinearPool::insert exactly: the pool after
 //     inserting c_1..c_m in order holds the `cap` smallest (dist, insertion
@@ -545,9 +548,9 @@ struct PoolState {
 
 // LinearPool::pop (neighbor.hpp:398-405): mark data[cur] checked, advance
 // cur to the next unchecked entry.
-__device__ __forceinline__ uint32_t pool_pop(const WaveLds &w, PoolState &ps) {
+__device__ __forceinline__ uint32_t pool_pop(const WaveLds &w, PoolState &ps, uint32_t known = kIdMask) {
     const int l = lane_id();
-    const uint32_t uid = uni(w.pid[ps.cur]);
+    const uint32_t uid = known != kIdMask ? known : uni(w.pid[ps.cur]);
     wave_sync();
     if (l == 0) w.pid[ps.cur] = uid | kChecked;
     int c = ps.cur + 1;
@@ -927,7 +930,7 @@ __global__ __launch_bounds__(256) void search_kernel(DeviceIndex ix, QueryBatch 
         uint32_t pred = kIdMask;
         int32_t pred_row = -1;
         while (ps.cur < ps.size && ps.cur < ps.ef) {
-            const uint32_t u = pool_pop(w, ps);
+            const uint32_t u = pool_pop(w, ps, pred);
             cnt_e += 1;
             const int32_t v = (u == pred) ? pred_row : (nj < ix.K ? ix.adj[(int64_t)u * ix.K + nj] : -1);
             if (spec) {
