@@ -697,11 +697,11 @@ __device__ uint32_t pool_merge_reg(const DeviceIndex &ix, const WaveLds &w, Pool
             mm &= mm - 1;
             const uint32_t ki = (uint32_t)__builtin_amdgcn_readlane((int)kj, i);
             r += (ki < kj) | ((ki == kj) & (i < l));
+            // unused chunks hold kKeyMax, which no distance key reaches, so
+            // their ballots are empty: no per-chunk branch in this loop
             int cnt = 0;
 #pragma unroll
-            for (int c = 0; c < kMaxChunks; ++c) {
-                if (c < nch) cnt += __builtin_popcountll(ballot(pkc[c] <= ki));
-            }
+            for (int c = 0; c < kMaxChunks; ++c) cnt += __builtin_popcountll(ballot(pkc[c] <= ki));
             ub = (l == i) ? cnt : ub;
         }
         const int i0 = __builtin_ctzll(ballot(act && r == 0));
