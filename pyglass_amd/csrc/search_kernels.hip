@@ -75,22 +75,24 @@ __device__ __forceinline__ int mbcnt(uint64_t m) {
     return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
 }
 
+// Minimum over the wave (all lanes active), returned to every lane: DPP
+// within each row of 16 (quads, then the first lane of each 8- and 16-lane
+// group), then the four row minima read with readlane.
 __device__ __forceinline__ uint32_t wave_min_u32(uint32_t v) {
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) {
-        uint32_t o = (uint32_t)__shfl_xor((int)v, off);
-        v = o < v ? o : v;
-    }
-    return v;
-}
-
-__device__ __forceinline__ uint64_t wave_min_u64(uint64_t v) {
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) {
-        uint64_t o = __shfl_xor(v, off);
-        v = o < v ? o : v;
-    }
-    return v;
+    uint32_t o = (uint32_t)dpp_xor1((int)v);
+    v = o < v ? o : v;
+    o = (uint32_t)dpp_xor2((int)v);
+    v = o < v ? o : v;
+    o = (uint32_t)dpp_down4((int)v);
+    v = o < v ? o : v;
+    o = (uint32_t)dpp_down8((int)v);
+    v = o < v ? o : v;
+    const uint32_t a = (uint32_t)__builtin_amdgcn_readlane((int)v, 0);
+    const uint32_t b = (uint32_t)__builtin_amdgcn_readlane((int)v, 16);
+    const uint32_t c = (uint32_t)__builtin_amdgcn_readlane((int)v, 32);
+    const uint32_t d = (uint32_t)__builtin_amdgcn_readlane((int)v, 48);
+    const uint32_t ab = a < b ? a : b, cd = c < d ? c : d;
+    return ab < cd ? ab : cd;
 }
 
 __device__ __forceinline__ int32_t sdot4(uint32_t a, uint32_t b, int32_t c) {
@@ -875,9 +877,9 @@ __global__ __launch_bounds__(256) void search_kernel(DeviceIndex ix, QueryBatch 
                     cnt_d += n;
                     cnt_u += n;
                     const uint32_t mk = l < n ? w.ck[l] : kKeyMax;
-                    const uint64_t best = wave_min_u64(((uint64_t)mk << 32) | (uint32_t)l);
-                    const uint32_t bk = uni((uint32_t)(best >> 32));
-                    const int bi = uni((int)(best & 0xffffffffu));
+                    // smallest key; its first lane is the first minimum in list order
+                    const uint32_t bk = uni(wave_min_u32(mk));
+                    const int bi = __builtin_ctzll(ballot(mk == bk));
                     wave_sync();
                     if (bk < du) {  // strict <: the first minimum of the pass wins
                         du = bk;
