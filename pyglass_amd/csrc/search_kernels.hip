@@ -276,8 +276,12 @@ struct IntQuery {
     // Distance of a row whose slices c4 the lanes of one group hold; valid in
     // the group's first lane (part 0).
     __device__ __forceinline__ int32_t row_dist(const uint4 (&c4)[DPL / 4], bool on) const {
-        int32_t acc = 0;
-        if (on && part < nparts) {
+        // no branch: lanes past the row's parts hold zero query and code
+        // dwords (partial 0), and a row that is off is loaded as zeros and
+        // its result is never used
+        (void)on;
+        int32_t acc;
+        {
             uint32_t qc = 0, cc = 0;
             int32_t ip = 0;
 #pragma unroll
@@ -678,7 +682,7 @@ __device__ uint32_t pool_merge_reg(const DeviceIndex &ix, const WaveLds &w, Pool
 #pragma unroll
     for (int c = 0; c < kMaxChunks; ++c) {
         const int p = 64 * c + l;
-        const bool v = c < nch && p < ps.size;
+        const bool v = p < ps.size;  // also false for every chunk past nch
         pkc[c] = v ? w.pk[p] : kKeyMax;
         pic[c] = v ? w.pid[p] : 0u;
     }
@@ -746,7 +750,7 @@ __device__ uint32_t pool_merge_reg(const DeviceIndex &ix, const WaveLds &w, Pool
         for (int c = 0; c < kMaxChunks; ++c) {
             const int p = 64 * c + l;
             const int dst = p + sh[c];
-            if (c < nch && p >= lo && p < ps.size && dst < ps.cap) {
+            if (p >= lo && p < ps.size && dst < ps.cap) {
                 w.pk[dst] = pkc[c];
                 w.pid[dst] = pic[c];
             }
