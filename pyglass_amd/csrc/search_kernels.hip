@@ -664,7 +664,8 @@ __device__ void pool_merge_lds(const WaveLds &w, PoolState &ps, int m) {
 }
 
 // Pools of up to kMaxChunks*64 entries are merged from registers: lane l
-// holds entries l, l+64, ... of the pool.  Each candidate's insertion point
+// holds entries l, l+64, ... of the pool, NCH chunks of 64 (NCH = 2 when the
+// launch's pools hold at most 128 entries, else 4).  Each candidate's insertion point
 // (entries with key <= its key) is a sum of ballots, the first moved entry
 // and the first inserted position come from the rank-0 candidate, and all
 // writes happen after all reads.  The node the next pop will take is known
@@ -674,13 +675,14 @@ constexpr int kMaxChunks = 4;
 
 // Candidates: lanes with `cand` set, keys kj / ids ij, insertion order = lane
 // order.  nj is the adjacency-row entry this lane loads for the prefetch.
+template <int NCH>
 __device__ uint32_t pool_merge_reg(const DeviceIndex &ix, const WaveLds &w, PoolState &ps, uint32_t kj,
                                    uint32_t ij, bool cand, int nj, int32_t &pred_row) {
     const int l = lane_id();
     const int nch = (ps.size + 63) >> 6;
-    uint32_t pkc[kMaxChunks], pic[kMaxChunks];
+    uint32_t pkc[NCH], pic[NCH];
 #pragma unroll
-    for (int c = 0; c < kMaxChunks; ++c) {
+    for (int c = 0; c < NCH; ++c) {
         const int p = 64 * c + l;
         const bool v = p < ps.size;  // also false for every chunk past nch
         pkc[c] = v ? w.pk[p] : kKeyMax;
@@ -707,7 +709,7 @@ __device__ uint32_t pool_merge_reg(const DeviceIndex &ix, const WaveLds &w, Pool
             // their ballots are empty: no per-chunk branch in this loop
             int cnt = 0;
 #pragma unroll
-            for (int c = 0; c < kMaxChunks; ++c) cnt += __builtin_popcountll(ballot(pkc[c] <= ki));
+            for (int c = 0; c < NCH; ++c) cnt += __builtin_popcountll(ballot(pkc[c] <= ki));
             ub = (l == i) ? cnt : ub;
         }
         const int i0 = __builtin_ctzll(ballot(act && r == 0));
@@ -734,20 +736,20 @@ __device__ uint32_t pool_merge_reg(const DeviceIndex &ix, const WaveLds &w, Pool
     if (am) {
         const int lo = np0 < ps.cap ? np0 : ps.cap;  // entries before the first insertion do not move
         // shift of each pool entry = number of candidates with a smaller key
-        int sh[kMaxChunks];
+        int sh[NCH];
 #pragma unroll
-        for (int c = 0; c < kMaxChunks; ++c) sh[c] = 0;
+        for (int c = 0; c < NCH; ++c) sh[c] = 0;
         uint64_t m2 = am;
         while (m2) {
             const int i = __builtin_ctzll(m2);
             m2 &= m2 - 1;
             const uint32_t ki = (uint32_t)__builtin_amdgcn_readlane((int)kj, i);
 #pragma unroll
-            for (int c = 0; c < kMaxChunks; ++c) sh[c] += ki < pkc[c];
+            for (int c = 0; c < NCH; ++c) sh[c] += ki < pkc[c];
         }
         wave_sync();
 #pragma unroll
-        for (int c = 0; c < kMaxChunks; ++c) {
+        for (int c = 0; c < NCH; ++c) {
             const int p = 64 * c + l;
             const int dst = p + sh[c];
             if (p >= lo && p < ps.size && dst < ps.cap) {
@@ -767,12 +769,13 @@ __device__ uint32_t pool_merge_reg(const DeviceIndex &ix, const WaveLds &w, Pool
 }
 
 // Candidates 0..m-1 staged in LDS (ck / ci).
+template <int NCH>
 __device__ __forceinline__ uint32_t pool_merge_reg_lds(const DeviceIndex &ix, const WaveLds &w, PoolState &ps, int m,
                                                        int32_t &pred_row) {
     const int l = lane_id();
     const uint32_t kj = l < m ? w.ck[l] : kKeyMax;
     const uint32_t ij = l < m ? w.ci[l] : 0u;
-    return pool_merge_reg(ix, w, ps, kj, ij, l < m, l, pred_row);
+    return pool_merge_reg<NCH>(ix, w, ps, kj, ij, l < m, l, pred_row);
 }
 
 // One result: separate id / distance arrays, or one 64-bit (id, dist) pair
@@ -911,7 +914,7 @@ __global__ __launch_bounds__(256) void search_kernel(DeviceIndex ix, QueryBatch 
             cnt_d += nep;
             if (lc.cap_max <= 64 * kMaxChunks) {
                 int32_t unused = -1;
-                pool_merge_reg_lds(ix, w, ps, nep, unused);
+                pool_merge_reg_lds<kMaxChunks>(ix, w, ps, nep, unused);
             } else {
                 pool_merge_lds(w, ps, nep);
             }
@@ -960,8 +963,10 @@ __global__ __launch_bounds__(256) void search_kernel(DeviceIndex ix, QueryBatch 
                 vs.account(m);
                 const uint32_t kj = key_of_int(iq.row_dist(c4, valid));
                 cnt_d += m;
-                if (lc.cap_max <= 64 * kMaxChunks) {
-                    pred = pool_merge_reg(ix, w, ps, kj, (uint32_t)v, isnew, nj, pred_row);
+                if (lc.cap_max <= 128) {
+                    pred = pool_merge_reg<2>(ix, w, ps, kj, (uint32_t)v, isnew, nj, pred_row);
+                } else if (lc.cap_max <= 64 * kMaxChunks) {
+                    pred = pool_merge_reg<kMaxChunks>(ix, w, ps, kj, (uint32_t)v, isnew, nj, pred_row);
                 } else {
                     if (isnew) {
                         w.ck[mbcnt(nm)] = kj;
@@ -992,8 +997,10 @@ __global__ __launch_bounds__(256) void search_kernel(DeviceIndex ix, QueryBatch 
             wave_sync();
             dists(m);
             cnt_d += m;
-            if (lc.cap_max <= 64 * kMaxChunks) {
-                pred = pool_merge_reg_lds(ix, w, ps, m, pred_row);
+            if (lc.cap_max <= 128) {
+                pred = pool_merge_reg_lds<2>(ix, w, ps, m, pred_row);
+            } else if (lc.cap_max <= 64 * kMaxChunks) {
+                pred = pool_merge_reg_lds<kMaxChunks>(ix, w, ps, m, pred_row);
             } else {
                 pool_merge_lds(w, ps, m);
                 pred = kIdMask;
