@@ -52,12 +52,25 @@ __device__ __forceinline__ float float_of_key(uint32_t k) {
     return __uint_as_float(b);
 }
 
-__device__ __forceinline__ uint64_t ballot(bool p) { return __ballot(p); }
+// The wave64 ballot builtin: the lane mask of a compare is used as it is
+// (__ballot goes through an int and compares again).
+__device__ __forceinline__ uint64_t ballot(bool p) { return __builtin_amdgcn_ballot_w64(p); }
 
 // Marks a wave-uniform value as such (moves it to a scalar register), so that
 // branches on it are scalar branches and arithmetic on it runs on the SALU.
 __device__ __forceinline__ int32_t uni(int32_t v) { return __builtin_amdgcn_readfirstlane(v); }
 __device__ __forceinline__ uint32_t uni(uint32_t v) { return (uint32_t)__builtin_amdgcn_readfirstlane((int)v); }
+
+// A wave-uniform value (already in a scalar register) that the compiler may
+// not hoist tests on out of loops: a hoisted uniform test lives on as a
+// 64-bit lane mask, and with many of them the kernel runs out of scalar
+// registers and spills them to vector lanes.  Tested where used, the value
+// costs one scalar register and an s_cmp.
+__device__ __forceinline__ int32_t at_use(int32_t v) {
+    v = uni(v);
+    asm volatile("" : "+s"(v));
+    return v;
+}
 
 // Cross-lane moves inside groups of lanes with DPP (no LDS round trip, unlike
 // ds_bpermute).  dpp_xor1/dpp_xor2: partner lane l^1 / l^2 (quad_perm);
@@ -309,13 +322,14 @@ struct IntQuery {
             }
             acc = (V == V_SQ8U_IP) ? ip : (int32_t)((uint32_t)qq + cc - 2u * qc);
         }
-        if (slot >= 2) acc += dpp_xor1(acc);
-        if (slot >= 4) acc += dpp_xor2(acc);
-        if (slot >= 8) acc += dpp_down4(acc);
-        if (slot >= 16) acc += dpp_down8(acc);
+        const int32_t s = at_use(slot);
+        if (s >= 2) acc += dpp_xor1(acc);
+        if (s >= 4) acc += dpp_xor2(acc);
+        if (s >= 8) acc += dpp_down4(acc);
+        if (s >= 16) acc += dpp_down8(acc);
         // across rows of 16: the first lane of each 16-lane group holds its partial sum
-        if (slot >= 32) acc += __shfl_xor(acc, 16);
-        if (slot >= 64) acc += __shfl_xor(acc, 32);
+        if (s >= 32) acc += __shfl_xor(acc, 16);
+        if (s >= 64) acc += __shfl_xor(acc, 32);
         return (V == V_SQ8U_IP) ? -acc : acc;
     }
 
@@ -562,7 +576,10 @@ __device__ __forceinline__ uint32_t pool_pop(const WaveLds &w, PoolState &ps, ui
     int c = ps.cur + 1;
     while (c < ps.size) {
         int p = c + l;
-        uint32_t x = p < ps.size ? w.pid[p] : kChecked;
+        // unconditional read: p < size + 63 <= cap_max + 63, which stays
+        // inside pid and the 256-byte ck array that follows it
+        const uint32_t y = w.pid[p];
+        const uint32_t x = p < ps.size ? y : kChecked;
         uint64_t um = ballot(!(x & kChecked));
         if (um) {
             c += __builtin_ctzll(um);
@@ -673,11 +690,11 @@ __device__ void pool_merge_lds(const WaveLds &w, PoolState &ps, int m) {
 // overlaps the rest of the merge and the pop.
 constexpr int kMaxChunks = 4;
 
-// Candidates: lanes with `cand` set, keys kj / ids ij, insertion order = lane
-// order.  nj is the adjacency-row entry this lane loads for the prefetch.
+// Candidates: lanes with `cand` set (cm = the mask of those lanes), keys kj /
+// ids ij, insertion order = lane order.  nj is the adjacency-row entry this lane loads for the prefetch.
 template <int NCH>
 __device__ uint32_t pool_merge_reg(const DeviceIndex &ix, const WaveLds &w, PoolState &ps, uint32_t kj,
-                                   uint32_t ij, bool cand, int nj, int32_t &pred_row) {
+                                   uint32_t ij, bool cand, uint64_t cm, int nj, int32_t &pred_row) {
     const int l = lane_id();
     const int nch = (ps.size + 63) >> 6;
     uint32_t pkc[NCH], pic[NCH];
@@ -685,34 +702,52 @@ __device__ uint32_t pool_merge_reg(const DeviceIndex &ix, const WaveLds &w, Pool
     for (int c = 0; c < NCH; ++c) {
         const int p = 64 * c + l;
         const bool v = p < ps.size;  // also false for every chunk past nch
-        pkc[c] = v ? w.pk[p] : kKeyMax;
-        pic[c] = v ? w.pid[p] : 0u;
+        // unconditional reads, then selects: p < 64 * NCH, and the arrays
+        // that follow pk and pid (pid, ck, ci: at least 512 + round16(4 *
+        // cap_max) bytes) cover 64 * NCH entries for both NCH = 2 and
+        // NCH = 4 (used only when cap_max > 128)
+        const uint32_t xk = w.pk[p], xi = w.pid[p];
+        pkc[c] = v ? xk : kKeyMax;
+        pic[c] = v ? xi : 0u;
     }
-    bool act = cand;
     // insert() rejects d >= data[size-1] when full; such a candidate would
-    // also land past cap in the merge and displaces nothing.
-    if (ps.size == ps.cap) act = act && (kj < w.pk[ps.size - 1]);
-    const uint64_t am = ballot(act);
+    // also land past cap in the merge and displaces nothing.  An empty pool
+    // reads entry 0, which it does not use (size < cap then).
+    const uint32_t last = w.pk[ps.size > 0 ? ps.size - 1 : 0];
+    const bool act = cand && (ps.size < ps.cap || kj < last);
+    const uint64_t am = cm & (ps.size < ps.cap ? ~0ull : ballot(kj < last));  // the lanes' act bits
     int new_size = ps.size, new_cur = ps.cur, np0 = 0x7fffffff;
     uint32_t id0 = 0;
     int r = 0, ub = 0;
+    // le[c]: active candidates whose key is >= pool entry 64c + l; the
+    // entry's shift in the merge is the number of the others, ma - le[c]
+    int le[NCH];
+#pragma unroll
+    for (int c = 0; c < NCH; ++c) le[c] = 0;
     if (am) {
-        // rank among the active candidates by (key, insertion order) and
-        // upper bound in the pool (new keys go after equal old keys)
+        // rank among the active candidates by (key, insertion order) -- one
+        // 64-bit compare of (key, lane) pairs -- and upper bound in the pool
+        // (new keys go after equal old keys)
+        const uint64_t kl = ((uint64_t)kj << 32) | (uint32_t)l;
         uint64_t mm = am;
         while (mm) {
             const int i = __builtin_ctzll(mm);
             mm &= mm - 1;
             const uint32_t ki = (uint32_t)__builtin_amdgcn_readlane((int)kj, i);
-            r += (ki < kj) | ((ki == kj) & (i < l));
+            r += (((uint64_t)ki << 32) | (uint32_t)i) < kl;
             // unused chunks hold kKeyMax, which no distance key reaches, so
             // their ballots are empty: no per-chunk branch in this loop
             int cnt = 0;
 #pragma unroll
-            for (int c = 0; c < NCH; ++c) cnt += __builtin_popcountll(ballot(pkc[c] <= ki));
+            for (int c = 0; c < NCH; ++c) {
+                const bool below = pkc[c] <= ki;
+                cnt += __builtin_popcountll(ballot(below));
+                le[c] += below;
+            }
             ub = (l == i) ? cnt : ub;
         }
-        const int i0 = __builtin_ctzll(ballot(act && r == 0));
+        // masks of plain compares, combined on the scalar unit
+        const int i0 = __builtin_ctzll(am & ballot(r == 0));
         const int ub0 = __builtin_amdgcn_readlane(ub, i0);
         id0 = (uint32_t)__builtin_amdgcn_readlane((int)ij, i0);
         np0 = ub0 < ps.cap ? ub0 : 0x7fffffff;
@@ -721,7 +756,7 @@ __device__ uint32_t pool_merge_reg(const DeviceIndex &ix, const WaveLds &w, Pool
         new_cur = 0x7fffffff;
         if (ps.cur < ps.size) {
             const uint32_t kc = w.pk[ps.cur];
-            new_cur = ps.cur + __builtin_popcountll(ballot(act && kj < kc));
+            new_cur = ps.cur + __builtin_popcountll(am & ballot(kj < kc));
         }
         new_cur = new_cur < np0 ? new_cur : np0;
         new_cur = new_cur < new_size ? new_cur : new_size;
@@ -736,17 +771,10 @@ __device__ uint32_t pool_merge_reg(const DeviceIndex &ix, const WaveLds &w, Pool
     if (am) {
         const int lo = np0 < ps.cap ? np0 : ps.cap;  // entries before the first insertion do not move
         // shift of each pool entry = number of candidates with a smaller key
+        const int ma = __builtin_popcountll(am);
         int sh[NCH];
 #pragma unroll
-        for (int c = 0; c < NCH; ++c) sh[c] = 0;
-        uint64_t m2 = am;
-        while (m2) {
-            const int i = __builtin_ctzll(m2);
-            m2 &= m2 - 1;
-            const uint32_t ki = (uint32_t)__builtin_amdgcn_readlane((int)kj, i);
-#pragma unroll
-            for (int c = 0; c < NCH; ++c) sh[c] += ki < pkc[c];
-        }
+        for (int c = 0; c < NCH; ++c) sh[c] = ma - le[c];
         wave_sync();
 #pragma unroll
         for (int c = 0; c < NCH; ++c) {
@@ -775,7 +803,7 @@ __device__ __forceinline__ uint32_t pool_merge_reg_lds(const DeviceIndex &ix, co
     const int l = lane_id();
     const uint32_t kj = l < m ? w.ck[l] : kKeyMax;
     const uint32_t ij = l < m ? w.ci[l] : 0u;
-    return pool_merge_reg<NCH>(ix, w, ps, kj, ij, l < m, l, pred_row);
+    return pool_merge_reg<NCH>(ix, w, ps, kj, ij, l < m, m >= 64 ? ~0ull : (1ull << m) - 1ull, l, pred_row);
 }
 
 // One result: separate id / distance arrays, or one 64-bit (id, dist) pair
@@ -934,17 +962,18 @@ __global__ __launch_bounds__(256) void search_kernel(DeviceIndex ix, QueryBatch 
         // visited test runs while the gathers are in flight, and the rows
         // that turn out visited are discarded.  Candidates stay in their
         // lanes, so lane order is adjacency order for the merge.
-        const bool spec = is_int_variant(V) && lc.spec && iq.slot * ix.K <= 64;
+        const int32_t spec = is_int_variant(V) && lc.spec && iq.slot * ix.K <= 64;
         const int nj = spec ? l / iq.slot : l;  // adjacency entry this lane reads
+        const uint64_t lead_k = ballot(iq.part == 0 && nj < ix.K);  // first lane of each group with an entry
         uint32_t pred = kIdMask;
         int32_t pred_row = -1;
         while (ps.cur < ps.size && ps.cur < ps.ef) {
             const uint32_t u = pool_pop(w, ps, pred);
             cnt_e += 1;
             const int32_t v = (u == pred) ? pred_row : (nj < ix.K ? ix.adj[(int64_t)u * ix.K + nj] : -1);
-            if (spec) {
+            if (at_use(spec)) {
                 const bool lead = iq.part == 0;
-                const uint64_t inv = ballot(lead && nj < ix.K && v == -1);
+                const uint64_t inv = lead_k & ballot(v == -1);
                 const int n = inv ? __builtin_ctzll(inv) / iq.slot : ix.K;
                 const bool valid = nj < n;
                 uint4 c4[DPL / 4];
@@ -963,10 +992,10 @@ __global__ __launch_bounds__(256) void search_kernel(DeviceIndex ix, QueryBatch 
                 vs.account(m);
                 const uint32_t kj = key_of_int(iq.row_dist(c4, valid));
                 cnt_d += m;
-                if (lc.cap_max <= 128) {
-                    pred = pool_merge_reg<2>(ix, w, ps, kj, (uint32_t)v, isnew, nj, pred_row);
+                if (at_use(lc.cap_max) <= 128) {
+                    pred = pool_merge_reg<2>(ix, w, ps, kj, (uint32_t)v, isnew, nm, nj, pred_row);
                 } else if (lc.cap_max <= 64 * kMaxChunks) {
-                    pred = pool_merge_reg<kMaxChunks>(ix, w, ps, kj, (uint32_t)v, isnew, nj, pred_row);
+                    pred = pool_merge_reg<kMaxChunks>(ix, w, ps, kj, (uint32_t)v, isnew, nm, nj, pred_row);
                 } else {
                     if (isnew) {
                         w.ck[mbcnt(nm)] = kj;
@@ -997,7 +1026,7 @@ __global__ __launch_bounds__(256) void search_kernel(DeviceIndex ix, QueryBatch 
             wave_sync();
             dists(m);
             cnt_d += m;
-            if (lc.cap_max <= 128) {
+            if (at_use(lc.cap_max) <= 128) {
                 pred = pool_merge_reg_lds<2>(ix, w, ps, m, pred_row);
             } else if (lc.cap_max <= 64 * kMaxChunks) {
                 pred = pool_merge_reg_lds<kMaxChunks>(ix, w, ps, m, pred_row);
