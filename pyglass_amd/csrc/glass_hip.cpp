@@ -1550,10 +1550,17 @@ int glass_hip_server_create(glass_hip_searcher *s, int32_t max_batch, int32_t ma
             if (!team_valid(s->index(), team)) continue;
             lc.team = team;
             for (int32_t w = 1; w <= 4; ++w) {
-                lc.waves_per_block = w;
-                lc.cap_max = 64;
-                lc.vis_hash = 256;
-                most = std::max(most, std::max(1, max_blocks_per_cu(s->index(), lc)) * std::max(1, s->cu_count) * w);
+                // both kernel paths (gather on / off, small / large pools)
+                for (int32_t sp = 0; sp <= 1; ++sp) {
+                    for (int32_t cap : {64, 129}) {
+                        lc.waves_per_block = w;
+                        lc.cap_max = cap;
+                        lc.vis_hash = 256;
+                        lc.spec = sp;
+                        most = std::max(most,
+                                        std::max(1, max_blocks_per_cu(s->index(), lc)) * std::max(1, s->cu_count) * w);
+                    }
+                }
             }
         }
         if (most == 0) most = 32 * std::max(1, s->cu_count);  // 8 waves per SIMD, 4 SIMDs per CU

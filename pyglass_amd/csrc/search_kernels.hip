@@ -818,10 +818,14 @@ __device__ __forceinline__ void put_result(const QueryBatch &qb, int64_t pos, in
 }
 
 // ---------------------------------------------------------------------------
-// The search kernel.
+// The search kernel.  P = 1 is the common case compiled on its own: pools of
+// at most 128 entries (two-chunk register merge) and, for integer codes, the
+// all-neighbour gather; the other paths are left out, which keeps the hot
+// loop small and its scalar registers unspilled.  P = 0 handles everything.
 
-template <int V, int DPL>
+template <int V, int DPL, int P>
 __global__ __launch_bounds__(256) void search_kernel(DeviceIndex ix, QueryBatch qb, LaunchConfig lc, SlotState ss) {
+    constexpr bool kFast = P == 1;
     extern __shared__ __attribute__((aligned(16))) uint8_t lds_raw[];
     const int l = lane_id();
     const int wv = threadIdx.x >> 6;
@@ -962,7 +966,7 @@ __global__ __launch_bounds__(256) void search_kernel(DeviceIndex ix, QueryBatch 
         // visited test runs while the gathers are in flight, and the rows
         // that turn out visited are discarded.  Candidates stay in their
         // lanes, so lane order is adjacency order for the merge.
-        const int32_t spec = is_int_variant(V) && lc.spec && iq.slot * ix.K <= 64;
+        const int32_t spec = kFast ? is_int_variant(V) : (is_int_variant(V) && lc.spec && iq.slot * ix.K <= 64);
         const int nj = spec ? l / iq.slot : l;  // adjacency entry this lane reads
         const uint64_t lead_k = ballot(iq.part == 0 && nj < ix.K);  // first lane of each group with an entry
         uint32_t pred = kIdMask;
@@ -971,7 +975,7 @@ __global__ __launch_bounds__(256) void search_kernel(DeviceIndex ix, QueryBatch 
             const uint32_t u = pool_pop(w, ps, pred);
             cnt_e += 1;
             const int32_t v = (u == pred) ? pred_row : (nj < ix.K ? ix.adj[(int64_t)u * ix.K + nj] : -1);
-            if (at_use(spec)) {
+            if (kFast ? is_int_variant(V) : at_use(spec) != 0) {
                 const bool lead = iq.part == 0;
                 const uint64_t inv = lead_k & ballot(v == -1);
                 const int n = inv ? __builtin_ctzll(inv) / iq.slot : ix.K;
@@ -992,7 +996,7 @@ __global__ __launch_bounds__(256) void search_kernel(DeviceIndex ix, QueryBatch 
                 vs.account(m);
                 const uint32_t kj = key_of_int(iq.row_dist(c4, valid));
                 cnt_d += m;
-                if (at_use(lc.cap_max) <= 128) {
+                if (kFast || at_use(lc.cap_max) <= 128) {
                     pred = pool_merge_reg<2>(ix, w, ps, kj, (uint32_t)v, isnew, nm, nj, pred_row);
                 } else if (lc.cap_max <= 64 * kMaxChunks) {
                     pred = pool_merge_reg<kMaxChunks>(ix, w, ps, kj, (uint32_t)v, isnew, nm, nj, pred_row);
@@ -1026,7 +1030,7 @@ __global__ __launch_bounds__(256) void search_kernel(DeviceIndex ix, QueryBatch 
             wave_sync();
             dists(m);
             cnt_d += m;
-            if (at_use(lc.cap_max) <= 128) {
+            if (kFast || at_use(lc.cap_max) <= 128) {
                 pred = pool_merge_reg_lds<2>(ix, w, ps, m, pred_row);
             } else if (lc.cap_max <= 64 * kMaxChunks) {
                 pred = pool_merge_reg_lds<kMaxChunks>(ix, w, ps, m, pred_row);
@@ -1086,18 +1090,45 @@ __global__ __launch_bounds__(256) void search_kernel(DeviceIndex ix, QueryBatch 
     }
 }
 
+// Whether a launch runs the P = 1 kernel: pools of at most 128 entries and,
+// for integer codes, the all-neighbour gather with one lane group per
+// neighbour (slot * K <= 64, slot as IntQuery::init computes it).
+template <int V, int DPL>
+bool fast_path(const DeviceIndex &ix, const LaunchConfig &lc) {
+    if (lc.cap_max > 128) return false;
+    if (!is_int_variant(V)) return true;
+    if (!lc.spec) return false;
+    const int code_dw = ix.code_size / 4;
+    const int nparts = (code_dw + DPL - 1) / DPL;
+    int slot = 1;
+    while (slot < nparts) slot <<= 1;
+    return slot * ix.K <= 64;
+}
+
+template <int V, int DPL>
+const void *kernel_fn(const DeviceIndex &ix, const LaunchConfig &lc) {
+    return fast_path<V, DPL>(ix, lc) ? (const void *)search_kernel<V, DPL, 1> : (const void *)search_kernel<V, DPL, 0>;
+}
+
 template <int V, int DPL>
 hipError_t launch_one(const DeviceIndex &ix, const QueryBatch &qb, const LaunchConfig &lc, const SlotState &ss,
                       hipStream_t stream) {
     const size_t lds = wave_lds_bytes(ix, lc.cap_max, lc.vis_hash) * (size_t)lc.waves_per_block;
     static bool attr_set = false;
     if (!attr_set) {
-        (void)hipFuncSetAttribute((const void *)search_kernel<V, DPL>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                            160 * 1024);
+        (void)hipFuncSetAttribute((const void *)search_kernel<V, DPL, 0>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  160 * 1024);
+        (void)hipFuncSetAttribute((const void *)search_kernel<V, DPL, 1>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  160 * 1024);
         attr_set = true;
     }
-    hipLaunchKernelGGL((search_kernel<V, DPL>), dim3(lc.blocks), dim3(64 * lc.waves_per_block), lds, stream, ix, qb,
-                       lc, ss);
+    if (fast_path<V, DPL>(ix, lc)) {
+        hipLaunchKernelGGL((search_kernel<V, DPL, 1>), dim3(lc.blocks), dim3(64 * lc.waves_per_block), lds, stream,
+                           ix, qb, lc, ss);
+    } else {
+        hipLaunchKernelGGL((search_kernel<V, DPL, 0>), dim3(lc.blocks), dim3(64 * lc.waves_per_block), lds, stream,
+                           ix, qb, lc, ss);
+    }
     return hipGetLastError();
 }
 
@@ -1108,7 +1139,7 @@ template <int V, int DPL>
 int occupancy_one(const DeviceIndex &ix, const LaunchConfig &lc) {
     const size_t lds = wave_lds_bytes(ix, lc.cap_max, lc.vis_hash) * (size_t)lc.waves_per_block;
     hipFuncAttributes attr;
-    if (hipFuncGetAttributes(&attr, (const void *)search_kernel<V, DPL>) != hipSuccess) return 0;
+    if (hipFuncGetAttributes(&attr, kernel_fn<V, DPL>(ix, lc)) != hipSuccess) return 0;
     int vgpr = attr.numRegs > 0 ? attr.numRegs : 128;
     vgpr = (vgpr + 7) / 8 * 8;
     int waves_simd = 512 / vgpr;
@@ -1147,18 +1178,18 @@ size_t lds_bytes_per_wave(const DeviceIndex &ix, int32_t cap_max, int32_t vis_ha
 }
 
 template <int V, int DPL>
-int vgprs_one() {
+int vgprs_one(const DeviceIndex &ix, const LaunchConfig &lc) {
     hipFuncAttributes attr;
-    if (hipFuncGetAttributes(&attr, (const void *)search_kernel<V, DPL>) != hipSuccess) return 0;
+    if (hipFuncGetAttributes(&attr, kernel_fn<V, DPL>(ix, lc)) != hipSuccess) return 0;
     return attr.numRegs;
 }
 
 template <int V>
-int vgprs_int(int team) {
-    switch (team) {
-        case 4: return vgprs_one<V, 4>();
-        case 8: return vgprs_one<V, 8>();
-        case 16: return vgprs_one<V, 16>();
+int vgprs_int(const DeviceIndex &ix, const LaunchConfig &lc) {
+    switch (lc.team) {
+        case 4: return vgprs_one<V, 4>(ix, lc);
+        case 8: return vgprs_one<V, 8>(ix, lc);
+        case 16: return vgprs_one<V, 16>(ix, lc);
         default: return 0;
     }
 }
@@ -1166,12 +1197,12 @@ int vgprs_int(int team) {
 int kernel_vgprs(const DeviceIndex &ix, const LaunchConfig &lc) {
     const bool l2 = ix.metric == MK_L2;
     switch (ix.quant) {
-        case QK_SQ8U: return l2 ? vgprs_int<V_SQ8U_L2>(lc.team) : vgprs_int<V_SQ8U_IP>(lc.team);
-        case QK_SQ4U: return vgprs_int<V_SQ4U>(lc.team);
-        case QK_SQ2U: return vgprs_int<V_SQ2U>(lc.team);
-        case QK_E5M2: return l2 ? vgprs_one<V_E5M2_L2, 4>() : vgprs_one<V_E5M2_IP, 4>();
-        case QK_SQ8: return l2 ? vgprs_one<V_SQ8_L2, 4>() : vgprs_one<V_SQ8_IP, 4>();
-        case QK_FP32: return l2 ? vgprs_one<V_FP32_L2, 4>() : vgprs_one<V_FP32_IP, 4>();
+        case QK_SQ8U: return l2 ? vgprs_int<V_SQ8U_L2>(ix, lc) : vgprs_int<V_SQ8U_IP>(ix, lc);
+        case QK_SQ4U: return vgprs_int<V_SQ4U>(ix, lc);
+        case QK_SQ2U: return vgprs_int<V_SQ2U>(ix, lc);
+        case QK_E5M2: return l2 ? vgprs_one<V_E5M2_L2, 4>(ix, lc) : vgprs_one<V_E5M2_IP, 4>(ix, lc);
+        case QK_SQ8: return l2 ? vgprs_one<V_SQ8_L2, 4>(ix, lc) : vgprs_one<V_SQ8_IP, 4>(ix, lc);
+        case QK_FP32: return l2 ? vgprs_one<V_FP32_L2, 4>(ix, lc) : vgprs_one<V_FP32_IP, 4>(ix, lc);
         default: return 0;
     }
 }
