@@ -564,11 +564,10 @@ bool spec_fits(const glass_hip_searcher &s, int32_t team) {
     return slot * s.graph.K <= 64;
 }
 
-int32_t round_cap(int32_t cap) {
-    int32_t c = 64;
-    while (c < cap) c <<= 1;
-    return c;
-}
+// Pool capacity the launch sizes LDS for: the batch's largest pool rounded up
+// to 16 entries (not to a power of two: LDS per wave decides how many waves a
+// CU holds).
+int32_t round_cap(int32_t cap) { return (std::max(cap, 1) + 15) / 16 * 16; }
 
 // Per-launch state for batches that run concurrently on several streams:
 // their own work counter and visited bitsets (max_slots x vis_stride words),
