@@ -458,6 +458,91 @@ __device__ __forceinline__ float float_row4(const DeviceIndex &ix, const WaveLds
     }
 }
 
+// Exact FP32 distances (Refiner, refiner.hpp:92-110) of pool entries [0, m),
+// written over their keys in pk.  Rows of S * 16 floats (S <= 8, a
+// compile-time count, so the loop has no branches): each 4-lane group takes
+// R rows per pass (entries base + 16r + g) and issues every load of the pass
+// before any arithmetic, so a pass waits on memory once; each accumulator
+// still sums its elements in increasing order, as float_row4 does.
+template <int VF, int R, int S>
+__device__ void rerank_rows(const DeviceIndex &ix, const WaveLds &w, int m) {
+    const int l = lane_id();
+    const int p4 = l & 3, g = l >> 2;
+    for (int base = 0; base < m; base += 16 * R) {
+        float4 y[R][S];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int j = base + 16 * r + g;
+            const uint32_t id = w.pid[j < m ? j : m - 1] & kIdMask;
+            const float *row = ix.rerank + (int64_t)id * (16 * S) + 4 * p4;
+#pragma unroll
+            for (int s = 0; s < S; ++s) y[r][s] = *(const float4 *)(row + 16 * s);
+        }
+        Acc4 acc[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) acc[r] = Acc4{{0.0f, 0.0f, 0.0f, 0.0f}};
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+            const float4 x = *(const float4 *)(w.qf + 16 * s + 4 * p4);
+            const float xs[4] = {x.x, x.y, x.z, x.w};
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const float ys[4] = {y[r][s].x, y[r][s].y, y[r][s].z, y[r][s].w};
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    if (VF == V_FP32_L2) {
+                        const float d = xs[e] - ys[e];
+                        acc[r].a[e] = fmaf(d, d, acc[r].a[e]);
+                    } else {
+                        acc[r].a[e] = fmaf(xs[e], ys[e], acc[r].a[e]);
+                    }
+                }
+            }
+        }
+        float rs[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) rs[r] = reduce16_4lanes<false>(acc[r]);
+        wave_sync();
+        if (p4 == 0) {
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const int j = base + 16 * r + g;
+                if (j < m) w.pk[j] = key_of_float(VF == V_FP32_L2 ? rs[r] : -rs[r]);
+            }
+        }
+        wave_sync();
+    }
+}
+
+// Exact FP32 distances of pool entries [0, m) into pk; rows longer than 128
+// floats go through float_row4, one row per 4-lane group per pass.
+template <int VF>
+__device__ void rerank_fp32(const DeviceIndex &ix, const WaveLds &w, int m) {
+    const int da = ix.r_dim_align;  // a multiple of 16
+    switch (da) {
+        case 16: return rerank_rows<VF, 1, 1>(ix, w, m);
+        case 32: return rerank_rows<VF, 1, 2>(ix, w, m);
+        case 48: return rerank_rows<VF, 1, 3>(ix, w, m);
+        case 64: return rerank_rows<VF, 1, 4>(ix, w, m);
+        case 80: return rerank_rows<VF, 1, 5>(ix, w, m);
+        case 96: return rerank_rows<VF, 1, 6>(ix, w, m);
+        case 112: return rerank_rows<VF, 1, 7>(ix, w, m);
+        case 128: return rerank_rows<VF, 1, 8>(ix, w, m);
+        default: break;
+    }
+    const int l = lane_id();
+    const int p4 = l & 3, g = l >> 2;
+    for (int base = 0; base < m; base += 16) {
+        const int j = base + g;
+        const uint32_t id = w.pid[j < m ? j : m - 1] & kIdMask;
+        const uint8_t *row = (const uint8_t *)(ix.rerank + (int64_t)id * da);
+        const float rsum = float_row4<VF>(ix, w, row, p4, MODE_BATCH, da);
+        wave_sync();
+        if (j < m && p4 == 0) w.pk[j] = key_of_float(VF == V_FP32_L2 ? rsum : -rsum);
+        wave_sync();
+    }
+}
+
 template <int V>
 __device__ void float_distances(const DeviceIndex &ix, const WaveLds &w, int m, int32_t mode) {
     if (m <= 0) return;
@@ -1057,17 +1142,10 @@ __global__ __launch_bounds__(256) void search_kernel(DeviceIndex ix, QueryBatch 
             // Refiner::RefineImpl (refiner.hpp:92-110): exact FP32 distances of the
             // first reorder_k pool entries, the k smallest by (dist, id) ascending.
             const int m = kk < ps.size ? kk : ps.size;
-            const int p4 = l & 3, rj = l >> 2;
-            for (int base = 0; base < m; base += 16) {
-                const int j = base + rj;
-                const int jj = j < m ? j : m - 1;
-                const uint32_t id = w.pid[jj] & kIdMask;
-                const uint8_t *row = (const uint8_t *)(ix.rerank + (int64_t)id * ix.r_dim_align);
-                const float rsum = ix.metric == MK_L2 ? float_row4<V_FP32_L2>(ix, w, row, p4, qb.mode, ix.r_dim_align)
-                                                      : float_row4<V_FP32_IP>(ix, w, row, p4, qb.mode, ix.r_dim_align);
-                wave_sync();
-                if (j < m && p4 == 0) w.pk[j] = key_of_float(ix.metric == MK_L2 ? rsum : -rsum);
-                wave_sync();
+            if (ix.metric == MK_L2) {
+                rerank_fp32<V_FP32_L2>(ix, w, m);
+            } else {
+                rerank_fp32<V_FP32_IP>(ix, w, m);
             }
             wave_sync();
             for (int j = l; j < m; j += 64) {
