@@ -908,8 +908,13 @@ __device__ __forceinline__ void put_result(const QueryBatch &qb, int64_t pos, in
 // all-neighbour gather; the other paths are left out, which keeps the hot
 // loop small and its scalar registers unspilled.  P = 0 handles everything.
 
+// At least 4 waves per SIMD (at most 128 VGPRs): left to itself the compiler
+// schedules the re-rank's loads into more registers than that.
 template <int V, int DPL, int P>
-__global__ __launch_bounds__(256) void search_kernel(DeviceIndex ix, QueryBatch qb, LaunchConfig lc, SlotState ss) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void search_kernel(DeviceIndex ix,
+                                                                                            QueryBatch qb,
+                                                                                            LaunchConfig lc,
+                                                                                            SlotState ss) {
     constexpr bool kFast = P == 1;
     extern __shared__ __attribute__((aligned(16))) uint8_t lds_raw[];
     const int l = lane_id();
