@@ -164,7 +164,7 @@ def gpu_sweep(s, q_dev, nq, k, gt, ids_dev, d_dev, cnt_dev, args, torch, stream,
         pts.append({"factor": factor, "ef": ef, "recall": r, "ms_per_step": ms, "qps": nq / (ms / 1e3),
                     "kernel_ms": float(np.mean(kern_ms)), "dist_cmps": float(cnt[:, 0].mean()),
                     "expansions": float(cnt[:, 1].mean()), "upper": float(cnt[:, 2].mean()),
-                    "ids": ids, "cnt": cnt})
+                    "launch": s.get_launch(), "ids": ids, "cnt": cnt})
         log(f"gpu: factor={factor} ef={ef} recall={r:.4f} qps={nq / (ms / 1e3):.0f} "
             f"kernel={np.mean(kern_ms):.3f}ms dist_cmps={cnt[:, 0].mean():.0f} exp={cnt[:, 1].mean():.1f} "
             f"launch={s.get_launch()} kernel={s.kernel_info()}")
@@ -497,7 +497,7 @@ def main():
     traffic, traffic_detail, occ = None, None, None
     if not args.no_pmc:
         s.set_ef(op["ef"])
-        lz = s.get_launch()
+        lz = op["launch"]
         pt, ln = f"{best_f}:{op['ef']}", f"{lz['team']}:{lz['waves_per_block']}:{lz['vis_entries']}"
         traffic, traffic_detail = pmc_traffic(pt, ln)
         log(f"pmc: traffic {traffic} bytes per launch ({traffic_detail})")
@@ -554,7 +554,7 @@ def main():
                          for f, pts in sweeps.items()},
             "reference_default_factor_1_5": [{kk: v for kk, v in p.items() if kk in ("ef", "recall", "qps")}
                                              for p in default_curve],
-            "launch": launch,
+            "launch": op["launch"],
             "graph_build_s": gmeta.get("seconds"),
             "plain_one_query_per_launch": {"qps": plain_qps, "recall": plain_recall, "queries": npl,
                                            "ef": op["ef"]},

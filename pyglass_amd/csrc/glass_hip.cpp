@@ -1152,8 +1152,8 @@ int glass_hip_searcher_get_launch(const glass_hip_searcher *s, int32_t *team, in
     if (team) *team = s->force_team ? s->force_team : tn.team;
     if (waves_per_block) *waves_per_block = s->force_wpb ? s->force_wpb : tn.waves_per_block;
     if (blocks) *blocks = s->last_launch.blocks;
-    if (slots) *slots = s->last_launch.vis_hash ? s->last_launch.vis_hash
-                                                : (tn.vis_hash ? tn.vis_hash : glass_hip_searcher::auto_vis(s->ef));
+    // the table a batch at the current ef would use (as run_batch picks it)
+    if (slots) *slots = s->vis_override ? s->vis_override : (tn.vis_hash ? tn.vis_hash : glass_hip_searcher::auto_vis(s->ef));
     return GLASS_HIP_OK;
 }
 
