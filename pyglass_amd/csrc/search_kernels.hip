@@ -652,13 +652,25 @@ struct PoolState {
 };
 
 // LinearPool::pop (neighbor.hpp:398-405): mark data[cur] checked, advance
-// cur to the next unchecked entry.
+// cur to the next unchecked entry.  Here the pop only marks the entry; the
+// scan for the next unchecked entry is left to whoever runs next: the
+// register merge finds it from the pool chunks it loads anyway; other paths
+// call pool_settle.  Until then ps.cur is a lower bound (every entry before
+// it is checked).
 __device__ __forceinline__ uint32_t pool_pop(const WaveLds &w, PoolState &ps, uint32_t known = kIdMask) {
     const int l = lane_id();
     const uint32_t uid = known != kIdMask ? known : uni(w.pid[ps.cur]);
     wave_sync();
     if (l == 0) w.pid[ps.cur] = uid | kChecked;
-    int c = ps.cur + 1;
+    wave_sync();
+    ps.cur += 1;
+    return uid & kIdMask;
+}
+
+// ps.cur <- the first unchecked entry at or after it (size if none).
+__device__ __forceinline__ void pool_settle(const WaveLds &w, PoolState &ps) {
+    const int l = lane_id();
+    int c = ps.cur;
     while (c < ps.size) {
         int p = c + l;
         // unconditional read: p < size + 63 <= cap_max + 63, which stays
@@ -673,8 +685,6 @@ __device__ __forceinline__ uint32_t pool_pop(const WaveLds &w, PoolState &ps, ui
         c += 64;
     }
     ps.cur = uni(c < ps.size ? c : ps.size);
-    wave_sync();
-    return uid & kIdMask;
 }
 
 __device__ __forceinline__ int upper_bound_lds(const uint32_t *pk, int n, uint32_t key) {
@@ -781,19 +791,37 @@ template <int NCH>
 __device__ uint32_t pool_merge_reg(const DeviceIndex &ix, const WaveLds &w, PoolState &ps, uint32_t kj,
                                    uint32_t ij, bool cand, uint64_t cm, int nj, int32_t &pred_row) {
     const int l = lane_id();
-    const int nch = (ps.size + 63) >> 6;
     uint32_t pkc[NCH], pic[NCH];
 #pragma unroll
     for (int c = 0; c < NCH; ++c) {
         const int p = 64 * c + l;
-        const bool v = p < ps.size;  // also false for every chunk past nch
+        const bool v = p < ps.size;
         // unconditional reads, then selects: p < 64 * NCH, and the arrays
         // that follow pk and pid (pid, ck, ci: at least 512 + round16(4 *
         // cap_max) bytes) cover 64 * NCH entries for both NCH = 2 and
-        // NCH = 4 (used only when cap_max > 128)
+        // NCH = 4 (used only when cap_max > 128).  Entries past the pool
+        // read as checked, with the largest key.
         const uint32_t xk = w.pk[p], xi = w.pid[p];
         pkc[c] = v ? xk : kKeyMax;
-        pic[c] = v ? xi : 0u;
+        pic[c] = v ? xi : kChecked;
+    }
+    // the first unchecked entry (pool_pop leaves ps.cur a lower bound), its
+    // key and its id, from the registers
+    int cur0 = ps.size;
+#pragma unroll
+    for (int c = NCH - 1; c >= 0; --c) {
+        const uint64_t um = ballot((int32_t)pic[c] >= 0);
+        cur0 = um ? 64 * c + __builtin_ctzll(um) : cur0;
+    }
+    uint32_t kc = kKeyMax, idc = kIdMask;
+    if (cur0 < ps.size) {
+#pragma unroll
+        for (int c = 0; c < NCH; ++c) {
+            if ((cur0 >> 6) == c) {
+                kc = (uint32_t)__builtin_amdgcn_readlane((int)pkc[c], cur0 & 63);
+                idc = (uint32_t)__builtin_amdgcn_readlane((int)pic[c], cur0 & 63) & kIdMask;
+            }
+        }
     }
     // insert() rejects d >= data[size-1] when full; such a candidate would
     // also land past cap in the merge and displaces nothing.  An empty pool
@@ -801,7 +829,7 @@ __device__ uint32_t pool_merge_reg(const DeviceIndex &ix, const WaveLds &w, Pool
     const uint32_t last = w.pk[ps.size > 0 ? ps.size - 1 : 0];
     const bool act = cand && (ps.size < ps.cap || kj < last);
     const uint64_t am = cm & (ps.size < ps.cap ? ~0ull : ballot(kj < last));  // the lanes' act bits
-    int new_size = ps.size, new_cur = ps.cur, np0 = 0x7fffffff;
+    int new_size = ps.size, new_cur = cur0, np0 = 0x7fffffff;
     uint32_t id0 = 0;
     int r = 0, ub = 0;
     // le[c]: active candidates whose key is >= pool entry 64c + l; the
@@ -838,11 +866,7 @@ __device__ uint32_t pool_merge_reg(const DeviceIndex &ix, const WaveLds &w, Pool
         np0 = ub0 < ps.cap ? ub0 : 0x7fffffff;
         const int ma = __builtin_popcountll(am);
         new_size = ps.size + ma < ps.cap ? ps.size + ma : ps.cap;
-        new_cur = 0x7fffffff;
-        if (ps.cur < ps.size) {
-            const uint32_t kc = w.pk[ps.cur];
-            new_cur = ps.cur + __builtin_popcountll(am & ballot(kj < kc));
-        }
+        new_cur = cur0 < ps.size ? cur0 + __builtin_popcountll(am & ballot(kj < kc)) : 0x7fffffff;
         new_cur = new_cur < np0 ? new_cur : np0;
         new_cur = new_cur < new_size ? new_cur : new_size;
     }
@@ -851,7 +875,7 @@ __device__ uint32_t pool_merge_reg(const DeviceIndex &ix, const WaveLds &w, Pool
     uint32_t next = kIdMask;
     new_size = uni(new_size);
     new_cur = uni(new_cur);
-    if (new_cur < new_size && new_cur < ps.ef) next = uni((new_cur == np0) ? id0 : (w.pid[ps.cur] & kIdMask));
+    if (new_cur < new_size && new_cur < ps.ef) next = uni((new_cur == np0) ? id0 : idc);
     if (next != kIdMask) pred_row = nj < ix.K ? ix.adj[(int64_t)next * ix.K + nj] : -1;
     if (am) {
         const int lo = np0 < ps.cap ? np0 : ps.cap;  // entries before the first insertion do not move
@@ -876,8 +900,8 @@ __device__ uint32_t pool_merge_reg(const DeviceIndex &ix, const WaveLds &w, Pool
         }
         wave_sync();
         ps.size = new_size;
-        ps.cur = new_cur;
     }
+    ps.cur = new_cur;
     return next;
 }
 
@@ -986,20 +1010,29 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void s
         if (ix.has_init) {
             // HNSWInitializer::initialize (HNSWInitializer.hpp:41-62)
             uint32_t u = (uint32_t)ix.ep;
+            const int top = uni(ix.levels[ix.ep]);
+            // offset of u's upper lists.  Each candidate's offset is read
+            // together with its id, so a hop's list load waits on nothing but
+            // the previous hop's distances.
+            int64_t uo = 0;
+            if (top > 0) {
+                const int64_t e = ix.up_off[u];
+                uo = ((int64_t)uni((int32_t)(e >> 32)) << 32) | (uint32_t)uni((int32_t)e);
+            }
             if (l == 0) w.ci[0] = u;
             wave_sync();
             dists(1);
             uint32_t du = uni(w.ck[0]);
             cnt_d += 1;
-            for (int level = uni(ix.levels[ix.ep]); level > 0; --level) {
+            for (int level = top; level > 0; --level) {
                 for (;;) {
-                    const int64_t uo = ((int64_t)uni((int32_t)(ix.up_off[u] >> 32)) << 32) |
-                                       (uint32_t)uni((int32_t)ix.up_off[u]);
                     const int32_t *list = ix.up_lists + uo + (int64_t)(level - 1) * ix.up_K;
                     const int32_t v = l < ix.up_K ? list[l] : -1;
                     const uint64_t inv = ballot(l < ix.up_K && v == -1);
                     const int n = inv ? __builtin_ctzll(inv) : ix.up_K;
                     if (n == 0) break;
+                    // a candidate of a level-l list has level >= l, so it has upper lists
+                    const int64_t vo = l < n ? ix.up_off[v] : 0;
                     if (l < n) w.ci[l] = (uint32_t)v;
                     wave_sync();
                     dists(n);
@@ -1013,6 +1046,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void s
                     if (bk < du) {  // strict <: the first minimum of the pass wins
                         du = bk;
                         u = (uint32_t)__builtin_amdgcn_readlane(v, bi);
+                        uo = ((int64_t)__builtin_amdgcn_readlane((int32_t)(vo >> 32), bi) << 32) |
+                             (uint32_t)__builtin_amdgcn_readlane((int32_t)vo, bi);
                     } else {
                         break;
                     }
@@ -1076,6 +1111,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void s
                 const uint64_t nm = ballot(isnew);
                 pred = kIdMask;
                 if (nm == 0) {
+                    pool_settle(w, ps);
                     if (ps.cur < ps.size && ps.cur < ps.ef) {
                         pred = uni(w.pid[ps.cur]) & kIdMask;
                         pred_row = nj < ix.K ? ix.adj[(int64_t)pred * ix.K + nj] : -1;
@@ -1096,6 +1132,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void s
                         w.ci[mbcnt(nm)] = (uint32_t)v;
                     }
                     wave_sync();
+                    pool_settle(w, ps);
                     pool_merge_lds(w, ps, m);
                 }
                 continue;
@@ -1108,6 +1145,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void s
                 // nothing new: the pool is unchanged and the next pop takes the
                 // first unchecked entry
                 pred = kIdMask;
+                pool_settle(w, ps);
                 if (ps.cur < ps.size && ps.cur < ps.ef) {
                     pred = uni(w.pid[ps.cur]) & kIdMask;
                     pred_row = nj < ix.K ? ix.adj[(int64_t)pred * ix.K + nj] : -1;
@@ -1125,6 +1163,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void s
             } else if (lc.cap_max <= 64 * kMaxChunks) {
                 pred = pool_merge_reg_lds<kMaxChunks>(ix, w, ps, m, pred_row);
             } else {
+                pool_settle(w, ps);
                 pool_merge_lds(w, ps, m);
                 pred = kIdMask;
             }
