@@ -949,10 +949,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void s
     uint32_t *vis = ss.visited + (int64_t)slot * ss.vis_stride;
     const int ql = qf_len(ix);
 
+    // The next query's index, and its row when ql <= 256 (xn, four floats
+    // per lane), are requested before the current query's output phase, so
+    // that their latency overlaps it.
+    int qi_next = 0;
+    if (l == 0) qi_next = atomicAdd(ss.work_counter, 1);
+    qi_next = uni(qi_next);
+    bool have_row = false;
+    float xn[4] = {0.0f, 0.0f, 0.0f, 0.0f};
     for (;;) {
-        int qi = 0;
-        if (l == 0) qi = atomicAdd(ss.work_counter, 1);
-        qi = uni(qi);
+        const int qi = qi_next;
         if (qi >= qb.nq) break;
 
         const int32_t ef = uni(qb.ef ? qb.ef[qi] : qb.ef_uniform);
@@ -976,7 +982,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void s
 
         // stage the fp32 query (zero padded, as MemCpyTag copies into a zeroed buffer)
         const float *x = qb.queries + (int64_t)qi * qb.dim;
-        for (int j = l; j < ql; j += 64) w.qf[j] = j < ix.dim ? x[j] : 0.0f;
+        if (have_row) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+                if (l + 64 * r < ql) w.qf[l + 64 * r] = xn[r];
+        } else {
+            for (int j = l; j < ql; j += 64) w.qf[j] = j < ix.dim ? x[j] : 0.0f;
+        }
         wave_sync();
         if (V == V_SQ8_L2 || V == V_SQ8_IP) {
             for (int j = l; j < ql; j += 64) {
@@ -1169,6 +1181,22 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void s
             }
         }
 
+        // take the next query now (see above)
+        {
+            int t = 0;
+            if (l == 0) t = atomicAdd(ss.work_counter, 1);
+            qi_next = uni(t);
+            have_row = qi_next < qb.nq && ql <= 256;
+            if (have_row) {
+                const float *xq = qb.queries + (int64_t)qi_next * qb.dim;
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int j = l + 64 * r;
+                    xn[r] = j < ix.dim ? xq[j] : 0.0f;
+                }
+            }
+        }
+
         // outputs
         const int64_t ob = qb.out_off ? qb.out_off[qi] : (int64_t)qi * qb.k_uniform;
         if (!refine) {
@@ -1192,14 +1220,39 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void s
                 rerank_fp32<V_FP32_IP>(ix, w, m);
             }
             wave_sync();
-            for (int j = l; j < m; j += 64) {
-                const uint32_t kj = w.pk[j], ij = w.pid[j] & kIdMask;
-                int rank = 0;
-                for (int i = 0; i < m; ++i) {
-                    const uint32_t ki = w.pk[i], ii = w.pid[i] & kIdMask;
-                    rank += (ki < kj) | ((ki == kj) & (ii < ij));
+            if (m <= 128) {
+                // (key, id) pairs in registers, two per lane; ids are distinct,
+                // so the 64-bit order is total and is the (dist, id) order
+                const int j1 = l + 64;
+                const uint32_t ka = w.pk[l < m ? l : 0], ia = w.pid[l < m ? l : 0] & kIdMask;
+                const uint32_t kb = w.pk[j1 < m ? j1 : 0], ib = w.pid[j1 < m ? j1 : 0] & kIdMask;
+                const uint64_t ca = ((uint64_t)ka << 32) | ia, cb = ((uint64_t)kb << 32) | ib;
+                int ra = 0, rb = 0;
+                const int m0 = m < 64 ? m : 64;
+                for (int i = 0; i < m0; ++i) {
+                    const uint64_t ci = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)ka, i) << 32) |
+                                        (uint32_t)__builtin_amdgcn_readlane((int)ia, i);
+                    ra += ci < ca;
+                    rb += ci < cb;
                 }
-                if (rank < k) put_result(qb, ob + rank, (int32_t)ij, float_of_key(kj));
+                for (int i = 64; i < m; ++i) {
+                    const uint64_t ci = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)kb, i - 64) << 32) |
+                                        (uint32_t)__builtin_amdgcn_readlane((int)ib, i - 64);
+                    ra += ci < ca;
+                    rb += ci < cb;
+                }
+                if (l < m && ra < k) put_result(qb, ob + ra, (int32_t)ia, float_of_key(ka));
+                if (j1 < m && rb < k) put_result(qb, ob + rb, (int32_t)ib, float_of_key(kb));
+            } else {
+                for (int j = l; j < m; j += 64) {
+                    const uint32_t kj = w.pk[j], ij = w.pid[j] & kIdMask;
+                    int rank = 0;
+                    for (int i = 0; i < m; ++i) {
+                        const uint32_t ki = w.pk[i], ii = w.pid[i] & kIdMask;
+                        rank += (ki < kj) | ((ki == kj) & (ii < ij));
+                    }
+                    if (rank < k) put_result(qb, ob + rank, (int32_t)ij, float_of_key(kj));
+                }
             }
             for (int i = m + l; i < k; i += 64) put_result(qb, ob + i, -1, __builtin_inff());
         }
