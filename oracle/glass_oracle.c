@@ -59,6 +59,20 @@ uint16_t orq_f32_to_f16(float f) {
     return (uint16_t)(sign | (r >> 13));
 }
 
+/* (bits + 0x8000) >> 16: round to nearest, ties away from zero in magnitude */
+uint16_t orq_f32_to_bf16(float f) {
+    uint32_t x;
+    memcpy(&x, &f, 4);
+    return (uint16_t)((x + 0x8000u) >> 16);
+}
+
+static float bf16_to_f32(uint16_t h) {
+    uint32_t x = (uint32_t)h << 16;
+    float f;
+    memcpy(&f, &x, 4);
+    return f;
+}
+
 float orq_f16_to_f32(uint16_t h) {
     uint32_t sign = (uint32_t)(h & 0x8000u) << 16;
     uint32_t e = (h >> 10) & 0x1fu, m = h & 0x3ffu, bits;
@@ -129,7 +143,9 @@ int orq_quant_init(orq_quant *q, int32_t kind, int32_t metric, int32_t dim) {
     q->dim = dim;
     /* Quantizer<metric, AlignWidth, NBits> per quantizer
      * (sq8u_quant.hpp:14, sq4u_quant.hpp:13, sq2u_quant.hpp:11,
-     *  e5m2_quant.hpp:11, sq8_quant.hpp:15-16, fp32_quant.hpp:8) */
+     *  e5m2_quant.hpp:11, sq8_quant.hpp:15-16, fp32_quant.hpp:8,
+     *  fp16_quant.hpp:9-15 and bf16_quant.hpp:9-15 without the AVX512FP16 /
+     *  AVX512BF16 opt-in macros) */
     switch (kind) {
         case ORQ_SQ8U: q->align = 64; q->nbits = 8; break;
         case ORQ_SQ4U: q->align = 128; q->nbits = 4; break;
@@ -137,6 +153,8 @@ int orq_quant_init(orq_quant *q, int32_t kind, int32_t metric, int32_t dim) {
         case ORQ_E5M2: q->align = 16; q->nbits = 8; break;
         case ORQ_SQ8: q->align = 16; q->nbits = 8; break;
         case ORQ_FP32: q->align = 16; q->nbits = 32; break;
+        case ORQ_FP16: q->align = 16; q->nbits = 16; break;
+        case ORQ_BF16: q->align = 16; q->nbits = 16; break;
         default: return -1;
     }
     /* Tensor ctor (glass/storage/tensor.hpp:30-35) */
@@ -248,12 +266,37 @@ void orq_encode(const orq_quant *q, const float *x, uint8_t *out) {
         case ORQ_FP32:
             memcpy(out, x, sizeof(float) * (size_t)d);
             break;
+        case ORQ_FP16: { /* FP16Quantizer::encode -> helpa::fp32_to_fp16 (RNE), padding as below */
+            uint16_t *o = (uint16_t *)out;
+            for (int32_t j = 0; j < d; ++j) o[j] = orq_f32_to_f16(x[j]);
+            break;
+        }
+        case ORQ_BF16: { /* BF16Quantizer::encode (bf16_quant.hpp:34-38), padding stays zero */
+            uint16_t *o = (uint16_t *)out;
+            for (int32_t j = 0; j < d; ++j) o[j] = orq_f32_to_bf16(x[j]);
+            break;
+        }
     }
 }
 
 void orq_encode_rows(const orq_quant *q, const float *data, int64_t n, uint8_t *codes) {
 #pragma omp parallel for schedule(static)
-    for (int64_t i = 0; i < n; ++i) orq_encode(q, data + i * q->dim, codes + i * q->code_size);
+    for (int64_t i = 0; i < n; ++i) {
+        orq_encode(q, data + i * q->dim, codes + i * q->code_size);
+        if (q->kind == ORQ_FP16) {
+            /* helpa::fp32_to_fp16 (x86/data_convert.hpp:18-31) converts whole
+             * 16-float blocks of the row, the last one included, so the
+             * padding [dim, dim_align) holds fp16 of the floats that follow
+             * the row in the input (the next row's first values).  Past the
+             * end of the input the reference reads unowned memory; zeros here. */
+            uint16_t *o = (uint16_t *)(codes + i * q->code_size);
+            const int64_t total = n * (int64_t)q->dim;
+            for (int32_t j = q->dim; j < q->dim_align; ++j) {
+                const int64_t at = i * (int64_t)q->dim + j;
+                o[j] = at < total ? orq_f32_to_f16(data[at]) : 0;
+            }
+        }
+    }
 }
 
 /* ComputerImpl ctor (glass/quant/computer.hpp:54-63): integer quantizers
@@ -375,16 +418,26 @@ double orq_distance_ctx(const orq_quant *q, const uint8_t *qbuf, const uint8_t *
             float r = reduce16_glass(acc);
             return q->metric == ORQ_L2 ? (double)r : (double)(-r);
         }
-        case ORQ_FP32: { /* helpa::l2_fp32_fp32 / dot_fp32_fp32 (x86/l2_impl.hpp:14-18,50-71; x86/dot_impl.hpp:16-20,52-71) */
-            const float *x = (const float *)qbuf, *y = (const float *)code;
+        case ORQ_FP32:   /* helpa::l2_fp32_fp32 / dot_fp32_fp32 (x86/l2_impl.hpp:14-18,50-71; x86/dot_impl.hpp:16-20,52-71) */
+        case ORQ_FP16:   /* helpa::l2_fp32_fp16 / dot_fp32_fp16 (x86/l2_impl.hpp:20-24,74-84; x86/dot_impl.hpp:22-26,74-84) */
+        case ORQ_BF16: { /* helpa::l2_fp32_bf16 / dot_fp32_bf16 (x86/l2_impl.hpp:32-36,145-156; x86/dot_impl.hpp:34-38,139-149) */
+            const float *x = (const float *)qbuf;
             float acc[16] = {0};
             for (int32_t i = 0; i < da; i += 16) {
                 for (int l = 0; l < 16; ++l) {
+                    float y;
+                    if (q->kind == ORQ_FP32) {
+                        y = ((const float *)code)[i + l];
+                    } else if (q->kind == ORQ_FP16) {
+                        y = orq_f16_to_f32(((const uint16_t *)code)[i + l]);
+                    } else {
+                        y = bf16_to_f32(((const uint16_t *)code)[i + l]);
+                    }
                     if (q->metric == ORQ_L2) {
-                        float t = x[i + l] - y[i + l];
+                        float t = x[i + l] - y;
                         acc[l] = fmaf(t, t, acc[l]);
                     } else {
-                        acc[l] = fmaf(x[i + l], y[i + l], acc[l]);
+                        acc[l] = fmaf(x[i + l], y, acc[l]);
                     }
                 }
             }

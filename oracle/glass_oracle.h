@@ -16,7 +16,7 @@ extern "C" {
 #endif
 
 /* quantizer kinds (names as in glass/quant/quant.hpp:23-43) */
-enum { ORQ_FP32 = 0, ORQ_E5M2 = 3, ORQ_SQ8U = 4, ORQ_SQ8 = 5, ORQ_SQ4U = 7, ORQ_SQ2U = 11 };
+enum { ORQ_FP32 = 0, ORQ_FP16 = 1, ORQ_BF16 = 2, ORQ_E5M2 = 3, ORQ_SQ8U = 4, ORQ_SQ8 = 5, ORQ_SQ4U = 7, ORQ_SQ2U = 11 };
 enum { ORQ_L2 = 0, ORQ_IP = 1 };
 
 typedef struct {
@@ -50,6 +50,8 @@ int orq_quant_train(orq_quant *q, const float *data, int64_t n);
 /* Encode one row (base or query) into code_size bytes (zero padded). */
 void orq_encode(const orq_quant *q, const float *x, uint8_t *out);
 void orq_encode_rows(const orq_quant *q, const float *data, int64_t n, uint8_t *codes);
+/* helpa::bf16(float) with ROUND_MODE_TO_NEAREST (helpa/types.hpp:22-80) */
+uint16_t orq_f32_to_bf16(float f);
 /* Query-side representation: encoded code (integer quantizers) or the fp32
  * query zero-padded to dim_align (E5M2, SQ8, FP32).  Size in bytes: */
 int64_t orq_query_size(const orq_quant *q);

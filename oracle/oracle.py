@@ -15,7 +15,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboracle.so")
 
-KINDS = {"FP32": 0, "E5M2": 3, "SQ8U": 4, "SQ8": 5, "SQ4U": 7, "SQ2U": 11}
+KINDS = {"FP32": 0, "FP16": 1, "BF16": 2, "E5M2": 3, "SQ8U": 4, "SQ8": 5, "SQ4U": 7, "SQ2U": 11}
 METRICS = {"L2": 0, "IP": 1}
 
 

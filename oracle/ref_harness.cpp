@@ -80,6 +80,8 @@ int quant_dispatch(const std::string &quant, const float *data, int n, int d, co
     if (quant == "E5M2") return quant_probe<E5M2Quantizer<M>>(data, n, d, qs, nq, ids, nids, codes_out, params_out, dist_out, cs, da);
     if (quant == "SQ8") return quant_probe<SQ8Quantizer<M>>(data, n, d, qs, nq, ids, nids, codes_out, params_out, dist_out, cs, da);
     if (quant == "FP32") return quant_probe<FP32Quantizer<M>>(data, n, d, qs, nq, ids, nids, codes_out, params_out, dist_out, cs, da);
+    if (quant == "FP16") return quant_probe<FP16Quantizer<M>>(data, n, d, qs, nq, ids, nids, codes_out, params_out, dist_out, cs, da);
+    if (quant == "BF16") return quant_probe<BF16Quantizer<M>>(data, n, d, qs, nq, ids, nids, codes_out, params_out, dist_out, cs, da);
     return -1;
 }
 

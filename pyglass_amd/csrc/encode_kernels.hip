@@ -1,5 +1,6 @@
 // Device-side SetData: encode every base row with the trained calibrator
-// (Quantizer::add, e.g. glass/quant/sq4u_quant.hpp:99-117) and lay out the
+// (Quantizer::add, e.g. glass/quant/sq4u_quant.hpp:99-117), or convert it to
+// fp16 / bf16 (fp16_quant.hpp:27-35, bf16_quant.hpp:26-38), and lay out the
 // FP32 re-rank rows (FP32Quantizer::add, glass/quant/fp32_quant.hpp:25-31).
 // One thread per output byte; the arithmetic is the reference's
 // transform(): (x - min) / dif, clamp to [0,1], round half away from zero.
@@ -56,6 +57,26 @@ __global__ void encode_kernel(const float *data, int64_t n, int32_t dim, int32_t
             case QK_SQ8:  // sq8_quant.hpp:43-47, AffinePerDimCalibrator<255>
                 if (b < dim) byte = affine_code(x[b], mins[b], difs[b], 255.0f) & 0xffu;
                 break;
+            case QK_FP16: {
+                // FP16Quantizer::encode -> helpa::fp32_to_fp16 (RNE).  Its
+                // block converter (x86/data_convert.hpp:18-31) also fills the
+                // padding [dim, dim_align) from the floats that follow the row
+                // in the input; past the end of the input the padding is 0.
+                const int32_t j = b >> 1;
+                const int64_t at = row * dim + j;
+                float v = 0.0f;
+                if (j < dim) v = x[j];
+                else if (at < n * (int64_t)dim) v = data[at];
+                const uint32_t h = __half_as_ushort(__float2half_rn(v));
+                byte = (b & 1) ? (h >> 8) : (h & 0xffu);
+                break;
+            }
+            case QK_BF16: {  // bf16_quant.hpp:34-38: helpa::bf16, (bits + 0x8000) >> 16; padding 0
+                const int32_t j = b >> 1;
+                const uint32_t h = j < dim ? ((__float_as_uint(x[j]) + 0x8000u) >> 16) : 0u;
+                byte = (b & 1) ? (h >> 8) : (h & 0xffu);
+                break;
+            }
             default:
                 break;
         }

@@ -103,6 +103,8 @@ int32_t parse_quant(const std::string &s) {
     if (s == "E5M2" || s == "FP8_E5M2") return QK_E5M2;
     if (s == "SQ8") return QK_SQ8;
     if (s == "FP32") return QK_FP32;
+    if (s == "FP16") return QK_FP16;
+    if (s == "BF16") return QK_BF16;
     return QK_NONE;
 }
 
@@ -116,6 +118,10 @@ void quant_layout(int32_t kind, int32_t dim, int32_t *dim_align, int32_t *code_s
         case QK_E5M2: align = 16; nbits = 8; break;
         case QK_SQ8: align = 16; nbits = 8; break;
         case QK_FP32: align = 16; nbits = 32; break;
+        // fp16_quant.hpp:9-15 / bf16_quant.hpp:9-15 without the AVX512FP16 /
+        // AVX512BF16 opt-in macros
+        case QK_FP16: align = 16; nbits = 16; break;
+        case QK_BF16: align = 16; nbits = 16; break;
     }
     *dim_align = (dim + align - 1) / align * align;  // Tensor ctor, glass/storage/tensor.hpp:30-35
     *code_size = *dim_align * nbits / 8;
@@ -205,7 +211,8 @@ struct glass_hip_searcher {
     DevBuf<int32_t> d_adj, d_levels, d_up_lists, d_eps;
     DevBuf<int64_t> d_up_off;
     DevBuf<uint8_t> d_codes;
-    DevBuf<float> d_rerank, d_mins, d_difs;
+    DevBuf<uint8_t> d_rerank;  // re-rank rows, fp32 or fp16 (refine)
+    DevBuf<float> d_mins, d_difs;
     // per-slot state
     DevBuf<uint32_t> d_visited;
     int64_t vis_stride = 0;
@@ -261,9 +268,10 @@ struct glass_hip_searcher {
         ix.qmax = qmax;
         ix.sq8_mins = d_mins.p;
         ix.sq8_difs = d_difs.p;
-        ix.rerank = refine == QK_FP32 ? d_rerank.p : nullptr;
-        ix.r_dim_align = refine == QK_FP32 ? r_dim_align : 0;
-        ix.refine_factor = refine == QK_FP32 ? factor() : 1.0f;
+        ix.rerank = refine != QK_NONE ? (const void *)d_rerank.p : nullptr;
+        ix.r_dim_align = refine != QK_NONE ? r_dim_align : 0;
+        ix.rerank_kind = refine;
+        ix.refine_factor = refine != QK_NONE ? factor() : 1.0f;
         return ix;
     }
 };
@@ -674,7 +682,7 @@ int run_batch(glass_hip_searcher &s, const float *qs_dev, int32_t nq, const int3
 }
 
 int32_t inner_k(const glass_hip_searcher &s, int32_t k) {
-    if (s.refine == QK_FP32) {
+    if (s.refine != QK_NONE) {
         int32_t rk = (int32_t)((float)k * s.factor());
         return std::max(rk, k);
     }
@@ -888,10 +896,12 @@ int glass_hip_searcher_create(glass_hip_graph *g, const char *metric, const char
     else return fail(GLASS_HIP_EINVAL, "unknown metric '%s'", metric);
     int32_t qk = parse_quant(q);
     if (qk == QK_NONE) return fail(GLASS_HIP_EINVAL, "unsupported quantizer '%s'", quantizer);
+    // make_refiner (graph_searcher.hpp:240-254): FP32 and FP16 re-rank rows
     int32_t rk = QK_NONE;
     if (!r.empty()) {
-        if (r != "FP32") return fail(GLASS_HIP_EINVAL, "unsupported refine quantizer '%s'", refine_quant);
-        rk = QK_FP32;
+        if (r == "FP32") rk = QK_FP32;
+        else if (r == "FP16") rk = QK_FP16;
+        else return fail(GLASS_HIP_EINVAL, "unsupported refine quantizer '%s'", refine_quant);
     }
     if (g->K > 64) return fail(GLASS_HIP_EINVAL, "graph degree %d > 64 is not supported", g->K);
     if (g->has_upper && g->up_K > 64) return fail(GLASS_HIP_EINVAL, "upper-layer degree %d > 64", g->up_K);
@@ -917,10 +927,8 @@ int glass_hip_searcher_set_data(glass_hip_searcher *s, const float *data, int32_
     s->nb = n;
     s->dim = dim;
     quant_layout(s->quant, dim, &s->dim_align, &s->code_size);
-    if (s->refine == QK_FP32) {
-        int32_t cs;
-        quant_layout(QK_FP32, dim, &s->r_dim_align, &cs);
-    }
+    int32_t r_row_bytes = 0;
+    if (s->refine != QK_NONE) quant_layout(s->refine, dim, &s->r_dim_align, &r_row_bytes);
     if (int r = train_quant(*s, data, n)) return r;
     // graph
     const glass_hip_graph &g = s->graph;
@@ -955,8 +963,12 @@ int glass_hip_searcher_set_data(glass_hip_searcher *s, const float *data, int32_
                               s->d_mins.p, s->d_difs.p, s->d_codes.p, s->stream));
     }
     if (s->refine == QK_FP32) {
-        HIP_TRY(s->d_rerank.alloc((size_t)n * s->r_dim_align));
-        HIP_TRY(launch_pad_rows(raw.p, n, dim, s->r_dim_align, s->d_rerank.p, s->stream));
+        HIP_TRY(s->d_rerank.alloc((size_t)n * r_row_bytes));
+        HIP_TRY(launch_pad_rows(raw.p, n, dim, s->r_dim_align, (float *)s->d_rerank.p, s->stream));
+    } else if (s->refine == QK_FP16) {  // FP16Quantizer::add, as for the search codes
+        HIP_TRY(s->d_rerank.alloc((size_t)n * r_row_bytes));
+        HIP_TRY(launch_encode(raw.p, n, dim, r_row_bytes, QK_FP16, s->metric, 0.0f, 0.0f, 0.0f, nullptr, nullptr,
+                              s->d_rerank.p, s->stream));
     }
     HIP_TRY(hipStreamSynchronize(s->stream));
     raw.release();
@@ -1189,10 +1201,11 @@ int glass_hip_searcher_info(const glass_hip_searcher *s, int32_t *code_size, int
                             int32_t *degree, int32_t *n, float *refine_factor) {
     if (!s) return fail(GLASS_HIP_EINVAL, "null searcher");
     if (code_size) *code_size = s->code_size;
-    if (rerank_row_bytes) *rerank_row_bytes = s->refine == QK_FP32 ? 4 * s->r_dim_align : 0;
+    if (rerank_row_bytes)
+        *rerank_row_bytes = s->refine == QK_FP32 ? 4 * s->r_dim_align : (s->refine == QK_FP16 ? 2 * s->r_dim_align : 0);
     if (degree) *degree = s->graph.K;
     if (n) *n = s->graph.N;
-    if (refine_factor) *refine_factor = s->refine == QK_FP32 ? s->factor() : 1.0f;
+    if (refine_factor) *refine_factor = s->refine != QK_NONE ? s->factor() : 1.0f;
     return GLASS_HIP_OK;
 }
 

@@ -34,9 +34,11 @@ struct DeviceIndex {
     float qmin, qdif, qmax;  // affine / symmetric calibration
     const float *sq8_mins;   // dim_align (SQ8 per-dim)
     const float *sq8_difs;
-    // FP32 re-rank rows (Refiner<FP32Quantizer>), or null
-    const float *rerank;
+    // re-rank rows (Refiner<FP32Quantizer> or Refiner<FP16Quantizer>), or
+    // null: N x r_dim_align fp32 (rerank_kind QK_FP32) or fp16 (QK_FP16)
+    const void *rerank;
     int32_t r_dim_align;
+    int32_t rerank_kind;
     float refine_factor;     // get_refine_factor(quantizer)
 };
 

@@ -124,7 +124,30 @@ enum Variant : int {
     V_SQ8_IP = 13,
     V_FP32_L2 = 14,
     V_FP32_IP = 15,
+    V_FP16_L2 = 16,  // helpa::l2_fp32_fp16 / dot_fp32_fp16 (x86/l2_impl.hpp:20-24,74-84, x86/dot_impl.hpp:22-26,74-84)
+    V_FP16_IP = 17,
+    V_BF16_L2 = 18,  // helpa::l2_fp32_bf16 / dot_fp32_bf16 (x86/l2_impl.hpp:32-36,145-156, x86/dot_impl.hpp:34-38,139-149)
+    V_BF16_IP = 19,
 };
+
+// 16-bit codes: the fp32 kernel's arithmetic on widened values.
+constexpr bool is_f16_variant(int v) { return v == V_FP16_L2 || v == V_FP16_IP; }
+constexpr bool is_bf16_variant(int v) { return v == V_BF16_L2 || v == V_BF16_IP; }
+constexpr bool is_l2_float(int v) { return v == V_FP32_L2 || v == V_FP16_L2 || v == V_BF16_L2; }
+
+// Four 16-bit codes (one uint2) widened to fp32: fp16 exactly, bf16 as the
+// high half of the float.
+template <int V>
+__device__ __forceinline__ float4 widen4(uint2 h) {
+    if (is_bf16_variant(V)) {
+        return make_float4(__uint_as_float(h.x << 16), __uint_as_float(h.x & 0xffff0000u), __uint_as_float(h.y << 16),
+                           __uint_as_float(h.y & 0xffff0000u));
+    }
+    return make_float4(__half2float(__ushort_as_half((unsigned short)(h.x & 0xffffu))),
+                       __half2float(__ushort_as_half((unsigned short)(h.x >> 16))),
+                       __half2float(__ushort_as_half((unsigned short)(h.y & 0xffffu))),
+                       __half2float(__ushort_as_half((unsigned short)(h.y >> 16))));
+}
 
 constexpr bool is_int_variant(int v) { return v < 10; }
 
@@ -436,17 +459,21 @@ __device__ __forceinline__ float float_row4(const DeviceIndex &ix, const WaveLds
             }
         }
         return reduce16_4lanes<true>(acc);
-    } else {  // FP32
-        const float *y = (const float *)row;
+    } else {  // FP32, FP16, BF16
 #pragma unroll 4
         for (int t = 0; t < da; t += 16) {
             const int j = t + 4 * p;
-            const float4 yv = *(const float4 *)(y + j);
+            float4 yv;
+            if (V == V_FP32_L2 || V == V_FP32_IP) {
+                yv = *(const float4 *)((const float *)row + j);
+            } else {
+                yv = widen4<V>(*(const uint2 *)((const uint16_t *)row + j));
+            }
             const float4 x = *(const float4 *)(w.qf + j);
             const float xs[4] = {x.x, x.y, x.z, x.w}, ys[4] = {yv.x, yv.y, yv.z, yv.w};
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                if (V == V_FP32_L2) {
+                if (is_l2_float(V)) {
                     const float d = xs[r] - ys[r];
                     acc.a[r] = fmaf(d, d, acc.a[r]);
                 } else {
@@ -458,25 +485,40 @@ __device__ __forceinline__ float float_row4(const DeviceIndex &ix, const WaveLds
     }
 }
 
-// Exact FP32 distances (Refiner, refiner.hpp:92-110) of pool entries [0, m),
-// written over their keys in pk.  Rows of S * 16 floats (S <= 8, a
-// compile-time count, so the loop has no branches): each 4-lane group takes
-// R rows per pass (entries base + 16r + g) and issues every load of the pass
-// before any arithmetic, so a pass waits on memory once; each accumulator
-// still sums its elements in increasing order, as float_row4 does.
+// Exact distances (Refiner, refiner.hpp:92-110) of pool entries [0, m),
+// written over their keys in pk, from fp32 rows (VF = V_FP32_*) or fp16 rows
+// (VF = V_FP16_*).  Rows of S * 16 values (S <= 8, a compile-time count, so
+// the loop has no branches): each 4-lane group takes R rows per pass
+// (entries base + 16r + g) and issues every load of the pass before any
+// arithmetic, so a pass waits on memory once; each accumulator still sums
+// its elements in increasing order, as float_row4 does.
 template <int VF, int R, int S>
 __device__ void rerank_rows(const DeviceIndex &ix, const WaveLds &w, int m) {
+    constexpr bool kHalf = is_f16_variant(VF);
     const int l = lane_id();
     const int p4 = l & 3, g = l >> 2;
     for (int base = 0; base < m; base += 16 * R) {
         float4 y[R][S];
+        uint2 h[R][S];
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             const int j = base + 16 * r + g;
             const uint32_t id = w.pid[j < m ? j : m - 1] & kIdMask;
-            const float *row = ix.rerank + (int64_t)id * (16 * S) + 4 * p4;
+            const int64_t at = (int64_t)id * (16 * S) + 4 * p4;
 #pragma unroll
-            for (int s = 0; s < S; ++s) y[r][s] = *(const float4 *)(row + 16 * s);
+            for (int s = 0; s < S; ++s) {
+                if (kHalf) {
+                    h[r][s] = *(const uint2 *)((const uint16_t *)ix.rerank + at + 16 * s);
+                } else {
+                    y[r][s] = *(const float4 *)((const float *)ix.rerank + at + 16 * s);
+                }
+            }
+        }
+        if (kHalf) {
+#pragma unroll
+            for (int r = 0; r < R; ++r)
+#pragma unroll
+                for (int s = 0; s < S; ++s) y[r][s] = widen4<VF>(h[r][s]);
         }
         Acc4 acc[R];
 #pragma unroll
@@ -490,7 +532,7 @@ __device__ void rerank_rows(const DeviceIndex &ix, const WaveLds &w, int m) {
                 const float ys[4] = {y[r][s].x, y[r][s].y, y[r][s].z, y[r][s].w};
 #pragma unroll
                 for (int e = 0; e < 4; ++e) {
-                    if (VF == V_FP32_L2) {
+                    if (is_l2_float(VF)) {
                         const float d = xs[e] - ys[e];
                         acc[r].a[e] = fmaf(d, d, acc[r].a[e]);
                     } else {
@@ -507,17 +549,17 @@ __device__ void rerank_rows(const DeviceIndex &ix, const WaveLds &w, int m) {
 #pragma unroll
             for (int r = 0; r < R; ++r) {
                 const int j = base + 16 * r + g;
-                if (j < m) w.pk[j] = key_of_float(VF == V_FP32_L2 ? rs[r] : -rs[r]);
+                if (j < m) w.pk[j] = key_of_float(is_l2_float(VF) ? rs[r] : -rs[r]);
             }
         }
         wave_sync();
     }
 }
 
-// Exact FP32 distances of pool entries [0, m) into pk; rows longer than 128
-// floats go through float_row4, one row per 4-lane group per pass.
+// Exact distances of pool entries [0, m) into pk; rows longer than 128
+// values go through float_row4, one row per 4-lane group per pass.
 template <int VF>
-__device__ void rerank_fp32(const DeviceIndex &ix, const WaveLds &w, int m) {
+__device__ void rerank(const DeviceIndex &ix, const WaveLds &w, int m) {
     const int da = ix.r_dim_align;  // a multiple of 16
     switch (da) {
         case 16: return rerank_rows<VF, 1, 1>(ix, w, m);
@@ -535,10 +577,10 @@ __device__ void rerank_fp32(const DeviceIndex &ix, const WaveLds &w, int m) {
     for (int base = 0; base < m; base += 16) {
         const int j = base + g;
         const uint32_t id = w.pid[j < m ? j : m - 1] & kIdMask;
-        const uint8_t *row = (const uint8_t *)(ix.rerank + (int64_t)id * da);
+        const uint8_t *row = (const uint8_t *)ix.rerank + (int64_t)id * da * (is_f16_variant(VF) ? 2 : 4);
         const float rsum = float_row4<VF>(ix, w, row, p4, MODE_BATCH, da);
         wave_sync();
-        if (j < m && p4 == 0) w.pk[j] = key_of_float(VF == V_FP32_L2 ? rsum : -rsum);
+        if (j < m && p4 == 0) w.pk[j] = key_of_float(is_l2_float(VF) ? rsum : -rsum);
         wave_sync();
     }
 }
@@ -553,7 +595,7 @@ __device__ void float_distances(const DeviceIndex &ix, const WaveLds &w, int m, 
         const int jj = j < m ? j : (m - 1);
         const uint32_t id = w.ci[jj];
         float r = float_row4<V>(ix, w, ix.codes + (int64_t)id * ix.code_size, p, mode, ix.dim_align);
-        bool neg = (V == V_E5M2_IP || V == V_SQ8_IP || V == V_FP32_IP);
+        bool neg = (V == V_E5M2_IP || V == V_SQ8_IP || V == V_FP32_IP || V == V_FP16_IP || V == V_BF16_IP);
         if (j < m && p == 0) w.ck[j] = key_of_float(neg ? -r : r);
     }
 }
@@ -1214,10 +1256,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void s
             // Refiner::RefineImpl (refiner.hpp:92-110): exact FP32 distances of the
             // first reorder_k pool entries, the k smallest by (dist, id) ascending.
             const int m = kk < ps.size ? kk : ps.size;
-            if (ix.metric == MK_L2) {
-                rerank_fp32<V_FP32_L2>(ix, w, m);
+            if (ix.rerank_kind == QK_FP16) {
+                if (ix.metric == MK_L2) {
+                    rerank<V_FP16_L2>(ix, w, m);
+                } else {
+                    rerank<V_FP16_IP>(ix, w, m);
+                }
+            } else if (ix.metric == MK_L2) {
+                rerank<V_FP32_L2>(ix, w, m);
             } else {
-                rerank_fp32<V_FP32_IP>(ix, w, m);
+                rerank<V_FP32_IP>(ix, w, m);
             }
             wave_sync();
             if (m <= 128) {
@@ -1378,6 +1426,8 @@ int kernel_vgprs(const DeviceIndex &ix, const LaunchConfig &lc) {
         case QK_E5M2: return l2 ? vgprs_one<V_E5M2_L2, 4>(ix, lc) : vgprs_one<V_E5M2_IP, 4>(ix, lc);
         case QK_SQ8: return l2 ? vgprs_one<V_SQ8_L2, 4>(ix, lc) : vgprs_one<V_SQ8_IP, 4>(ix, lc);
         case QK_FP32: return l2 ? vgprs_one<V_FP32_L2, 4>(ix, lc) : vgprs_one<V_FP32_IP, 4>(ix, lc);
+        case QK_FP16: return l2 ? vgprs_one<V_FP16_L2, 4>(ix, lc) : vgprs_one<V_FP16_IP, 4>(ix, lc);
+        case QK_BF16: return l2 ? vgprs_one<V_BF16_L2, 4>(ix, lc) : vgprs_one<V_BF16_IP, 4>(ix, lc);
         default: return 0;
     }
 }
@@ -1391,6 +1441,8 @@ int max_blocks_per_cu(const DeviceIndex &ix, const LaunchConfig &lc) {
         case QK_E5M2: return l2 ? occupancy_one<V_E5M2_L2, 4>(ix, lc) : occupancy_one<V_E5M2_IP, 4>(ix, lc);
         case QK_SQ8: return l2 ? occupancy_one<V_SQ8_L2, 4>(ix, lc) : occupancy_one<V_SQ8_IP, 4>(ix, lc);
         case QK_FP32: return l2 ? occupancy_one<V_FP32_L2, 4>(ix, lc) : occupancy_one<V_FP32_IP, 4>(ix, lc);
+        case QK_FP16: return l2 ? occupancy_one<V_FP16_L2, 4>(ix, lc) : occupancy_one<V_FP16_IP, 4>(ix, lc);
+        case QK_BF16: return l2 ? occupancy_one<V_BF16_L2, 4>(ix, lc) : occupancy_one<V_BF16_IP, 4>(ix, lc);
         default: return 0;
     }
 }
@@ -1424,6 +1476,8 @@ hipError_t launch_search(const DeviceIndex &ix, const QueryBatch &qb, const Laun
         case QK_E5M2: return l2 ? launch_one<V_E5M2_L2, 4>(ix, qb, lc, ss, stream) : launch_one<V_E5M2_IP, 4>(ix, qb, lc, ss, stream);
         case QK_SQ8: return l2 ? launch_one<V_SQ8_L2, 4>(ix, qb, lc, ss, stream) : launch_one<V_SQ8_IP, 4>(ix, qb, lc, ss, stream);
         case QK_FP32: return l2 ? launch_one<V_FP32_L2, 4>(ix, qb, lc, ss, stream) : launch_one<V_FP32_IP, 4>(ix, qb, lc, ss, stream);
+        case QK_FP16: return l2 ? launch_one<V_FP16_L2, 4>(ix, qb, lc, ss, stream) : launch_one<V_FP16_IP, 4>(ix, qb, lc, ss, stream);
+        case QK_BF16: return l2 ? launch_one<V_BF16_L2, 4>(ix, qb, lc, ss, stream) : launch_one<V_BF16_IP, 4>(ix, qb, lc, ss, stream);
         default: return hipErrorInvalidValue;
     }
 }

@@ -106,3 +106,91 @@ def test_search_matches_reference(sz, dsets, golden_dir, gname, quant, refine):
                 np.testing.assert_allclose(sd, sz[f"{key}/single_dists"], rtol=2.5e-7, err_msg=key)
             else:
                 np.testing.assert_array_equal(sd, sz[f"{key}/single_dists"], err_msg=key)
+
+
+# FP16 / BF16 (tests/golden/make_golden_f16.py)
+
+@pytest.fixture(scope="module")
+def qz16(golden_dir):
+    return dict(np.load(os.path.join(golden_dir, "quant_f16.npz")))
+
+
+@pytest.fixture(scope="module")
+def sz16(golden_dir):
+    return np.load(os.path.join(golden_dir, "search_f16.npz"))
+
+
+def _datasets_f16():
+    d = _datasets()
+    X100 = synth.gaussian(3000, 100, synth.BASE_SEED)
+    Q100 = synth.gaussian(200, 100, synth.QUERY_SEED)
+    return {"c1_hnsw_l2": d["c1_hnsw_l2"], "ip_hnsw": d["ip_hnsw"], "g100_hnsw_l2": (X100, Q100, "L2")}
+
+
+@pytest.fixture(scope="module")
+def dsets16():
+    return _datasets_f16()
+
+
+@pytest.mark.parametrize("d", [128, 100])
+@pytest.mark.parametrize("metric", ["L2", "IP"])
+@pytest.mark.parametrize("quant", ["FP16", "BF16"])
+def test_f16_quantizer_matches_reference(qz16, dsets16, quant, metric, d):
+    # d = 100: the fp16 padding of every row holds the floats that follow it
+    # in memory (the next row's first values), as the reference's block
+    # converter leaves it; past the last row the fixture's input is zeros
+    X, Q, _ = dsets16["c1_hnsw_l2" if d == 128 else "g100_hnsw_l2"]
+    assert sha(X) == str(qz16[f"{d}/data_sha"])
+    key = f"{quant}_{metric}_{d}"
+    oq = O.Quant(quant, metric, d).fit(X)
+    assert oq.code_size == int(qz16[f"{key}/code_size"])
+    assert oq.dim_align == int(qz16[f"{key}/dim_align"])
+    np.testing.assert_array_equal(oq.codes[:8], qz16[f"{key}/codes_head"])
+    assert sha(oq.codes) == str(qz16[f"{key}/codes_sha"])
+    ids = qz16[f"{d}/ids"]
+    got = np.stack([oq.distances(Q[i], ids) for i in range(16)])
+    np.testing.assert_array_equal(got, qz16[f"{key}/dist"])
+
+
+def test_f16_padding_holds_the_next_row():
+    # the layout rule itself, on numbers small enough to read: dim 3, padded
+    # to 16; row i's padding j (3 <= j < 16) is fp16(data[i*3 + j]) while
+    # that lies inside the input, else 0; bf16 padding is always 0
+    X = np.arange(1, 1 + 6 * 3, dtype=np.float32).reshape(6, 3)
+    q16 = O.Quant("FP16", "L2", 3).fit(X)
+    h = q16.codes.view(np.uint16)
+    flat = X.reshape(-1)
+    for i in range(6):
+        for j in range(16):
+            at = i * 3 + j
+            want = O.f32_to_f16(float(flat[at])) if at < flat.size else 0
+            assert h[i, j] == want, (i, j)
+    qb = O.Quant("BF16", "L2", 3).fit(X)
+    hb = qb.codes.view(np.uint16)
+    assert (hb[:, 3:] == 0).all()
+    np.testing.assert_array_equal(hb[:, :3], (X.view(np.uint32) + 0x8000) >> 16)
+
+
+CASES16 = [(g, q, "none") for g in ["c1_hnsw_l2", "ip_hnsw", "g100_hnsw_l2"] for q in ["FP16", "BF16"]]
+CASES16 += [(g, "SQ4U", "FP16") for g in ["c1_hnsw_l2", "ip_hnsw"]]
+
+
+@pytest.mark.parametrize("gname,quant,refine", CASES16)
+def test_f16_search_matches_reference(sz16, dsets16, golden_dir, gname, quant, refine):
+    XX, QQ, metric = dsets16[gname]
+    assert sha(XX) == str(sz16[f"{gname}/data_sha"])
+    g = O.Graph(os.path.join(golden_dir, f"{gname}.glass"))
+    s = O.Searcher(g, XX, metric, quant, "" if refine == "none" else refine)
+    nq = 64
+    for ef in [16, 40, 96]:
+        for k in [1, 10, 50]:
+            key = f"{gname}/{quant}/{refine}/{ef}/{k}"
+            s.set_ef(ef)
+            I, D = s.batch_search(QQ[:nq], k)
+            np.testing.assert_array_equal(I, sz16[f"{key}/batch_ids"], err_msg=key)
+            np.testing.assert_array_equal(D, sz16[f"{key}/batch_dists"], err_msg=key)
+            assert s.get_last_search_dist_cmps() == float(sz16[f"{key}/batch_dist_cmps"]), key
+            si = np.stack([s.search(QQ[i], k)[0] for i in range(nq)])
+            sd = np.stack([s.search(QQ[i], k)[1] for i in range(nq)])
+            np.testing.assert_array_equal(si, sz16[f"{key}/single_ids"], err_msg=key)
+            np.testing.assert_array_equal(sd, sz16[f"{key}/single_dists"], err_msg=key)
