@@ -154,6 +154,7 @@ int orq_quant_init(orq_quant *q, int32_t kind, int32_t metric, int32_t dim) {
         case ORQ_SQ8: q->align = 16; q->nbits = 8; break;
         case ORQ_FP32: q->align = 16; q->nbits = 32; break;
         case ORQ_FP16: q->align = 16; q->nbits = 16; break;
+        case ORQ_SQ6: q->align = 8; q->nbits = 6; break; /* sq6_quant.hpp:13-15 */
         case ORQ_BF16: q->align = 16; q->nbits = 16; break;
         default: return -1;
     }
@@ -209,6 +210,22 @@ int orq_quant_train(orq_quant *q, const float *data, int64_t n) {
                     if (p[i] < lo) lo = p[i];
                     if (p[i] > hi) hi = p[i];
                 }
+                q->mins[j] = lo;
+                q->difs[j] = hi - lo;
+            }
+            break;
+        }
+        case ORQ_SQ6: {
+            /* AffinePerDimCalibrator<63> with drop ratio 0.01 through
+             * find_minmax_perdim (sq6_quant.hpp:17-31, utils.hpp:52-70):
+             * chunk j = flattened items [j*n, (j+1)*n), its top-th smallest
+             * and largest values, top = int64(n * 0.01f) + 1; padding dims
+             * keep min = dif = 0 (mins/difs resized to dim_align). */
+            q->mins = (float *)calloc((size_t)q->dim_align, sizeof(float));
+            q->difs = (float *)calloc((size_t)q->dim_align, sizeof(float));
+            for (int32_t j = 0; j < q->dim; ++j) {
+                float lo, hi;
+                find_minmax(data + (int64_t)j * n, n, 0.01f, &lo, &hi);
                 q->mins[j] = lo;
                 q->difs[j] = hi - lo;
             }
@@ -271,6 +288,18 @@ void orq_encode(const orq_quant *q, const float *x, uint8_t *out) {
             for (int32_t j = 0; j < d; ++j) o[j] = orq_f32_to_f16(x[j]);
             break;
         }
+        case ORQ_SQ6: /* SQ6Quantizer::encode_component (sq6_quant.hpp:41-62): 4 components per 3 bytes */
+            for (int32_t j = 0; j < d; ++j) {
+                const int32_t c = affine_code(x[j], q->mins[j], q->difs[j], 63);
+                uint8_t *g = out + (j >> 2) * 3;
+                switch (j & 3) {
+                    case 0: g[0] |= (uint8_t)c; break;
+                    case 1: g[0] |= (uint8_t)(c << 6); g[1] |= (uint8_t)(c >> 2); break;
+                    case 2: g[1] |= (uint8_t)(c << 4); g[2] |= (uint8_t)(c >> 4); break;
+                    default: g[2] |= (uint8_t)(c << 2); break;
+                }
+            }
+            break;
         case ORQ_BF16: { /* BF16Quantizer::encode (bf16_quant.hpp:34-38), padding stays zero */
             uint16_t *o = (uint16_t *)out;
             for (int32_t j = 0; j < d; ++j) o[j] = orq_f32_to_bf16(x[j]);
@@ -338,6 +367,7 @@ static float reduce16_helpa(const float a[16]) {
     return (c[0] + c[2]) + (c[1] + c[3]);
 }
 
+
 /* ctx selects the FMA contraction GCC -Ofast chose where the reference's
  * SQ8 L2 kernel is inlined: ORQ_CTX_COMPUTER (a bare computer call) and
  * ORQ_CTX_SEARCH (Search/SearchImpl1) fuse as fma(-c, dif, 256x - 256min);
@@ -403,7 +433,7 @@ double orq_distance_ctx(const orq_quant *q, const uint8_t *qbuf, const uint8_t *
                     float c = (float)code[i + l] + 0.5f;
                     if (q->metric == ORQ_L2) {
                         float t;
-                        if (ctx == ORQ_CTX_BATCH) {
+                        if ((ctx & 3) == ORQ_CTX_BATCH) {
                             t = x[i + l] * 256.0f - fmaf(c, q->difs[i + l], q->mins[i + l] * 256.0f);
                         } else {
                             t = fmaf(-c, q->difs[i + l], x[i + l] * 256.0f - q->mins[i + l] * 256.0f);
@@ -416,6 +446,48 @@ double orq_distance_ctx(const orq_quant *q, const uint8_t *qbuf, const uint8_t *
                 }
             }
             float r = reduce16_glass(acc);
+            return q->metric == ORQ_L2 ? (double)r : (double)(-r);
+        }
+        case ORQ_SQ6: { /* glass::L2SqrSQ6_ext / IPSQ6_ext (glass/simd/avx512.hpp:79-104), 8 accumulators */
+            const float *x = (const float *)qbuf;
+            float acc[8] = {0};
+            for (int32_t i = 0; i < da; i += 8) {
+                for (int l = 0; l < 8; ++l) {
+                    const int32_t j = i + l;
+                    const uint8_t *g = code + (j >> 2) * 3;
+                    int32_t c;
+                    switch (j & 3) { /* SQ6::load6 (avx2_utils.hpp:32-49) */
+                        case 0: c = g[0] & 63; break;
+                        case 1: c = (g[0] >> 6) | ((g[1] & 15) << 2); break;
+                        case 2: c = (g[1] >> 4) | ((g[2] & 3) << 4); break;
+                        default: c = g[2] >> 2; break;
+                    }
+                    /* GCC -Ofast reassociates the decode ((c + 0.5) / 63) * dif
+                     * + mi per inlining context (x86 code GCC emits for
+                     * glass/simd/avx512.hpp:79-104 with the reference's flags,
+                     * checked against its golden distances).  With a = c + 0.5:
+                     * a bare computer and Search's expansions scale dif first,
+                     * kd = (1/63) * dif: L2 t = fma(-a, kd, x - mi), IP yy =
+                     * fma(a, kd, mi); SearchBatch and initialize_search scale a
+                     * first, ad = a * dif: L2 t = fma(-ad, 1/63, x - mi), IP
+                     * yy = fma(ad, 1/63, mi). */
+                    const float a = (float)c + 0.5f;
+                    const int ad_form = (ctx & ORQ_CTX_INIT) || (ctx & 3) == ORQ_CTX_BATCH;
+                    if (q->metric == ORQ_L2) {
+                        const float t = ad_form ? fmaf(-(a * q->difs[j]), 1.0f / 63.0f, x[j] - q->mins[j])
+                                                : fmaf(-a, (1.0f / 63.0f) * q->difs[j], x[j] - q->mins[j]);
+                        acc[l] = fmaf(t, t, acc[l]);
+                    } else {
+                        const float yy = ad_form ? fmaf(a * q->difs[j], 1.0f / 63.0f, q->mins[j])
+                                                 : fmaf(a, (1.0f / 63.0f) * q->difs[j], q->mins[j]);
+                        acc[l] = fmaf(x[j], yy, acc[l]);
+                    }
+                }
+            }
+            /* glass::reduce_add_f32x8 (avx2_utils.hpp:11-16): h_i = a_i + a_(i+4), (h0 + h2) + (h1 + h3) */
+            float h[4];
+            for (int i = 0; i < 4; ++i) h[i] = acc[i] + acc[i + 4];
+            const float r = (h[0] + h[2]) + (h[1] + h[3]);
             return q->metric == ORQ_L2 ? (double)r : (double)(-r);
         }
         case ORQ_FP32:   /* helpa::l2_fp32_fp32 / dot_fp32_fp32 (x86/l2_impl.hpp:14-18,50-71; x86/dot_impl.hpp:16-20,52-71) */
@@ -628,7 +700,7 @@ void orq_search(const orq_graph *g, const orq_quant *q, const uint8_t *codes, co
         /* HNSWInitializer::initialize (HNSWInitializer.hpp:41-62): the list
          * scanned in one pass is the one of the node current at pass start. */
         int32_t u = g->ep;
-        double du = eval(q, qbuf, codes, u, cnt, ctx);
+        double du = eval(q, qbuf, codes, u, cnt, ctx | ORQ_CTX_INIT);
         for (int32_t level = g->levels[u]; level > 0; --level) {
             int changed = 1;
             while (changed) {
@@ -636,7 +708,7 @@ void orq_search(const orq_graph *g, const orq_quant *q, const uint8_t *codes, co
                 const int32_t *list = g->lists + g->offsets[u] + (int64_t)(level - 1) * g->init_K;
                 for (int32_t i = 0; i < g->init_K && list[i] != -1; ++i) {
                     int32_t v = list[i];
-                    double dv = eval(q, qbuf, codes, v, cnt, ctx);
+                    double dv = eval(q, qbuf, codes, v, cnt, ctx | ORQ_CTX_INIT);
                     cnt->upper_evals++;
                     if (dv < du) {
                         du = dv;
@@ -650,7 +722,7 @@ void orq_search(const orq_graph *g, const orq_quant *q, const uint8_t *codes, co
         p.vis[u] = 1;
     } else {
         for (int32_t i = 0; i < g->nep; ++i) {
-            pool_insert(&p, g->eps[i], eval(q, qbuf, codes, g->eps[i], cnt, ctx));
+            pool_insert(&p, g->eps[i], eval(q, qbuf, codes, g->eps[i], cnt, ctx | ORQ_CTX_INIT));
             p.vis[g->eps[i]] = 1;
         }
     }

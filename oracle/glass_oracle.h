@@ -16,7 +16,7 @@ extern "C" {
 #endif
 
 /* quantizer kinds (names as in glass/quant/quant.hpp:23-43) */
-enum { ORQ_FP32 = 0, ORQ_FP16 = 1, ORQ_BF16 = 2, ORQ_E5M2 = 3, ORQ_SQ8U = 4, ORQ_SQ8 = 5, ORQ_SQ4U = 7, ORQ_SQ2U = 11 };
+enum { ORQ_FP32 = 0, ORQ_FP16 = 1, ORQ_BF16 = 2, ORQ_E5M2 = 3, ORQ_SQ8U = 4, ORQ_SQ8 = 5, ORQ_SQ6 = 6, ORQ_SQ4U = 7, ORQ_SQ2U = 11 };
 enum { ORQ_L2 = 0, ORQ_IP = 1 };
 
 typedef struct {
@@ -58,7 +58,9 @@ int64_t orq_query_size(const orq_quant *q);
 void orq_prepare_query(const orq_quant *q, const float *x, uint8_t *qbuf);
 /* Distance of prepared query to one code row, as the reference computer. */
 double orq_distance(const orq_quant *q, const uint8_t *qbuf, const uint8_t *code);
-enum { ORQ_CTX_COMPUTER = 0, ORQ_CTX_SEARCH = 1, ORQ_CTX_BATCH = 2 };
+/* ctx: where the reference's computer is inlined (its FMA contraction can
+ * differ per context); ORQ_CTX_INIT is or-ed in for initialize_search. */
+enum { ORQ_CTX_COMPUTER = 0, ORQ_CTX_SEARCH = 1, ORQ_CTX_BATCH = 2, ORQ_CTX_INIT = 4 };
 double orq_distance_ctx(const orq_quant *q, const uint8_t *qbuf, const uint8_t *code, int32_t ctx);
 
 /* .glass graph I/O (glass/graph.hpp:106-153) */

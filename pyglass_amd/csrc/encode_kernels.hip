@@ -57,6 +57,19 @@ __global__ void encode_kernel(const float *data, int64_t n, int32_t dim, int32_t
             case QK_SQ8:  // sq8_quant.hpp:43-47, AffinePerDimCalibrator<255>
                 if (b < dim) byte = affine_code(x[b], mins[b], difs[b], 255.0f) & 0xffu;
                 break;
+            case QK_SQ6: {
+                // SQ6Quantizer::encode_component (sq6_quant.hpp:41-62): byte r
+                // of group g holds bits [8r, 8r+8) of components 4g..4g+3,
+                // six bits each from the least significant end
+                const int32_t g = b / 3, r = b - 3 * g;
+                uint32_t bits = 0;
+                for (int s = 0; s < 4; ++s) {
+                    const int32_t j = 4 * g + s;
+                    if (j < dim) bits |= (affine_code(x[j], mins[j], difs[j], 63.0f) & 63u) << (6 * s);
+                }
+                byte = (bits >> (8 * r)) & 0xffu;
+                break;
+            }
             case QK_FP16: {
                 // FP16Quantizer::encode -> helpa::fp32_to_fp16 (RNE).  Its
                 // block converter (x86/data_convert.hpp:18-31) also fills the

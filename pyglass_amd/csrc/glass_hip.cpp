@@ -102,6 +102,7 @@ int32_t parse_quant(const std::string &s) {
     if (s == "SQ2U") return QK_SQ2U;
     if (s == "E5M2" || s == "FP8_E5M2") return QK_E5M2;
     if (s == "SQ8") return QK_SQ8;
+    if (s == "SQ6") return QK_SQ6;
     if (s == "FP32") return QK_FP32;
     if (s == "FP16") return QK_FP16;
     if (s == "BF16") return QK_BF16;
@@ -117,6 +118,7 @@ void quant_layout(int32_t kind, int32_t dim, int32_t *dim_align, int32_t *code_s
         case QK_SQ2U: align = 256; nbits = 2; break;
         case QK_E5M2: align = 16; nbits = 8; break;
         case QK_SQ8: align = 16; nbits = 8; break;
+        case QK_SQ6: align = 8; nbits = 6; break;  // sq6_quant.hpp:13-15
         case QK_FP32: align = 16; nbits = 32; break;
         // fp16_quant.hpp:9-15 / bf16_quant.hpp:9-15 without the AVX512FP16 /
         // AVX512BF16 opt-in macros
@@ -508,6 +510,22 @@ int train_quant(glass_hip_searcher &s, const float *data, int64_t n) {
                 auto mm = std::minmax_element(data + (int64_t)j * n, data + (int64_t)(j + 1) * n);
                 s.sq8_mins[j] = *mm.first;
                 s.sq8_difs[j] = *mm.second - *mm.first;
+            }
+            break;
+        }
+        case QK_SQ6: {
+            // AffinePerDimCalibrator<63> with drop ratio 0.01 (sq6_quant.hpp:17-31):
+            // chunk j of find_minmax_perdim gives the top-th smallest and largest
+            // of items [j*n, (j+1)*n), top = int64(float(n) * 0.01f) + 1
+            s.sq8_mins.assign(s.dim_align, 0.0f);
+            s.sq8_difs.assign(s.dim_align, 0.0f);
+            const int64_t top = (int64_t)((float)n * 0.01f) + 1;
+            std::vector<float> v;
+            for (int32_t j = 0; j < s.dim; ++j) {
+                v.assign(data + (int64_t)j * n, data + (int64_t)(j + 1) * n);
+                auto mm = kth_extremes(v, top);
+                s.sq8_mins[j] = mm.first;
+                s.sq8_difs[j] = mm.second - mm.first;
             }
             break;
         }
@@ -949,7 +967,7 @@ int glass_hip_searcher_set_data(glass_hip_searcher *s, const float *data, int32_
     DevBuf<float> raw;
     HIP_TRY(raw.alloc((size_t)n * dim));
     HIP_TRY(hipMemcpy(raw.p, data, sizeof(float) * (size_t)n * dim, hipMemcpyHostToDevice));
-    if (s->quant == QK_SQ8) {
+    if (s->quant == QK_SQ8 || s->quant == QK_SQ6) {
         HIP_TRY(s->d_mins.alloc(s->dim_align));
         HIP_TRY(s->d_difs.alloc(s->dim_align));
         HIP_TRY(hipMemcpy(s->d_mins.p, s->sq8_mins.data(), 4 * (size_t)s->dim_align, hipMemcpyHostToDevice));
@@ -1218,7 +1236,7 @@ int glass_hip_searcher_get_codes(const glass_hip_searcher *s, uint8_t *codes) {
 
 int glass_hip_searcher_get_calibration(const glass_hip_searcher *s, float *params, int32_t *n) {
     if (!s || !n) return fail(GLASS_HIP_EINVAL, "null argument");
-    if (s->quant == QK_SQ8) {
+    if (s->quant == QK_SQ8 || s->quant == QK_SQ6) {
         *n = 2 * s->dim_align;
         if (params) {
             std::copy(s->sq8_mins.begin(), s->sq8_mins.end(), params);

@@ -128,7 +128,14 @@ enum Variant : int {
     V_FP16_IP = 17,
     V_BF16_L2 = 18,  // helpa::l2_fp32_bf16 / dot_fp32_bf16 (x86/l2_impl.hpp:32-36,145-156, x86/dot_impl.hpp:34-38,139-149)
     V_BF16_IP = 19,
+    V_SQ6_L2 = 20,   // glass::L2SqrSQ6_ext / IPSQ6_ext (glass/simd/avx512.hpp:79-104)
+    V_SQ6_IP = 21,
 };
+
+// Or-ed into the search mode for initialize_search's evaluations: the
+// compiled reference's SQ6 kernel rounds differently there than in Search's
+// expansions.
+constexpr int32_t kInitCtx = 4;
 
 // 16-bit codes: the fp32 kernel's arithmetic on widened values.
 constexpr bool is_f16_variant(int v) { return v == V_FP16_L2 || v == V_FP16_IP; }
@@ -159,7 +166,7 @@ struct WaveLds {
     uint32_t *ck;   // candidate keys [64]
     uint32_t *ci;   // candidate ids [64]
     float *qf;      // query, zero padded [qf_len]
-    float *qm;      // SQ8 helpers: 256x-256min [qf_len], 256min [qf_len], dif [qf_len]
+    float *qm;      // SQ8 helpers: 256x-256min, 256min, dif; SQ6: x-min, min, dif [3 x qf_len]
     uint8_t *qcode; // encoded query [code_size rounded to 16]
 };
 
@@ -176,7 +183,7 @@ __host__ __device__ inline size_t wave_lds_bytes(const DeviceIndex &ix, int32_t 
     b += (size_t)round16((int64_t)cap_max * 4) * 2;
     b += 64 * 4 * 2;
     b += (size_t)qf_len(ix) * 4;
-    if (ix.quant == QK_SQ8) b += (size_t)qf_len(ix) * 4 * 3;
+    if (ix.quant == QK_SQ8 || ix.quant == QK_SQ6) b += (size_t)qf_len(ix) * 4 * 3;
     b += (size_t)round16(ix.code_size);
     return b;
 }
@@ -198,7 +205,7 @@ __device__ inline WaveLds carve(uint8_t *base, const DeviceIndex &ix, int32_t ca
     w.qf = (float *)(base + off);
     off += (size_t)ql * 4;
     w.qm = (float *)(base + off);
-    if (ix.quant == QK_SQ8) off += (size_t)ql * 4 * 3;
+    if (ix.quant == QK_SQ8 || ix.quant == QK_SQ6) off += (size_t)ql * 4 * 3;
     w.qcode = base + off;
     return w;
 }
@@ -444,7 +451,7 @@ __device__ __forceinline__ float float_row4(const DeviceIndex &ix, const WaveLds
                 const float c = (float)((c4 >> (8 * r)) & 0xffu) + 0.5f;
                 if (V == V_SQ8_L2) {
                     float d;
-                    if (mode == MODE_BATCH) {
+                    if ((mode & 3) == MODE_BATCH) {
                         // SearchImpl2 context: yy = fma(c, dif, 256min), d = 256x - yy
                         d = w.qf[j + r] * 256.0f - fmaf(c, dif[j + r], m256[j + r]);
                     } else {
@@ -483,6 +490,75 @@ __device__ __forceinline__ float float_row4(const DeviceIndex &ix, const WaveLds
         }
         return reduce16_4lanes<false>(acc);
     }
+}
+
+// SQ6 codes (SQ6Quantizer::encode_component, glass/quant/sq6_quant.hpp:41-62):
+// components 4g..4g+3 in the 24 bits of bytes 3g..3g+2, six bits each from
+// the least significant end.  The reference keeps 8 accumulators, acc[l]
+// summing elements j = l, l+8, ... (glass/simd/avx512.hpp:79-104); here lane p
+// of a 4-lane group owns acc[2p] and acc[2p+1], i.e. bits [12p, 12p+12) of
+// every 48-bit step of 8 components.  The decode ((c + 0.5) / 63) * dif + min
+// is rounded as GCC -Ofast compiled it, which depends on the inlining
+// context: Search's expansions scale dif first (kd = dif / 63, then
+// fma(-a, kd, x - min) or fma(a, kd, min)); SearchBatch and
+// initialize_search scale a first (ad = a * dif, then fma(-ad, 1/63, x - min)
+// or fma(ad, 1/63, min)).
+template <int V>
+__device__ __forceinline__ float sq6_row4(const DeviceIndex &ix, const WaveLds &w, const uint8_t *row, int p,
+                                          int32_t mode, int32_t da) {
+    constexpr float kInv63 = 1.0f / 63.0f;
+    const int ql = qf_len(ix);
+    const float *xm = w.qm, *mi = w.qm + ql, *dif = w.qm + 2 * ql;
+    const bool ad = mode != MODE_SEARCH;
+    const uint32_t sh = 12u * (uint32_t)p;
+    float acc0 = 0.0f, acc1 = 0.0f;
+    auto step = [&](uint64_t bits48, int t) {
+        const uint32_t b = (uint32_t)(bits48 >> sh);
+        const int j = 8 * t + 2 * p;
+        const float ca = (float)(b & 63u) + 0.5f, cb = (float)((b >> 6) & 63u) + 0.5f;
+        const float2 dv = *(const float2 *)(dif + j);
+        if (V == V_SQ6_L2) {
+            const float2 xv = *(const float2 *)(xm + j);
+            const float ta = ad ? fmaf(-(ca * dv.x), kInv63, xv.x) : fmaf(-ca, kInv63 * dv.x, xv.x);
+            const float tb = ad ? fmaf(-(cb * dv.y), kInv63, xv.y) : fmaf(-cb, kInv63 * dv.y, xv.y);
+            acc0 = fmaf(ta, ta, acc0);
+            acc1 = fmaf(tb, tb, acc1);
+        } else {
+            const float2 xv = *(const float2 *)(w.qf + j);
+            const float2 mv = *(const float2 *)(mi + j);
+            const float ya = ad ? fmaf(ca * dv.x, kInv63, mv.x) : fmaf(ca, kInv63 * dv.x, mv.x);
+            const float yb = ad ? fmaf(cb * dv.y, kInv63, mv.y) : fmaf(cb, kInv63 * dv.y, mv.y);
+            acc0 = fmaf(xv.x, ya, acc0);
+            acc1 = fmaf(xv.y, yb, acc1);
+        }
+    };
+    const int steps = da >> 3;
+    if ((ix.code_size & 3) == 0) {
+        // rows are dword aligned and hold an even number of steps: 12 bytes per two steps
+#pragma unroll 2
+        for (int t = 0; t < steps; t += 2) {
+            const uint32_t *r32 = (const uint32_t *)(row + 6 * t);
+            const uint32_t d0 = r32[0], d1 = r32[1], d2 = r32[2];
+            step(((uint64_t)d1 << 32) | d0, t);
+            step(((uint64_t)d2 << 16) | (d1 >> 16), t + 1);
+        }
+    } else {
+        // rows of 6 * odd bytes are only 2-byte aligned
+#pragma unroll 2
+        for (int t = 0; t < steps; ++t) {
+            const uint16_t *r16 = (const uint16_t *)(row + 6 * t);
+            step(((uint64_t)r16[2] << 32) | ((uint32_t)r16[1] << 16) | r16[0], t);
+        }
+    }
+    // glass::reduce_add_f32x8 (glass/simd/avx2_utils.hpp:11-16): h_i = a_i +
+    // a_(i+4), result (h0 + h2) + (h1 + h3).  a_(i+4) sits two lanes up, and
+    // after that step lane p holds h_(2p), h_(2p+1) for p < 2: h_(i+2) is one
+    // lane up.
+    acc0 = acc0 + dpp_xor2(acc0);
+    acc1 = acc1 + dpp_xor2(acc1);
+    acc0 = acc0 + dpp_xor1(acc0);
+    acc1 = acc1 + dpp_xor1(acc1);
+    return acc0 + acc1;
 }
 
 // Exact distances (Refiner, refiner.hpp:92-110) of pool entries [0, m),
@@ -594,8 +670,15 @@ __device__ void float_distances(const DeviceIndex &ix, const WaveLds &w, int m, 
         const int j = base + cj;
         const int jj = j < m ? j : (m - 1);
         const uint32_t id = w.ci[jj];
-        float r = float_row4<V>(ix, w, ix.codes + (int64_t)id * ix.code_size, p, mode, ix.dim_align);
-        bool neg = (V == V_E5M2_IP || V == V_SQ8_IP || V == V_FP32_IP || V == V_FP16_IP || V == V_BF16_IP);
+        const uint8_t *row = ix.codes + (int64_t)id * ix.code_size;
+        float r;
+        if (V == V_SQ6_L2 || V == V_SQ6_IP) {
+            r = sq6_row4<V>(ix, w, row, p, mode, ix.dim_align);
+        } else {
+            r = float_row4<V>(ix, w, row, p, mode, ix.dim_align);
+        }
+        bool neg = (V == V_E5M2_IP || V == V_SQ8_IP || V == V_FP32_IP || V == V_FP16_IP || V == V_BF16_IP ||
+                    V == V_SQ6_IP);
         if (j < m && p == 0) w.ck[j] = key_of_float(neg ? -r : r);
     }
 }
@@ -1040,6 +1123,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void s
                 w.qm[ql + j] = mn * 256.0f;
                 w.qm[2 * ql + j] = df;
             }
+        } else if (V == V_SQ6_L2 || V == V_SQ6_IP) {
+            for (int j = l; j < ql; j += 64) {
+                float mn = j < ix.dim_align ? ix.sq8_mins[j] : 0.0f;
+                float df = j < ix.dim_align ? ix.sq8_difs[j] : 0.0f;
+                w.qm[j] = w.qf[j] - mn;
+                w.qm[ql + j] = mn;
+                w.qm[2 * ql + j] = df;
+            }
         }
         IntQuery<V, DPL> iq;
         if (is_int_variant(V)) {
@@ -1051,11 +1142,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void s
         VisitedSet vs;
         vs.reset(w.vt, lc.vis_hash, vis, ss.vis_stride);
 
-        auto dists = [&](int m) {
+        auto dists = [&](int m, int32_t mode) {
             if (is_int_variant(V)) {
                 iq.distances(ix, w, m);
             } else {
-                float_distances<V>(ix, w, m, qb.mode);
+                float_distances<V>(ix, w, m, mode);
             }
             wave_sync();
         };
@@ -1075,7 +1166,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void s
             }
             if (l == 0) w.ci[0] = u;
             wave_sync();
-            dists(1);
+            dists(1, qb.mode | kInitCtx);
             uint32_t du = uni(w.ck[0]);
             cnt_d += 1;
             for (int level = top; level > 0; --level) {
@@ -1089,7 +1180,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void s
                     const int64_t vo = l < n ? ix.up_off[v] : 0;
                     if (l < n) w.ci[l] = (uint32_t)v;
                     wave_sync();
-                    dists(n);
+                    dists(n, qb.mode | kInitCtx);
                     cnt_d += n;
                     cnt_u += n;
                     const uint32_t mk = l < n ? w.ck[l] : kKeyMax;
@@ -1121,7 +1212,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void s
             const int nep = ix.nep < 64 ? ix.nep : 64;
             if (l < nep) w.ci[l] = (uint32_t)ix.eps[l];
             wave_sync();
-            dists(nep);
+            dists(nep, qb.mode | kInitCtx);
             cnt_d += nep;
             if (lc.cap_max <= 64 * kMaxChunks) {
                 int32_t unused = -1;
@@ -1210,7 +1301,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void s
             vs.account(m);
             if (isnew) w.ci[mbcnt(nm)] = (uint32_t)v;
             wave_sync();
-            dists(m);
+            dists(m, qb.mode);
             cnt_d += m;
             if (kFast || at_use(lc.cap_max) <= 128) {
                 pred = pool_merge_reg_lds<2>(ix, w, ps, m, pred_row);
@@ -1428,6 +1519,7 @@ int kernel_vgprs(const DeviceIndex &ix, const LaunchConfig &lc) {
         case QK_FP32: return l2 ? vgprs_one<V_FP32_L2, 4>(ix, lc) : vgprs_one<V_FP32_IP, 4>(ix, lc);
         case QK_FP16: return l2 ? vgprs_one<V_FP16_L2, 4>(ix, lc) : vgprs_one<V_FP16_IP, 4>(ix, lc);
         case QK_BF16: return l2 ? vgprs_one<V_BF16_L2, 4>(ix, lc) : vgprs_one<V_BF16_IP, 4>(ix, lc);
+        case QK_SQ6: return l2 ? vgprs_one<V_SQ6_L2, 4>(ix, lc) : vgprs_one<V_SQ6_IP, 4>(ix, lc);
         default: return 0;
     }
 }
@@ -1443,6 +1535,7 @@ int max_blocks_per_cu(const DeviceIndex &ix, const LaunchConfig &lc) {
         case QK_FP32: return l2 ? occupancy_one<V_FP32_L2, 4>(ix, lc) : occupancy_one<V_FP32_IP, 4>(ix, lc);
         case QK_FP16: return l2 ? occupancy_one<V_FP16_L2, 4>(ix, lc) : occupancy_one<V_FP16_IP, 4>(ix, lc);
         case QK_BF16: return l2 ? occupancy_one<V_BF16_L2, 4>(ix, lc) : occupancy_one<V_BF16_IP, 4>(ix, lc);
+        case QK_SQ6: return l2 ? occupancy_one<V_SQ6_L2, 4>(ix, lc) : occupancy_one<V_SQ6_IP, 4>(ix, lc);
         default: return 0;
     }
 }
@@ -1478,6 +1571,7 @@ hipError_t launch_search(const DeviceIndex &ix, const QueryBatch &qb, const Laun
         case QK_FP32: return l2 ? launch_one<V_FP32_L2, 4>(ix, qb, lc, ss, stream) : launch_one<V_FP32_IP, 4>(ix, qb, lc, ss, stream);
         case QK_FP16: return l2 ? launch_one<V_FP16_L2, 4>(ix, qb, lc, ss, stream) : launch_one<V_FP16_IP, 4>(ix, qb, lc, ss, stream);
         case QK_BF16: return l2 ? launch_one<V_BF16_L2, 4>(ix, qb, lc, ss, stream) : launch_one<V_BF16_IP, 4>(ix, qb, lc, ss, stream);
+        case QK_SQ6: return l2 ? launch_one<V_SQ6_L2, 4>(ix, qb, lc, ss, stream) : launch_one<V_SQ6_IP, 4>(ix, qb, lc, ss, stream);
         default: return hipErrorInvalidValue;
     }
 }

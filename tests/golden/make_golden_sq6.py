@@ -1,0 +1,98 @@
+"""Writes the SQ6 golden vectors under tests/golden/ from the compiled
+reference (glass/quant/sq6_quant.hpp with AffinePerDimCalibrator<63>,
+glass/simd/avx512.hpp:79-104).
+
+Run in the build container only (it needs oracle/_ref/libglassref.so, built
+from /root/reference by `make -C oracle ref`):
+
+    python tests/golden/make_golden_sq6.py
+
+Outputs (inputs are regenerated from seeds; their SHA-256 is stored):
+  quant_sq6.npz   per (L2|IP, dim 128|100): per-dim calibration (mins then
+                  difs), digest of the codes, first 8 code rows, 16 x 97
+                  query->base distances of a bare computer
+  search_sq6.npz  per (graph, refine, ef, k): Search and SearchBatch ids /
+                  distances over 64 queries and the batch dist-cmp average, on
+                  the HNSW graphs (upper-layer descent), the NSG graph (fixed
+                  entry point) and the 100-d HNSW graph (rows of 78 bytes)
+"""
+
+from __future__ import annotations
+
+import hashlib
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+from oracle import refwrap as R  # noqa: E402
+from pyglass_amd import synth  # noqa: E402
+
+
+def sha(a: np.ndarray) -> str:
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+def datasets():
+    X = synth.gaussian(10000, 128, synth.BASE_SEED)
+    Q = synth.gaussian(1000, 128, synth.QUERY_SEED)
+    Xn = (X[:5000] / np.linalg.norm(X[:5000], axis=1, keepdims=True)).astype(np.float32)
+    Qn = (Q / np.linalg.norm(Q, axis=1, keepdims=True)).astype(np.float32)
+    X100 = synth.gaussian(3000, 100, synth.BASE_SEED)
+    Q100 = synth.gaussian(200, 100, synth.QUERY_SEED)
+    return {"c1_hnsw_l2": (X, Q, "L2"), "ip_hnsw": (Xn, Qn, "IP"), "nsg_l2": (X[:5000].copy(), Q, "L2"),
+            "g100_hnsw_l2": (X100, Q100, "L2")}
+
+
+def main():
+    R.set_num_threads(8)
+    ds = datasets()
+
+    out_q = {}
+    for gname, d in [("c1_hnsw_l2", 128), ("g100_hnsw_l2", 100)]:
+        X, Q, _ = ds[gname]
+        n = X.shape[0]
+        ids = np.arange(3, n - 1, (n - 4) // 97 + 1, dtype=np.int32)[:97]
+        out_q[f"{d}/ids"] = ids
+        out_q[f"{d}/data_sha"] = np.array(sha(X))
+        out_q[f"{d}/query_sha"] = np.array(sha(Q))
+        for metric in ["L2", "IP"]:
+            codes, params, dist, cs, da = R.quant_probe("SQ6", metric, X, Q[:16], ids)
+            key = f"SQ6_{metric}_{d}"
+            out_q[f"{key}/params"] = params
+            out_q[f"{key}/codes_sha"] = np.array(sha(codes))
+            out_q[f"{key}/codes_head"] = codes[:8].copy()
+            out_q[f"{key}/code_size"] = np.int32(cs)
+            out_q[f"{key}/dim_align"] = np.int32(da)
+            out_q[f"{key}/dist"] = dist
+    np.savez_compressed(os.path.join(HERE, "quant_sq6.npz"), **out_q)
+
+    out_s = {}
+    nq = 64
+    cases = [(g, "") for g in ds] + [("c1_hnsw_l2", "FP32")]
+    for gname, refine in cases:
+        XX, QQ, metric = ds[gname]
+        out_s[f"{gname}/data_sha"] = np.array(sha(XX))
+        s = R.RefSearcher(os.path.join(HERE, f"{gname}.glass"), XX, metric, "SQ6", refine)
+        for ef in [16, 40, 96]:
+            for k in [1, 10, 50]:
+                s.set_ef(ef)
+                key = f"{gname}/SQ6/{refine or 'none'}/{ef}/{k}"
+                I, D = s.batch_search(QQ[:nq], k)
+                out_s[f"{key}/batch_ids"] = I
+                out_s[f"{key}/batch_dists"] = D
+                out_s[f"{key}/batch_dist_cmps"] = np.float64(s.get_last_search_dist_cmps())
+                si, sd = zip(*[s.search(QQ[i], k) for i in range(nq)])
+                out_s[f"{key}/single_ids"] = np.stack(si)
+                out_s[f"{key}/single_dists"] = np.stack(sd)
+        del s
+        print(gname, refine or "none", flush=True)
+    np.savez_compressed(os.path.join(HERE, "search_sq6.npz"), **out_s)
+
+
+if __name__ == "__main__":
+    main()

@@ -194,3 +194,129 @@ def test_f16_search_matches_reference(sz16, dsets16, golden_dir, gname, quant, r
             sd = np.stack([s.search(QQ[i], k)[1] for i in range(nq)])
             np.testing.assert_array_equal(si, sz16[f"{key}/single_ids"], err_msg=key)
             np.testing.assert_array_equal(sd, sz16[f"{key}/single_dists"], err_msg=key)
+
+
+# SQ6 (tests/golden/make_golden_sq6.py)
+
+@pytest.fixture(scope="module")
+def qz6(golden_dir):
+    return dict(np.load(os.path.join(golden_dir, "quant_sq6.npz")))
+
+
+@pytest.fixture(scope="module")
+def sz6(golden_dir):
+    return np.load(os.path.join(golden_dir, "search_sq6.npz"))
+
+
+def _unpack6(codes: np.ndarray, da: int) -> np.ndarray:
+    """6-bit components from packed rows: 4 per 3 bytes, least significant first."""
+    b = codes.astype(np.uint32).reshape(codes.shape[0], -1, 3)
+    w = b[..., 0] | (b[..., 1] << 8) | (b[..., 2] << 16)
+    c = np.stack([(w >> (6 * s)) & 63 for s in range(4)], axis=-1)
+    return c.reshape(codes.shape[0], -1)[:, :da]
+
+
+@pytest.mark.parametrize("d", [128, 100])
+@pytest.mark.parametrize("metric", ["L2", "IP"])
+def test_sq6_quantizer_matches_reference(qz6, dsets16, metric, d):
+    X, Q, _ = dsets16["c1_hnsw_l2" if d == 128 else "g100_hnsw_l2"]
+    assert sha(X) == str(qz6[f"{d}/data_sha"])
+    key = f"SQ6_{metric}_{d}"
+    oq = O.Quant("SQ6", metric, d).fit(X)
+    assert oq.code_size == int(qz6[f"{key}/code_size"])
+    assert oq.dim_align == int(qz6[f"{key}/dim_align"])
+    np.testing.assert_array_equal(oq.params(), qz6[f"{key}/params"])
+    np.testing.assert_array_equal(oq.codes[:8], qz6[f"{key}/codes_head"])
+    assert sha(oq.codes) == str(qz6[f"{key}/codes_sha"])
+    ids = qz6[f"{d}/ids"]
+    got = np.stack([oq.distances(Q[i], ids) for i in range(16)])
+    np.testing.assert_array_equal(got, qz6[f"{key}/dist"])
+
+
+def test_sq6_codes_follow_the_quantile_rule(dsets16):
+    """Calibration and packing restated in numpy: chunk j of the flattened
+    input (items [j*n, (j+1)*n)) gives min = its top-th smallest value and
+    max its top-th largest, top = int(n * 0.01f) + 1; code =
+    round(clamp((x - min) / (max - min), 0, 1) * 63), half away from zero."""
+    X, _, _ = dsets16["g100_hnsw_l2"]
+    n, d = X.shape
+    oq = O.Quant("SQ6", "L2", d).fit(X)
+    flat = X.reshape(-1)
+    top = int(np.float32(n) * np.float32(0.01)) + 1
+    mins = np.zeros(d, np.float32)
+    difs = np.zeros(d, np.float32)
+    for j in range(d):
+        s = np.sort(flat[j * n:(j + 1) * n])
+        mins[j], difs[j] = s[top - 1], np.float32(s[n - top] - s[top - 1])
+    da = oq.dim_align
+    np.testing.assert_array_equal(oq.params()[:d], mins)
+    np.testing.assert_array_equal(oq.params()[da:da + d], difs)
+    v = np.clip((X - mins) / difs, np.float32(0), np.float32(1)) * np.float32(63)
+    want = np.floor(v).astype(np.int64) + (v - np.floor(v) >= 0.5)
+    got = _unpack6(oq.codes, da)
+    np.testing.assert_array_equal(got[:, :d], want)
+    assert (got[:, d:] == 0).all()
+
+
+@pytest.mark.parametrize("metric", ["L2", "IP"])
+def test_sq6_distance_is_the_decoded_closed_form(dsets16, metric):
+    """float32 kernel vs the float64 sum over decoded values
+    y = (c + 0.5) / 63 * dif + min; a wrong component order, shift or
+    decode offset moves it far outside the tolerance."""
+    X, Q, _ = dsets16["g100_hnsw_l2"]
+    d = X.shape[1]
+    oq = O.Quant("SQ6", metric, d).fit(X)
+    da = oq.dim_align
+    p = oq.params().astype(np.float64)
+    ids = np.arange(0, X.shape[0], 37)
+    y = (_unpack6(oq.codes[ids], da) + 0.5) / 63.0 * p[da:2 * da] + p[:da]
+    for qi in range(8):
+        x = np.zeros(da)
+        x[:d] = Q[qi]
+        want = ((x - y) ** 2).sum(1) if metric == "L2" else -(x * y).sum(1)
+        got = oq.distances(Q[qi], ids)
+        np.testing.assert_allclose(got, want, rtol=2e-5, atol=2e-4)
+
+
+def test_sq6_contexts_round_differently(dsets16):
+    """Search's expansions and SearchBatch evaluate SQ6 with different
+    roundings; the golden Search and SearchBatch distances below tell them
+    apart only if the two forms disagree somewhere."""
+    X, Q, _ = dsets16["c1_hnsw_l2"]
+    oq = O.Quant("SQ6", "L2", 128).fit(X)
+    ids = np.arange(0, 2000, 3)
+    a = np.concatenate([oq.distances(Q[i], ids, "search") for i in range(4)])
+    b = np.concatenate([oq.distances(Q[i], ids, "batch") for i in range(4)])
+    assert (a != b).sum() > 0
+    np.testing.assert_allclose(a, b, rtol=1e-5)
+
+
+CASES6 = [(g, "none") for g in ["c1_hnsw_l2", "ip_hnsw", "nsg_l2", "g100_hnsw_l2"]] + [("c1_hnsw_l2", "FP32")]
+
+
+@pytest.fixture(scope="module")
+def dsets6(dsets, dsets16):
+    return {**dsets, "g100_hnsw_l2": dsets16["g100_hnsw_l2"]}
+
+
+@pytest.mark.parametrize("gname,refine", CASES6)
+def test_sq6_search_matches_reference(sz6, dsets6, golden_dir, gname, refine):
+    """Ids, distances and dist-cmp counts of Search and SearchBatch; FP32
+    re-rank is a no-op for SQ6 (get_refine_factor 1.0)."""
+    XX, QQ, metric = dsets6[gname]
+    assert sha(XX) == str(sz6[f"{gname}/data_sha"])
+    g = O.Graph(os.path.join(golden_dir, f"{gname}.glass"))
+    s = O.Searcher(g, XX, metric, "SQ6", "" if refine == "none" else refine)
+    nq = 64
+    for ef in [16, 40, 96]:
+        for k in [1, 10, 50]:
+            key = f"{gname}/SQ6/{refine}/{ef}/{k}"
+            s.set_ef(ef)
+            I, D = s.batch_search(QQ[:nq], k)
+            np.testing.assert_array_equal(I, sz6[f"{key}/batch_ids"], err_msg=key)
+            np.testing.assert_array_equal(D, sz6[f"{key}/batch_dists"], err_msg=key)
+            assert s.get_last_search_dist_cmps() == float(sz6[f"{key}/batch_dist_cmps"]), key
+            si = np.stack([s.search(QQ[i], k)[0] for i in range(nq)])
+            sd = np.stack([s.search(QQ[i], k)[1] for i in range(nq)])
+            np.testing.assert_array_equal(si, sz6[f"{key}/single_ids"], err_msg=key)
+            np.testing.assert_array_equal(sd, sz6[f"{key}/single_dists"], err_msg=key)
