@@ -71,7 +71,7 @@ int glass_hip_build_graph(const char *index_type, const float *data, int32_t n, 
 
 /* create_searcher(graph, metric, quantizer, refine_quant): metric "L2"/"IP";
  * quantizer "SQ8U", "SQ4U", "SQ2U", "E5M2" (alias "FP8_E5M2"), "SQ8", "SQ6",
- * "FP32", "FP16", "BF16" (graph_searcher.hpp:263-300); refine_quant "", "FP32" or
+ * "SQ4", "FP32", "FP16", "BF16" (graph_searcher.hpp:263-381); refine_quant "", "FP32" or
  * "FP16" (make_refiner, graph_searcher.hpp:240-254).  The graph's contents move into the searcher
  * and `g` is left empty (python/bindings.cc:94-98). */
 int glass_hip_searcher_create(glass_hip_graph *g, const char *metric, const char *quantizer, const char *refine_quant,

@@ -155,6 +155,7 @@ int orq_quant_init(orq_quant *q, int32_t kind, int32_t metric, int32_t dim) {
         case ORQ_FP32: q->align = 16; q->nbits = 32; break;
         case ORQ_FP16: q->align = 16; q->nbits = 16; break;
         case ORQ_SQ6: q->align = 8; q->nbits = 6; break; /* sq6_quant.hpp:13-15 */
+        case ORQ_SQ4: q->align = 16; q->nbits = 4; break; /* sq4_quant.hpp:13-15 */
         case ORQ_BF16: q->align = 16; q->nbits = 16; break;
         default: return -1;
     }
@@ -215,9 +216,11 @@ int orq_quant_train(orq_quant *q, const float *data, int64_t n) {
             }
             break;
         }
-        case ORQ_SQ6: {
-            /* AffinePerDimCalibrator<63> with drop ratio 0.01 through
-             * find_minmax_perdim (sq6_quant.hpp:17-31, utils.hpp:52-70):
+        case ORQ_SQ6:
+        case ORQ_SQ4: {
+            /* AffinePerDimCalibrator<63> (SQ6) / <15> (SQ4) with drop ratio
+             * 0.01 through find_minmax_perdim (sq6_quant.hpp:17-31,
+             * sq4_quant.hpp:17-31, utils.hpp:52-70):
              * chunk j = flattened items [j*n, (j+1)*n), its top-th smallest
              * and largest values, top = int64(n * 0.01f) + 1; padding dims
              * keep min = dif = 0 (mins/difs resized to dim_align). */
@@ -288,6 +291,12 @@ void orq_encode(const orq_quant *q, const float *x, uint8_t *out) {
             for (int32_t j = 0; j < d; ++j) o[j] = orq_f32_to_f16(x[j]);
             break;
         }
+        case ORQ_SQ4: /* SQ4Quantizer::encode (sq4_quant.hpp:42-52): dim j -> nibble (j&1 ? hi : lo) of byte j/2 */
+            for (int32_t j = 0; j < d; ++j) {
+                const int32_t c = affine_code(x[j], q->mins[j], q->difs[j], 15);
+                out[j >> 1] |= (uint8_t)((j & 1) ? c << 4 : c);
+            }
+            break;
         case ORQ_SQ6: /* SQ6Quantizer::encode_component (sq6_quant.hpp:41-62): 4 components per 3 bytes */
             for (int32_t j = 0; j < d; ++j) {
                 const int32_t c = affine_code(x[j], q->mins[j], q->difs[j], 63);
@@ -447,6 +456,34 @@ double orq_distance_ctx(const orq_quant *q, const uint8_t *qbuf, const uint8_t *
             }
             float r = reduce16_glass(acc);
             return q->metric == ORQ_L2 ? (double)r : (double)(-r);
+        }
+        case ORQ_SQ4: { /* glass::L2SqrSQ4_ext / IPSQ4_ext (glass/simd/avx512.hpp:106-144) */
+            /* A scalar loop over element pairs inside FAST_BEGIN (associative
+             * math).  The optimize pragma keeps GCC from inlining it, so every
+             * context calls the same code, which GCC -Ofast compiles (x86
+             * code of the reference built with its own flags, checked against
+             * its golden distances) as one serial sum of per-pair terms, with
+             * a = c + 0.5:
+             *   L2  t = fma(-a, dif, (x - min) * 16),  p = fma(t0, t0, t1 * t1)
+             *   IP  y = fma(a, dif * (1/16), min),      p = fma(x0, y0, x1 * y1)
+             *   sum = ((p_0 + p_1) + p_2) + ...;  IP returns -sum. */
+            const float *x = (const float *)qbuf;
+            float sum = 0.0f;
+            for (int32_t i = 0; i < da; i += 2) {
+                const float a0 = (float)(code[i >> 1] & 15) + 0.5f, a1 = (float)(code[i >> 1] >> 4) + 0.5f;
+                float p;
+                if (q->metric == ORQ_L2) {
+                    const float t0 = fmaf(-a0, q->difs[i], (x[i] - q->mins[i]) * 16.0f);
+                    const float t1 = fmaf(-a1, q->difs[i + 1], (x[i + 1] - q->mins[i + 1]) * 16.0f);
+                    p = fmaf(t0, t0, t1 * t1);
+                } else {
+                    const float y0 = fmaf(a0, q->difs[i] * 0.0625f, q->mins[i]);
+                    const float y1 = fmaf(a1, q->difs[i + 1] * 0.0625f, q->mins[i + 1]);
+                    p = fmaf(x[i], y0, x[i + 1] * y1);
+                }
+                sum = i == 0 ? p : sum + p;
+            }
+            return q->metric == ORQ_L2 ? (double)sum : (double)(-sum);
         }
         case ORQ_SQ6: { /* glass::L2SqrSQ6_ext / IPSQ6_ext (glass/simd/avx512.hpp:79-104), 8 accumulators */
             const float *x = (const float *)qbuf;

@@ -16,7 +16,7 @@ extern "C" {
 #endif
 
 /* quantizer kinds (names as in glass/quant/quant.hpp:23-43) */
-enum { ORQ_FP32 = 0, ORQ_FP16 = 1, ORQ_BF16 = 2, ORQ_E5M2 = 3, ORQ_SQ8U = 4, ORQ_SQ8 = 5, ORQ_SQ6 = 6, ORQ_SQ4U = 7, ORQ_SQ2U = 11 };
+enum { ORQ_FP32 = 0, ORQ_FP16 = 1, ORQ_BF16 = 2, ORQ_E5M2 = 3, ORQ_SQ8U = 4, ORQ_SQ8 = 5, ORQ_SQ6 = 6, ORQ_SQ4U = 7, ORQ_SQ4 = 9, ORQ_SQ2U = 11 };
 enum { ORQ_L2 = 0, ORQ_IP = 1 };
 
 typedef struct {

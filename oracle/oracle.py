@@ -15,7 +15,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboracle.so")
 
-KINDS = {"FP32": 0, "FP16": 1, "BF16": 2, "E5M2": 3, "SQ8U": 4, "SQ8": 5, "SQ6": 6, "SQ4U": 7, "SQ2U": 11}
+KINDS = {"FP32": 0, "FP16": 1, "BF16": 2, "E5M2": 3, "SQ8U": 4, "SQ8": 5, "SQ6": 6, "SQ4U": 7, "SQ4": 9, "SQ2U": 11}
 METRICS = {"L2": 0, "IP": 1}
 
 
@@ -119,7 +119,7 @@ class Quant:
                 out[0] = self.s.max
             else:
                 out[0], out[1] = self.s.min, self.s.dif
-        elif self.kind in ("SQ8", "SQ6"):
+        elif self.kind in ("SQ8", "SQ6", "SQ4"):
             out[:da] = np.ctypeslib.as_array(self.s.mins, (da,))
             out[da:2 * da] = np.ctypeslib.as_array(self.s.difs, (da,))
         return out

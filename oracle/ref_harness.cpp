@@ -83,6 +83,7 @@ int quant_dispatch(const std::string &quant, const float *data, int n, int d, co
     if (quant == "FP16") return quant_probe<FP16Quantizer<M>>(data, n, d, qs, nq, ids, nids, codes_out, params_out, dist_out, cs, da);
     if (quant == "BF16") return quant_probe<BF16Quantizer<M>>(data, n, d, qs, nq, ids, nids, codes_out, params_out, dist_out, cs, da);
     if (quant == "SQ6") return quant_probe<SQ6Quantizer<M>>(data, n, d, qs, nq, ids, nids, codes_out, params_out, dist_out, cs, da);
+    if (quant == "SQ4") return quant_probe<SQ4Quantizer<M>>(data, n, d, qs, nq, ids, nids, codes_out, params_out, dist_out, cs, da);
     return -1;
 }
 

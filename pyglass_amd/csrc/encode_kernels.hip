@@ -57,6 +57,12 @@ __global__ void encode_kernel(const float *data, int64_t n, int32_t dim, int32_t
             case QK_SQ8:  // sq8_quant.hpp:43-47, AffinePerDimCalibrator<255>
                 if (b < dim) byte = affine_code(x[b], mins[b], difs[b], 255.0f) & 0xffu;
                 break;
+            case QK_SQ4: {  // sq4_quant.hpp:42-52, AffinePerDimCalibrator<15>: dim j -> nibble j&1 of byte j/2
+                const int32_t j = 2 * b;
+                if (j < dim) byte |= affine_code(x[j], mins[j], difs[j], 15.0f) & 0xfu;
+                if (j + 1 < dim) byte |= (affine_code(x[j + 1], mins[j + 1], difs[j + 1], 15.0f) & 0xfu) << 4;
+                break;
+            }
             case QK_SQ6: {
                 // SQ6Quantizer::encode_component (sq6_quant.hpp:41-62): byte r
                 // of group g holds bits [8r, 8r+8) of components 4g..4g+3,

@@ -103,6 +103,7 @@ int32_t parse_quant(const std::string &s) {
     if (s == "E5M2" || s == "FP8_E5M2") return QK_E5M2;
     if (s == "SQ8") return QK_SQ8;
     if (s == "SQ6") return QK_SQ6;
+    if (s == "SQ4") return QK_SQ4;
     if (s == "FP32") return QK_FP32;
     if (s == "FP16") return QK_FP16;
     if (s == "BF16") return QK_BF16;
@@ -119,6 +120,7 @@ void quant_layout(int32_t kind, int32_t dim, int32_t *dim_align, int32_t *code_s
         case QK_E5M2: align = 16; nbits = 8; break;
         case QK_SQ8: align = 16; nbits = 8; break;
         case QK_SQ6: align = 8; nbits = 6; break;  // sq6_quant.hpp:13-15
+        case QK_SQ4: align = 16; nbits = 4; break;  // sq4_quant.hpp:13-15
         case QK_FP32: align = 16; nbits = 32; break;
         // fp16_quant.hpp:9-15 / bf16_quant.hpp:9-15 without the AVX512FP16 /
         // AVX512BF16 opt-in macros
@@ -513,8 +515,10 @@ int train_quant(glass_hip_searcher &s, const float *data, int64_t n) {
             }
             break;
         }
-        case QK_SQ6: {
-            // AffinePerDimCalibrator<63> with drop ratio 0.01 (sq6_quant.hpp:17-31):
+        case QK_SQ6:
+        case QK_SQ4: {
+            // AffinePerDimCalibrator<63> (SQ6) / <15> (SQ4) with drop ratio 0.01
+            // (sq6_quant.hpp:17-31, sq4_quant.hpp:17-31):
             // chunk j of find_minmax_perdim gives the top-th smallest and largest
             // of items [j*n, (j+1)*n), top = int64(float(n) * 0.01f) + 1
             s.sq8_mins.assign(s.dim_align, 0.0f);
@@ -967,7 +971,7 @@ int glass_hip_searcher_set_data(glass_hip_searcher *s, const float *data, int32_
     DevBuf<float> raw;
     HIP_TRY(raw.alloc((size_t)n * dim));
     HIP_TRY(hipMemcpy(raw.p, data, sizeof(float) * (size_t)n * dim, hipMemcpyHostToDevice));
-    if (s->quant == QK_SQ8 || s->quant == QK_SQ6) {
+    if (s->quant == QK_SQ8 || s->quant == QK_SQ6 || s->quant == QK_SQ4) {
         HIP_TRY(s->d_mins.alloc(s->dim_align));
         HIP_TRY(s->d_difs.alloc(s->dim_align));
         HIP_TRY(hipMemcpy(s->d_mins.p, s->sq8_mins.data(), 4 * (size_t)s->dim_align, hipMemcpyHostToDevice));
@@ -1236,7 +1240,7 @@ int glass_hip_searcher_get_codes(const glass_hip_searcher *s, uint8_t *codes) {
 
 int glass_hip_searcher_get_calibration(const glass_hip_searcher *s, float *params, int32_t *n) {
     if (!s || !n) return fail(GLASS_HIP_EINVAL, "null argument");
-    if (s->quant == QK_SQ8 || s->quant == QK_SQ6) {
+    if (s->quant == QK_SQ8 || s->quant == QK_SQ6 || s->quant == QK_SQ4) {
         *n = 2 * s->dim_align;
         if (params) {
             std::copy(s->sq8_mins.begin(), s->sq8_mins.end(), params);

@@ -10,7 +10,7 @@ namespace glass_hip {
 
 // Quantizer kinds; the numbering follows glass/quant/quant.hpp:23 (QuantizerType).
 enum QuantKind : int32_t { QK_FP32 = 0, QK_FP16 = 1, QK_BF16 = 2, QK_E5M2 = 3, QK_SQ8U = 4, QK_SQ8 = 5,
-                           QK_SQ6 = 6, QK_SQ4U = 7, QK_SQ2U = 11, QK_NONE = -1 };
+                           QK_SQ6 = 6, QK_SQ4U = 7, QK_SQ4 = 9, QK_SQ2U = 11, QK_NONE = -1 };
 enum MetricKind : int32_t { MK_L2 = 0, MK_IP = 1 };
 
 // Search-mode semantics of one query (glass/searcher/graph_searcher.hpp):
@@ -32,7 +32,7 @@ struct DeviceIndex {
     int32_t quant, metric, dim, dim_align, code_size;
     const uint8_t *codes;    // N x code_size
     float qmin, qdif, qmax;  // affine / symmetric calibration
-    const float *sq8_mins;   // dim_align (SQ8 / SQ6 per-dim)
+    const float *sq8_mins;   // dim_align (SQ8 / SQ6 / SQ4 per-dim)
     const float *sq8_difs;
     // re-rank rows (Refiner<FP32Quantizer> or Refiner<FP16Quantizer>), or
     // null: N x r_dim_align fp32 (rerank_kind QK_FP32) or fp16 (QK_FP16)

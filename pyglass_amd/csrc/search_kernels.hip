@@ -83,6 +83,11 @@ __device__ __forceinline__ int32_t dpp_down4(int32_t v) { return __builtin_amdgc
 __device__ __forceinline__ int32_t dpp_down8(int32_t v) { return __builtin_amdgcn_mov_dpp(v, 0x128, 0xf, 0xf, false); }
 __device__ __forceinline__ float dpp_xor1(float v) { return __int_as_float(dpp_xor1(__float_as_int(v))); }
 __device__ __forceinline__ float dpp_xor2(float v) { return __int_as_float(dpp_xor2(__float_as_int(v))); }
+// lane Q of each group of four, broadcast to the group (quad_perm [Q,Q,Q,Q])
+template <int Q>
+__device__ __forceinline__ float dpp_quad_bcast(float v) {
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x55 * Q, 0xf, 0xf, false));
+}
 
 __device__ __forceinline__ int mbcnt(uint64_t m) {
     return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
@@ -130,6 +135,8 @@ enum Variant : int {
     V_BF16_IP = 19,
     V_SQ6_L2 = 20,   // glass::L2SqrSQ6_ext / IPSQ6_ext (glass/simd/avx512.hpp:79-104)
     V_SQ6_IP = 21,
+    V_SQ4_L2 = 22,   // glass::L2SqrSQ4_ext / IPSQ4_ext (glass/simd/avx512.hpp:106-144)
+    V_SQ4_IP = 23,
 };
 
 // Or-ed into the search mode for initialize_search's evaluations: the
@@ -166,7 +173,7 @@ struct WaveLds {
     uint32_t *ck;   // candidate keys [64]
     uint32_t *ci;   // candidate ids [64]
     float *qf;      // query, zero padded [qf_len]
-    float *qm;      // SQ8 helpers: 256x-256min, 256min, dif; SQ6: x-min, min, dif [3 x qf_len]
+    float *qm;      // SQ8 helpers: 256x-256min, 256min, dif; SQ6: x-min, min, dif; SQ4: 16(x-min), min, dif [3 x qf_len]
     uint8_t *qcode; // encoded query [code_size rounded to 16]
 };
 
@@ -183,7 +190,7 @@ __host__ __device__ inline size_t wave_lds_bytes(const DeviceIndex &ix, int32_t 
     b += (size_t)round16((int64_t)cap_max * 4) * 2;
     b += 64 * 4 * 2;
     b += (size_t)qf_len(ix) * 4;
-    if (ix.quant == QK_SQ8 || ix.quant == QK_SQ6) b += (size_t)qf_len(ix) * 4 * 3;
+    if (ix.quant == QK_SQ8 || ix.quant == QK_SQ6 || ix.quant == QK_SQ4) b += (size_t)qf_len(ix) * 4 * 3;
     b += (size_t)round16(ix.code_size);
     return b;
 }
@@ -205,7 +212,7 @@ __device__ inline WaveLds carve(uint8_t *base, const DeviceIndex &ix, int32_t ca
     w.qf = (float *)(base + off);
     off += (size_t)ql * 4;
     w.qm = (float *)(base + off);
-    if (ix.quant == QK_SQ8 || ix.quant == QK_SQ6) off += (size_t)ql * 4 * 3;
+    if (ix.quant == QK_SQ8 || ix.quant == QK_SQ6 || ix.quant == QK_SQ4) off += (size_t)ql * 4 * 3;
     w.qcode = base + off;
     return w;
 }
@@ -561,6 +568,53 @@ __device__ __forceinline__ float sq6_row4(const DeviceIndex &ix, const WaveLds &
     return acc0 + acc1;
 }
 
+// SQ4 codes (SQ4Quantizer::encode, glass/quant/sq4_quant.hpp:42-52): dim j in
+// the low (even j) or high (odd j) nibble of byte j/2.  The reference's
+// distance is a scalar loop over element pairs that GCC -Ofast (associative
+// math, never inlined) compiles to one serial sum of per-pair terms, with
+// a = c + 0.5:
+//   L2  t = fma(-a, dif, 16(x - min)),  p = fma(t0, t0, t1 * t1)
+//   IP  y = fma(a, dif / 16, min),       p = fma(x0, y0, x1 * y1), negated at the end
+//   sum = ((p_0 + p_1) + p_2) + ...
+// Lane q of a 4-lane group computes pair 4t + q (byte q of dword t of the
+// row); the group then adds the four pairs of each dword in order, every
+// lane holding the running sum.  Starting from -0 the first add returns p_0
+// exactly, as the reference's first term does.
+template <int V>
+__device__ __forceinline__ float sq4_row4(const DeviceIndex &ix, const WaveLds &w, const uint8_t *row, int q,
+                                          int32_t da) {
+    const int ql = qf_len(ix);
+    const float *xm = w.qm, *mi = w.qm + ql, *dif = w.qm + 2 * ql;
+    float sum = -0.0f;
+    // da is a multiple of 16, so a row is a whole number of 8-byte words
+#pragma unroll 2
+    for (int t = 0; t < (da >> 3); t += 2) {
+        const uint2 c2 = *(const uint2 *)(row + 4 * t);
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const uint32_t byte = ((h ? c2.y : c2.x) >> (8 * q)) & 0xffu;
+            const int j = 8 * (t + h) + 2 * q;
+            const float a0 = (float)(byte & 15u) + 0.5f, a1 = (float)(byte >> 4) + 0.5f;
+            const float2 dv = *(const float2 *)(dif + j);
+            float p;
+            if (V == V_SQ4_L2) {
+                const float2 xv = *(const float2 *)(xm + j);
+                const float t0 = fmaf(-a0, dv.x, xv.x), t1 = fmaf(-a1, dv.y, xv.y);
+                p = fmaf(t0, t0, t1 * t1);
+            } else {
+                const float2 xv = *(const float2 *)(w.qf + j), mv = *(const float2 *)(mi + j);
+                const float y0 = fmaf(a0, dv.x * 0.0625f, mv.x), y1 = fmaf(a1, dv.y * 0.0625f, mv.y);
+                p = fmaf(xv.x, y0, xv.y * y1);
+            }
+            sum = sum + dpp_quad_bcast<0>(p);
+            sum = sum + dpp_quad_bcast<1>(p);
+            sum = sum + dpp_quad_bcast<2>(p);
+            sum = sum + dpp_quad_bcast<3>(p);
+        }
+    }
+    return sum;
+}
+
 // Exact distances (Refiner, refiner.hpp:92-110) of pool entries [0, m),
 // written over their keys in pk, from fp32 rows (VF = V_FP32_*) or fp16 rows
 // (VF = V_FP16_*).  Rows of S * 16 values (S <= 8, a compile-time count, so
@@ -674,11 +728,13 @@ __device__ void float_distances(const DeviceIndex &ix, const WaveLds &w, int m, 
         float r;
         if (V == V_SQ6_L2 || V == V_SQ6_IP) {
             r = sq6_row4<V>(ix, w, row, p, mode, ix.dim_align);
+        } else if (V == V_SQ4_L2 || V == V_SQ4_IP) {
+            r = sq4_row4<V>(ix, w, row, p, ix.dim_align);
         } else {
             r = float_row4<V>(ix, w, row, p, mode, ix.dim_align);
         }
         bool neg = (V == V_E5M2_IP || V == V_SQ8_IP || V == V_FP32_IP || V == V_FP16_IP || V == V_BF16_IP ||
-                    V == V_SQ6_IP);
+                    V == V_SQ6_IP || V == V_SQ4_IP);
         if (j < m && p == 0) w.ck[j] = key_of_float(neg ? -r : r);
     }
 }
@@ -1123,11 +1179,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void s
                 w.qm[ql + j] = mn * 256.0f;
                 w.qm[2 * ql + j] = df;
             }
-        } else if (V == V_SQ6_L2 || V == V_SQ6_IP) {
+        } else if (V == V_SQ6_L2 || V == V_SQ6_IP || V == V_SQ4_L2 || V == V_SQ4_IP) {
+            constexpr float scale = (V == V_SQ4_L2 || V == V_SQ4_IP) ? 16.0f : 1.0f;
             for (int j = l; j < ql; j += 64) {
                 float mn = j < ix.dim_align ? ix.sq8_mins[j] : 0.0f;
                 float df = j < ix.dim_align ? ix.sq8_difs[j] : 0.0f;
-                w.qm[j] = w.qf[j] - mn;
+                w.qm[j] = (w.qf[j] - mn) * scale;
                 w.qm[ql + j] = mn;
                 w.qm[2 * ql + j] = df;
             }
@@ -1520,6 +1577,7 @@ int kernel_vgprs(const DeviceIndex &ix, const LaunchConfig &lc) {
         case QK_FP16: return l2 ? vgprs_one<V_FP16_L2, 4>(ix, lc) : vgprs_one<V_FP16_IP, 4>(ix, lc);
         case QK_BF16: return l2 ? vgprs_one<V_BF16_L2, 4>(ix, lc) : vgprs_one<V_BF16_IP, 4>(ix, lc);
         case QK_SQ6: return l2 ? vgprs_one<V_SQ6_L2, 4>(ix, lc) : vgprs_one<V_SQ6_IP, 4>(ix, lc);
+        case QK_SQ4: return l2 ? vgprs_one<V_SQ4_L2, 4>(ix, lc) : vgprs_one<V_SQ4_IP, 4>(ix, lc);
         default: return 0;
     }
 }
@@ -1536,6 +1594,7 @@ int max_blocks_per_cu(const DeviceIndex &ix, const LaunchConfig &lc) {
         case QK_FP16: return l2 ? occupancy_one<V_FP16_L2, 4>(ix, lc) : occupancy_one<V_FP16_IP, 4>(ix, lc);
         case QK_BF16: return l2 ? occupancy_one<V_BF16_L2, 4>(ix, lc) : occupancy_one<V_BF16_IP, 4>(ix, lc);
         case QK_SQ6: return l2 ? occupancy_one<V_SQ6_L2, 4>(ix, lc) : occupancy_one<V_SQ6_IP, 4>(ix, lc);
+        case QK_SQ4: return l2 ? occupancy_one<V_SQ4_L2, 4>(ix, lc) : occupancy_one<V_SQ4_IP, 4>(ix, lc);
         default: return 0;
     }
 }
@@ -1572,6 +1631,7 @@ hipError_t launch_search(const DeviceIndex &ix, const QueryBatch &qb, const Laun
         case QK_FP16: return l2 ? launch_one<V_FP16_L2, 4>(ix, qb, lc, ss, stream) : launch_one<V_FP16_IP, 4>(ix, qb, lc, ss, stream);
         case QK_BF16: return l2 ? launch_one<V_BF16_L2, 4>(ix, qb, lc, ss, stream) : launch_one<V_BF16_IP, 4>(ix, qb, lc, ss, stream);
         case QK_SQ6: return l2 ? launch_one<V_SQ6_L2, 4>(ix, qb, lc, ss, stream) : launch_one<V_SQ6_IP, 4>(ix, qb, lc, ss, stream);
+        case QK_SQ4: return l2 ? launch_one<V_SQ4_L2, 4>(ix, qb, lc, ss, stream) : launch_one<V_SQ4_IP, 4>(ix, qb, lc, ss, stream);
         default: return hipErrorInvalidValue;
     }
 }

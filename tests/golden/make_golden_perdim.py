@@ -1,17 +1,20 @@
-"""Writes the SQ6 golden vectors under tests/golden/ from the compiled
-reference (glass/quant/sq6_quant.hpp with AffinePerDimCalibrator<63>,
-glass/simd/avx512.hpp:79-104).
+"""Writes the golden vectors of the per-dim scalar quantizers SQ6
+(glass/quant/sq6_quant.hpp, AffinePerDimCalibrator<63>,
+glass/simd/avx512.hpp:79-104) and SQ4 (glass/quant/sq4_quant.hpp,
+AffinePerDimCalibrator<15>, glass/simd/avx512.hpp:106-144) under tests/golden/
+from the compiled reference.
 
 Run in the build container only (it needs oracle/_ref/libglassref.so, built
 from /root/reference by `make -C oracle ref`):
 
-    python tests/golden/make_golden_sq6.py
+    python tests/golden/make_golden_perdim.py
 
-Outputs (inputs are regenerated from seeds; their SHA-256 is stored):
-  quant_sq6.npz   per (L2|IP, dim 128|100): per-dim calibration (mins then
+Outputs per quantizer q in (sq6, sq4); inputs are regenerated from seeds and
+their SHA-256 is stored:
+  quant_<q>.npz   per (L2|IP, dim 128|100): per-dim calibration (mins then
                   difs), digest of the codes, first 8 code rows, 16 x 97
                   query->base distances of a bare computer
-  search_sq6.npz  per (graph, refine, ef, k): Search and SearchBatch ids /
+  search_<q>.npz  per (graph, refine, ef, k): Search and SearchBatch ids /
                   distances over 64 queries and the batch dist-cmp average, on
                   the HNSW graphs (upper-layer descent), the NSG graph (fixed
                   entry point) and the 100-d HNSW graph (rows of 78 bytes)
@@ -48,10 +51,7 @@ def datasets():
             "g100_hnsw_l2": (X100, Q100, "L2")}
 
 
-def main():
-    R.set_num_threads(8)
-    ds = datasets()
-
+def write(quant: str, ds):
     out_q = {}
     for gname, d in [("c1_hnsw_l2", 128), ("g100_hnsw_l2", 100)]:
         X, Q, _ = ds[gname]
@@ -61,15 +61,15 @@ def main():
         out_q[f"{d}/data_sha"] = np.array(sha(X))
         out_q[f"{d}/query_sha"] = np.array(sha(Q))
         for metric in ["L2", "IP"]:
-            codes, params, dist, cs, da = R.quant_probe("SQ6", metric, X, Q[:16], ids)
-            key = f"SQ6_{metric}_{d}"
+            codes, params, dist, cs, da = R.quant_probe(quant, metric, X, Q[:16], ids)
+            key = f"{quant}_{metric}_{d}"
             out_q[f"{key}/params"] = params
             out_q[f"{key}/codes_sha"] = np.array(sha(codes))
             out_q[f"{key}/codes_head"] = codes[:8].copy()
             out_q[f"{key}/code_size"] = np.int32(cs)
             out_q[f"{key}/dim_align"] = np.int32(da)
             out_q[f"{key}/dist"] = dist
-    np.savez_compressed(os.path.join(HERE, "quant_sq6.npz"), **out_q)
+    np.savez_compressed(os.path.join(HERE, f"quant_{quant.lower()}.npz"), **out_q)
 
     out_s = {}
     nq = 64
@@ -77,11 +77,11 @@ def main():
     for gname, refine in cases:
         XX, QQ, metric = ds[gname]
         out_s[f"{gname}/data_sha"] = np.array(sha(XX))
-        s = R.RefSearcher(os.path.join(HERE, f"{gname}.glass"), XX, metric, "SQ6", refine)
+        s = R.RefSearcher(os.path.join(HERE, f"{gname}.glass"), XX, metric, quant, refine)
         for ef in [16, 40, 96]:
             for k in [1, 10, 50]:
                 s.set_ef(ef)
-                key = f"{gname}/SQ6/{refine or 'none'}/{ef}/{k}"
+                key = f"{gname}/{quant}/{refine or 'none'}/{ef}/{k}"
                 I, D = s.batch_search(QQ[:nq], k)
                 out_s[f"{key}/batch_ids"] = I
                 out_s[f"{key}/batch_dists"] = D
@@ -90,8 +90,15 @@ def main():
                 out_s[f"{key}/single_ids"] = np.stack(si)
                 out_s[f"{key}/single_dists"] = np.stack(sd)
         del s
-        print(gname, refine or "none", flush=True)
-    np.savez_compressed(os.path.join(HERE, "search_sq6.npz"), **out_s)
+        print(quant, gname, refine or "none", flush=True)
+    np.savez_compressed(os.path.join(HERE, f"search_{quant.lower()}.npz"), **out_s)
+
+
+def main():
+    R.set_num_threads(8)
+    ds = datasets()
+    for quant in ["SQ6", "SQ4"]:
+        write(quant, ds)
 
 
 if __name__ == "__main__":

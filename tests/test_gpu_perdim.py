@@ -1,11 +1,11 @@
-"""SQ6 quantizer: device calibration and codes, HIP search vs the oracle
-through the C ABI, and HIP ids vs the compiled reference's stored ids
-(tests/golden/search_sq6.npz).
+"""Per-dim scalar quantizers SQ6 and SQ4: device calibration and codes, HIP
+search vs the oracle through the C ABI, and HIP ids vs the compiled
+reference's stored ids (tests/golden/search_sq6.npz, search_sq4.npz).
 
 Bar: calibration, codes, ids, distances and dist_cmps bit-exact.  The device
-kernel follows the reference's 8-accumulator order and its per-context
-rounding of the decode (Search's expansions vs SearchBatch and the
-initializer), so distances are equal, not merely close.
+kernels follow the reference's summation orders (SQ6: 8 accumulators and the
+per-context rounding of the decode; SQ4: one serial sum over element pairs),
+so distances are equal, not merely close.
 """
 
 import os
@@ -18,6 +18,8 @@ from oracle import oracle as O
 from pyglass_amd import synth
 
 pytestmark = pytest.mark.gpu
+
+PERDIM = ["SQ6", "SQ4"]
 
 
 def _datasets():
@@ -42,32 +44,34 @@ def need_gpu():
         pytest.skip("no HIP device")
 
 
+@pytest.mark.parametrize("quant", PERDIM)
 @pytest.mark.parametrize("gname", ["c1_hnsw_l2", "g100_hnsw_l2"])
 @pytest.mark.parametrize("metric", ["L2", "IP"])
-def test_device_codes_match_oracle(dsets, golden_dir, metric, gname):
-    # 128-d rows are 96 bytes (dword aligned), 100-d rows 78 bytes (2-byte aligned)
+def test_device_codes_match_oracle(dsets, golden_dir, metric, gname, quant):
+    # SQ6: 128-d rows are 96 bytes (dword aligned), 100-d rows 78 bytes (2-byte aligned)
     X, _, _ = dsets[gname]
-    print(f"codes SQ6 {metric} {gname}", flush=True)
-    s = pg.Searcher(pg.Graph(os.path.join(golden_dir, f"{gname}.glass")), X, metric, "SQ6")
-    oq = O.Quant("SQ6", metric, X.shape[1]).fit(X)
+    print(f"codes {quant} {metric} {gname}", flush=True)
+    s = pg.Searcher(pg.Graph(os.path.join(golden_dir, f"{gname}.glass")), X, metric, quant)
+    oq = O.Quant(quant, metric, X.shape[1]).fit(X)
     da = oq.dim_align
     np.testing.assert_array_equal(s.calibration()[:2 * da], oq.params()[:2 * da])
     np.testing.assert_array_equal(s.codes(), oq.codes)
 
 
-CASES = [(g, "") for g in ["c1_hnsw_l2", "ip_hnsw", "nsg_l2", "g100_hnsw_l2"]] + [("c1_hnsw_l2", "FP32")]
+CASES = [(q, g, "") for q in PERDIM for g in ["c1_hnsw_l2", "ip_hnsw", "nsg_l2", "g100_hnsw_l2"]]
+CASES += [(q, "c1_hnsw_l2", "FP32") for q in PERDIM]
 
 
-@pytest.mark.parametrize("gname,refine", CASES)
-def test_search_matches_oracle(dsets, golden_dir, gname, refine):
+@pytest.mark.parametrize("quant,gname,refine", CASES)
+def test_search_matches_oracle(dsets, golden_dir, quant, gname, refine):
     XX, QQ, metric = dsets[gname]
     path = os.path.join(golden_dir, f"{gname}.glass")
-    s = pg.Searcher(pg.Graph(path), XX, metric, "SQ6", refine)
-    o = O.Searcher(O.Graph(path), XX, metric, "SQ6", refine)
+    s = pg.Searcher(pg.Graph(path), XX, metric, quant, refine)
+    o = O.Searcher(O.Graph(path), XX, metric, quant, refine)
     nq = 200
     for ef in [16, 40, 96, 200]:
         for k in [1, 10, 50]:
-            print(f"{gname} SQ6 {refine or 'none'} ef={ef} k={k}", flush=True)
+            print(f"{gname} {quant} {refine or 'none'} ef={ef} k={k}", flush=True)
             s.set_ef(ef)
             o.set_ef(ef)
             I, D = s.batch_search(QQ[:nq], k)
@@ -82,15 +86,16 @@ def test_search_matches_oracle(dsets, golden_dir, gname, refine):
                 np.testing.assert_array_equal(b, d)
 
 
+@pytest.mark.parametrize("quant", PERDIM)
 @pytest.mark.parametrize("gname", ["c1_hnsw_l2", "ip_hnsw", "nsg_l2", "g100_hnsw_l2"])
-def test_reference_ids_directly(dsets, golden_dir, gname):
+def test_reference_ids_directly(dsets, golden_dir, gname, quant):
     """HIP results equal the compiled reference's stored ones (not only the oracle's)."""
-    z = np.load(os.path.join(golden_dir, "search_sq6.npz"))
+    z = np.load(os.path.join(golden_dir, f"search_{quant.lower()}.npz"))
     XX, QQ, metric = dsets[gname]
-    s = pg.Searcher(pg.Graph(os.path.join(golden_dir, f"{gname}.glass")), XX, metric, "SQ6")
+    s = pg.Searcher(pg.Graph(os.path.join(golden_dir, f"{gname}.glass")), XX, metric, quant)
     for ef in [16, 40, 96]:
         for k in [1, 10, 50]:
-            key = f"{gname}/SQ6/none/{ef}/{k}"
+            key = f"{gname}/{quant}/none/{ef}/{k}"
             print(key, flush=True)
             s.set_ef(ef)
             I, D = s.batch_search(QQ[:64], k)

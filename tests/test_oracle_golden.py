@@ -196,51 +196,63 @@ def test_f16_search_matches_reference(sz16, dsets16, golden_dir, gname, quant, r
             np.testing.assert_array_equal(sd, sz16[f"{key}/single_dists"], err_msg=key)
 
 
-# SQ6 (tests/golden/make_golden_sq6.py)
+# Per-dim scalar quantizers SQ6 and SQ4 (tests/golden/make_golden_perdim.py)
 
-@pytest.fixture(scope="module")
-def qz6(golden_dir):
-    return dict(np.load(os.path.join(golden_dir, "quant_sq6.npz")))
+PERDIM = ["SQ6", "SQ4"]
 
 
 @pytest.fixture(scope="module")
-def sz6(golden_dir):
-    return np.load(os.path.join(golden_dir, "search_sq6.npz"))
+def qzp(golden_dir):
+    return {q: dict(np.load(os.path.join(golden_dir, f"quant_{q.lower()}.npz"))) for q in PERDIM}
 
 
-def _unpack6(codes: np.ndarray, da: int) -> np.ndarray:
-    """6-bit components from packed rows: 4 per 3 bytes, least significant first."""
-    b = codes.astype(np.uint32).reshape(codes.shape[0], -1, 3)
-    w = b[..., 0] | (b[..., 1] << 8) | (b[..., 2] << 16)
-    c = np.stack([(w >> (6 * s)) & 63 for s in range(4)], axis=-1)
+@pytest.fixture(scope="module")
+def szp(golden_dir):
+    return {q: np.load(os.path.join(golden_dir, f"search_{q.lower()}.npz")) for q in PERDIM}
+
+
+def _unpack(quant: str, codes: np.ndarray, da: int) -> np.ndarray:
+    """Components of packed rows.  SQ6: 4 per 3 bytes, six bits each from the
+    least significant end; SQ4: dim j in the low (even j) or high nibble of
+    byte j/2."""
+    if quant == "SQ4":
+        b = codes.astype(np.uint32)
+        c = np.stack([b & 15, b >> 4], axis=-1)
+    else:
+        b = codes.astype(np.uint32).reshape(codes.shape[0], -1, 3)
+        w = b[..., 0] | (b[..., 1] << 8) | (b[..., 2] << 16)
+        c = np.stack([(w >> (6 * s)) & 63 for s in range(4)], axis=-1)
     return c.reshape(codes.shape[0], -1)[:, :da]
 
 
+@pytest.mark.parametrize("quant", PERDIM)
 @pytest.mark.parametrize("d", [128, 100])
 @pytest.mark.parametrize("metric", ["L2", "IP"])
-def test_sq6_quantizer_matches_reference(qz6, dsets16, metric, d):
+def test_perdim_quantizer_matches_reference(qzp, dsets16, quant, metric, d):
+    z = qzp[quant]
     X, Q, _ = dsets16["c1_hnsw_l2" if d == 128 else "g100_hnsw_l2"]
-    assert sha(X) == str(qz6[f"{d}/data_sha"])
-    key = f"SQ6_{metric}_{d}"
-    oq = O.Quant("SQ6", metric, d).fit(X)
-    assert oq.code_size == int(qz6[f"{key}/code_size"])
-    assert oq.dim_align == int(qz6[f"{key}/dim_align"])
-    np.testing.assert_array_equal(oq.params(), qz6[f"{key}/params"])
-    np.testing.assert_array_equal(oq.codes[:8], qz6[f"{key}/codes_head"])
-    assert sha(oq.codes) == str(qz6[f"{key}/codes_sha"])
-    ids = qz6[f"{d}/ids"]
+    assert sha(X) == str(z[f"{d}/data_sha"])
+    key = f"{quant}_{metric}_{d}"
+    oq = O.Quant(quant, metric, d).fit(X)
+    assert oq.code_size == int(z[f"{key}/code_size"])
+    assert oq.dim_align == int(z[f"{key}/dim_align"])
+    np.testing.assert_array_equal(oq.params(), z[f"{key}/params"])
+    np.testing.assert_array_equal(oq.codes[:8], z[f"{key}/codes_head"])
+    assert sha(oq.codes) == str(z[f"{key}/codes_sha"])
+    ids = z[f"{d}/ids"]
     got = np.stack([oq.distances(Q[i], ids) for i in range(16)])
-    np.testing.assert_array_equal(got, qz6[f"{key}/dist"])
+    np.testing.assert_array_equal(got, z[f"{key}/dist"])
 
 
-def test_sq6_codes_follow_the_quantile_rule(dsets16):
+@pytest.mark.parametrize("quant,levels", [("SQ6", 63), ("SQ4", 15)])
+def test_perdim_codes_follow_the_quantile_rule(dsets16, quant, levels):
     """Calibration and packing restated in numpy: chunk j of the flattened
     input (items [j*n, (j+1)*n)) gives min = its top-th smallest value and
     max its top-th largest, top = int(n * 0.01f) + 1; code =
-    round(clamp((x - min) / (max - min), 0, 1) * 63), half away from zero."""
+    round(clamp((x - min) / (max - min), 0, 1) * levels), half away from zero."""
     X, _, _ = dsets16["g100_hnsw_l2"]
     n, d = X.shape
-    oq = O.Quant("SQ6", "L2", d).fit(X)
+    oq = O.Quant(quant, "L2", d).fit(X)
     flat = X.reshape(-1)
     top = int(np.float32(n) * np.float32(0.01)) + 1
     mins = np.zeros(d, np.float32)
@@ -251,31 +263,35 @@ def test_sq6_codes_follow_the_quantile_rule(dsets16):
     da = oq.dim_align
     np.testing.assert_array_equal(oq.params()[:d], mins)
     np.testing.assert_array_equal(oq.params()[da:da + d], difs)
-    v = np.clip((X - mins) / difs, np.float32(0), np.float32(1)) * np.float32(63)
+    v = np.clip((X - mins) / difs, np.float32(0), np.float32(1)) * np.float32(levels)
     want = np.floor(v).astype(np.int64) + (v - np.floor(v) >= 0.5)
-    got = _unpack6(oq.codes, da)
+    got = _unpack(quant, oq.codes, da)
     np.testing.assert_array_equal(got[:, :d], want)
     assert (got[:, d:] == 0).all()
 
 
+@pytest.mark.parametrize("quant", PERDIM)
 @pytest.mark.parametrize("metric", ["L2", "IP"])
-def test_sq6_distance_is_the_decoded_closed_form(dsets16, metric):
-    """float32 kernel vs the float64 sum over decoded values
-    y = (c + 0.5) / 63 * dif + min; a wrong component order, shift or
-    decode offset moves it far outside the tolerance."""
+def test_perdim_distance_is_the_decoded_closed_form(dsets16, quant, metric):
+    """float32 kernel vs the float64 sum over decoded values.  SQ6 decodes
+    y = (c + 0.5) / 63 * dif + min.  SQ4 decodes with (c + 0.5) / 16 although
+    it encodes with 15 levels, and its L2 is that of 16x against 16y (a factor
+    256).  A wrong component order, shift or decode offset moves the value far
+    outside the tolerance."""
     X, Q, _ = dsets16["g100_hnsw_l2"]
     d = X.shape[1]
-    oq = O.Quant("SQ6", metric, d).fit(X)
+    oq = O.Quant(quant, metric, d).fit(X)
     da = oq.dim_align
     p = oq.params().astype(np.float64)
     ids = np.arange(0, X.shape[0], 37)
-    y = (_unpack6(oq.codes[ids], da) + 0.5) / 63.0 * p[da:2 * da] + p[:da]
+    div, l2s = (63.0, 1.0) if quant == "SQ6" else (16.0, 256.0)
+    y = (_unpack(quant, oq.codes[ids], da) + 0.5) / div * p[da:2 * da] + p[:da]
     for qi in range(8):
         x = np.zeros(da)
         x[:d] = Q[qi]
-        want = ((x - y) ** 2).sum(1) if metric == "L2" else -(x * y).sum(1)
+        want = l2s * ((x - y) ** 2).sum(1) if metric == "L2" else -(x * y).sum(1)
         got = oq.distances(Q[qi], ids)
-        np.testing.assert_allclose(got, want, rtol=2e-5, atol=2e-4)
+        np.testing.assert_allclose(got, want, rtol=2e-5, atol=2e-4 * l2s)
 
 
 def test_sq6_contexts_round_differently(dsets16):
@@ -291,32 +307,34 @@ def test_sq6_contexts_round_differently(dsets16):
     np.testing.assert_allclose(a, b, rtol=1e-5)
 
 
-CASES6 = [(g, "none") for g in ["c1_hnsw_l2", "ip_hnsw", "nsg_l2", "g100_hnsw_l2"]] + [("c1_hnsw_l2", "FP32")]
+CASESP = [(q, g, "none") for q in PERDIM for g in ["c1_hnsw_l2", "ip_hnsw", "nsg_l2", "g100_hnsw_l2"]]
+CASESP += [(q, "c1_hnsw_l2", "FP32") for q in PERDIM]
 
 
 @pytest.fixture(scope="module")
-def dsets6(dsets, dsets16):
+def dsetsp(dsets, dsets16):
     return {**dsets, "g100_hnsw_l2": dsets16["g100_hnsw_l2"]}
 
 
-@pytest.mark.parametrize("gname,refine", CASES6)
-def test_sq6_search_matches_reference(sz6, dsets6, golden_dir, gname, refine):
-    """Ids, distances and dist-cmp counts of Search and SearchBatch; FP32
-    re-rank is a no-op for SQ6 (get_refine_factor 1.0)."""
-    XX, QQ, metric = dsets6[gname]
-    assert sha(XX) == str(sz6[f"{gname}/data_sha"])
+@pytest.mark.parametrize("quant,gname,refine", CASESP)
+def test_perdim_search_matches_reference(szp, dsetsp, golden_dir, quant, gname, refine):
+    """Ids, distances and dist-cmp counts of Search and SearchBatch; an FP32
+    re-rank is a no-op for SQ6 and SQ4 (get_refine_factor 1.0)."""
+    z = szp[quant]
+    XX, QQ, metric = dsetsp[gname]
+    assert sha(XX) == str(z[f"{gname}/data_sha"])
     g = O.Graph(os.path.join(golden_dir, f"{gname}.glass"))
-    s = O.Searcher(g, XX, metric, "SQ6", "" if refine == "none" else refine)
+    s = O.Searcher(g, XX, metric, quant, "" if refine == "none" else refine)
     nq = 64
     for ef in [16, 40, 96]:
         for k in [1, 10, 50]:
-            key = f"{gname}/SQ6/{refine}/{ef}/{k}"
+            key = f"{gname}/{quant}/{refine}/{ef}/{k}"
             s.set_ef(ef)
             I, D = s.batch_search(QQ[:nq], k)
-            np.testing.assert_array_equal(I, sz6[f"{key}/batch_ids"], err_msg=key)
-            np.testing.assert_array_equal(D, sz6[f"{key}/batch_dists"], err_msg=key)
-            assert s.get_last_search_dist_cmps() == float(sz6[f"{key}/batch_dist_cmps"]), key
+            np.testing.assert_array_equal(I, z[f"{key}/batch_ids"], err_msg=key)
+            np.testing.assert_array_equal(D, z[f"{key}/batch_dists"], err_msg=key)
+            assert s.get_last_search_dist_cmps() == float(z[f"{key}/batch_dist_cmps"]), key
             si = np.stack([s.search(QQ[i], k)[0] for i in range(nq)])
             sd = np.stack([s.search(QQ[i], k)[1] for i in range(nq)])
-            np.testing.assert_array_equal(si, sz6[f"{key}/single_ids"], err_msg=key)
-            np.testing.assert_array_equal(sd, sz6[f"{key}/single_dists"], err_msg=key)
+            np.testing.assert_array_equal(si, z[f"{key}/single_ids"], err_msg=key)
+            np.testing.assert_array_equal(sd, z[f"{key}/single_dists"], err_msg=key)
