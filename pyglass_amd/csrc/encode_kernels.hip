@@ -1,4 +1,5 @@
-// Device-side SetData: encode every base row with the trained calibrator
+// Device-side SetData: the calibrators' order statistics (radix select below),
+// then encode every base row with the trained calibrator
 // (Quantizer::add, e.g. glass/quant/sq4u_quant.hpp:99-117), or convert it to
 // fp16 / bf16 (fp16_quant.hpp:27-35, bf16_quant.hpp:26-38), and lay out the
 // FP32 re-rank rows (FP32Quantizer::add, glass/quant/fp32_quant.hpp:25-31).
@@ -118,6 +119,58 @@ __global__ void fill_i32_kernel(int32_t *p, int64_t n, int32_t v) {
         p[i] = v;
 }
 
+// Order-preserving 32-bit key of a float (of |f| when absval): unsigned
+// comparison of keys is the float order, and key_value inverts it.
+__device__ __forceinline__ uint32_t order_key(float f, int32_t absval) {
+    uint32_t u = __float_as_uint(f);
+    if (absval) u &= 0x7fffffffu;
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+
+// One radix-select pass: per segment (blockIdx.y) and per target t in {0, 1},
+// a 256-bin histogram of digit (key >> shift) & 255 over the items whose key
+// agrees with prefix[t] above that digit.
+__global__ void radix_hist_kernel(const float *data, int64_t seg_len, int64_t seg_stride, int32_t absval,
+                                  int32_t shift, const uint32_t *prefix, unsigned long long *hist) {
+    __shared__ unsigned long long h[512];
+    for (int i = threadIdx.x; i < 512; i += blockDim.x) h[i] = 0;
+    __syncthreads();
+    const int64_t seg = blockIdx.y;
+    const float *p = data + seg * seg_stride;
+    const uint32_t p0 = prefix[2 * seg], p1 = prefix[2 * seg + 1];
+    const uint32_t hm = shift >= 24 ? 0u : (0xffffffffu << (shift + 8));
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < seg_len;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const uint32_t k = order_key(p[i], absval);
+        const uint32_t d = (k >> shift) & 255u;
+        if (((k ^ p0) & hm) == 0) atomicAdd(&h[d], 1ull);
+        if (((k ^ p1) & hm) == 0) atomicAdd(&h[256 + d], 1ull);
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < 512; i += blockDim.x) {
+        const unsigned long long v = h[i];
+        if (v) atomicAdd(&hist[seg * 512 + i], v);
+    }
+}
+
+// Picks, for every (segment, target), the bin holding the target's remaining
+// rank: appends the digit to the prefix and makes the rank relative to it.
+__global__ void radix_pick_kernel(int32_t ntargets, int32_t shift, const unsigned long long *hist, uint32_t *prefix,
+                                  unsigned long long *rank) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ntargets) return;
+    const unsigned long long *h = hist + (int64_t)t * 256;
+    const unsigned long long r = rank[t];
+    unsigned long long c = 0;
+    int b = 0;
+    for (; b < 255; ++b) {
+        if (c + h[b] > r) break;
+        c += h[b];
+    }
+    rank[t] = r - c;
+    prefix[t] |= (uint32_t)b << shift;
+}
+
 int grid_for(int64_t total) {
     int64_t g = (total + 255) / 256;
     if (g > 65536) g = 65536;
@@ -141,6 +194,23 @@ hipError_t launch_pad_rows(const float *data_dev, int64_t n, int32_t dim, int32_
     const int64_t total = n * (int64_t)dim_align;
     hipLaunchKernelGGL(pad_rows_kernel, dim3(grid_for(total)), dim3(256), 0, stream, data_dev, n, dim, dim_align,
                        out);
+    return hipGetLastError();
+}
+
+hipError_t launch_radix_hist(const float *data, int32_t nseg, int64_t seg_len, int64_t seg_stride, int32_t absval,
+                             int32_t shift, const uint32_t *prefix, unsigned long long *hist, hipStream_t stream) {
+    int64_t bx = (seg_len + 256 * 16 - 1) / (256 * 16);
+    if (bx > 2048) bx = 2048;
+    if (bx < 1) bx = 1;
+    hipLaunchKernelGGL(radix_hist_kernel, dim3((unsigned)bx, (unsigned)nseg), dim3(256), 0, stream, data, seg_len,
+                       seg_stride, absval, shift, prefix, hist);
+    return hipGetLastError();
+}
+
+hipError_t launch_radix_pick(int32_t ntargets, int32_t shift, const unsigned long long *hist, uint32_t *prefix,
+                             unsigned long long *rank, hipStream_t stream) {
+    hipLaunchKernelGGL(radix_pick_kernel, dim3((ntargets + 255) / 256), dim3(256), 0, stream, ntargets, shift, hist,
+                       prefix, rank);
     return hipGetLastError();
 }
 

@@ -36,6 +36,10 @@ hipError_t launch_encode(const float *data_dev, int64_t n, int32_t dim, int32_t 
 hipError_t launch_pad_rows(const float *data_dev, int64_t n, int32_t dim, int32_t dim_align, float *out,
                            hipStream_t stream);
 hipError_t launch_fill_i32(int32_t *p, int64_t n, int32_t v, hipStream_t stream);
+hipError_t launch_radix_hist(const float *data, int32_t nseg, int64_t seg_len, int64_t seg_stride, int32_t absval,
+                             int32_t shift, const uint32_t *prefix, unsigned long long *hist, hipStream_t stream);
+hipError_t launch_radix_pick(int32_t ntargets, int32_t shift, const unsigned long long *hist, uint32_t *prefix,
+                             unsigned long long *rank, hipStream_t stream);
 }  // namespace glass_hip
 
 namespace glass_build {
@@ -136,17 +140,6 @@ float refine_factor_of(int32_t kind) {
     if (kind == QK_SQ8U || kind == QK_SQ8 || kind == QK_SQ4U) return 1.5f;
     if (kind == QK_SQ2U) return 3.0f;
     return 1.0f;
-}
-
-// k-th smallest / k-th largest of v (the MaxHeap order statistic of
-// find_minmax, glass/quant/utils.hpp:31-40).
-std::pair<float, float> kth_extremes(std::vector<float> &v, int64_t top) {
-    top = std::min<int64_t>(top, (int64_t)v.size());
-    std::nth_element(v.begin(), v.begin() + (top - 1), v.end());
-    float lo = v[top - 1];
-    std::nth_element(v.begin(), v.begin() + ((int64_t)v.size() - top), v.end());
-    float hi = v[(int64_t)v.size() - top];
-    return {lo, hi};
 }
 
 }  // namespace
@@ -473,19 +466,70 @@ int load_diskann(const char *path, glass_hip_graph &g) {
     return GLASS_HIP_OK;
 }
 
-// Host calibration (train) for the search quantizer.
-int train_quant(glass_hip_searcher &s, const float *data, int64_t n) {
+// Inverse of the device's order-preserving float key.
+float key_value(uint32_t k) {
+    const uint32_t u = (k & 0x80000000u) ? (k & 0x7fffffffu) : ~k;
+    float f;
+    std::memcpy(&f, &u, 4);
+    return f;
+}
+
+// The top-th smallest and top-th largest value of each of nseg segments of
+// seg_len device floats (segment j at data + j * seg_stride), or of their
+// absolute values: the order statistic find_minmax / find_absmax keep in a
+// MaxHeap (glass/quant/utils.hpp:31-50), i.e. the elements at positions
+// top - 1 and seg_len - top of the sorted segment.  Found exactly on the
+// device by a radix select over order-preserving keys, 8 bits per pass, both
+// targets of every segment in one pass over the data.
+int device_order_stats(hipStream_t stream, const float *data, int32_t nseg, int64_t seg_len, int64_t seg_stride,
+                       bool absval, int64_t top, std::vector<float> &lo, std::vector<float> &hi) {
+    if (seg_len <= 0) return fail(GLASS_HIP_EINVAL, "too few rows to calibrate");
+    if (nseg < 1 || nseg > 65535) return fail(GLASS_HIP_EINVAL, "%d calibration segments (at most 65535)", nseg);
+    top = std::max<int64_t>(1, std::min<int64_t>(top, seg_len));
+    const int32_t nt = 2 * nseg;
+    std::vector<unsigned long long> rank(nt);
+    std::vector<uint32_t> prefix(nt, 0u);
+    for (int32_t j = 0; j < nseg; ++j) {
+        rank[2 * j] = (unsigned long long)(top - 1);
+        rank[2 * j + 1] = (unsigned long long)(seg_len - top);
+    }
+    DevBuf<unsigned long long> d_rank, d_hist;
+    DevBuf<uint32_t> d_prefix;
+    HIP_TRY(d_rank.alloc(nt));
+    HIP_TRY(d_prefix.alloc(nt));
+    HIP_TRY(d_hist.alloc((size_t)nt * 256));
+    HIP_TRY(hipMemcpyAsync(d_rank.p, rank.data(), sizeof(unsigned long long) * nt, hipMemcpyHostToDevice, stream));
+    HIP_TRY(hipMemcpyAsync(d_prefix.p, prefix.data(), 4 * (size_t)nt, hipMemcpyHostToDevice, stream));
+    for (int shift = 24; shift >= 0; shift -= 8) {
+        HIP_TRY(launch_fill_i32((int32_t *)d_hist.p, (int64_t)nt * 256 * 2, 0, stream));
+        HIP_TRY(launch_radix_hist(data, nseg, seg_len, seg_stride, absval ? 1 : 0, shift, d_prefix.p, d_hist.p,
+                                  stream));
+        HIP_TRY(launch_radix_pick(nt, shift, d_hist.p, d_prefix.p, d_rank.p, stream));
+    }
+    HIP_TRY(hipMemcpyAsync(prefix.data(), d_prefix.p, 4 * (size_t)nt, hipMemcpyDeviceToHost, stream));
+    HIP_TRY(hipStreamSynchronize(stream));
+    lo.resize(nseg);
+    hi.resize(nseg);
+    for (int32_t j = 0; j < nseg; ++j) {
+        lo[j] = key_value(prefix[2 * j]);
+        hi[j] = key_value(prefix[2 * j + 1]);
+    }
+    return GLASS_HIP_OK;
+}
+
+// Calibration (train) of the search quantizer from the rows already in HBM.
+int train_quant(glass_hip_searcher &s, const float *data_dev, int64_t n) {
     const int64_t nd = n * (int64_t)s.dim;
+    std::vector<float> lo, hi;
     switch (s.quant) {
         case QK_SQ8U:
-            if (s.metric == MK_L2) {  // AffineCalibrator<127>, drop 0 (sq8u_quant.hpp:19-28)
-                auto mm = std::minmax_element(data, data + nd);
-                s.qmin = *mm.first;
-                s.qdif = *mm.second - *mm.first;
-            } else {  // SymCalibrator<127>: absolute maximum
-                float m = 0.0f;
-                for (int64_t i = 0; i < nd; ++i) m = std::max(m, std::fabs(data[i]));
-                s.qmax = m;
+            if (s.metric == MK_L2) {  // AffineCalibrator<127>, drop 0 (sq8u_quant.hpp:19-28): min and max
+                if (int r = device_order_stats(s.stream, data_dev, 1, nd, 0, false, 1, lo, hi)) return r;
+                s.qmin = lo[0];
+                s.qdif = hi[0] - lo[0];
+            } else {  // SymCalibrator<127>: largest absolute value
+                if (int r = device_order_stats(s.stream, data_dev, 1, nd, 0, true, 1, lo, hi)) return r;
+                s.qmax = hi[0];
             }
             break;
         case QK_SQ4U:
@@ -494,42 +538,29 @@ int train_quant(glass_hip_searcher &s, const float *data, int64_t n) {
             // counts computed in float as in sq4u_quant.hpp:95-97 and utils.hpp:31-33
             const int64_t nitems = (int64_t)((float)nd * 0.1f);
             const float ratio = s.quant == QK_SQ4U ? 0.01f : 0.05f;
-            const int64_t top = (int64_t)((float)nitems * ratio) + 1;
             if (nitems <= 0) return fail(GLASS_HIP_EINVAL, "too few rows to calibrate");
-            std::vector<float> v(data, data + nitems);
-            auto mm = kth_extremes(v, top);
-            s.qmin = mm.first;
-            s.qdif = mm.second - mm.first;
+            const int64_t top = (int64_t)((float)nitems * ratio) + 1;
+            if (int r = device_order_stats(s.stream, data_dev, 1, nitems, 0, false, top, lo, hi)) return r;
+            s.qmin = lo[0];
+            s.qdif = hi[0] - lo[0];
             break;
         }
-        case QK_SQ8: {
-            // AffinePerDimCalibrator<255> through find_minmax_perdim
-            // (glass/quant/utils.hpp:52-70), which bins flattened item i into
-            // "dimension" i / n; padding dims keep min = dif = 0.
-            s.sq8_mins.assign(s.dim_align, 0.0f);
-            s.sq8_difs.assign(s.dim_align, 0.0f);
-            for (int32_t j = 0; j < s.dim; ++j) {
-                auto mm = std::minmax_element(data + (int64_t)j * n, data + (int64_t)(j + 1) * n);
-                s.sq8_mins[j] = *mm.first;
-                s.sq8_difs[j] = *mm.second - *mm.first;
-            }
-            break;
-        }
+        case QK_SQ8:
         case QK_SQ6:
         case QK_SQ4: {
-            // AffinePerDimCalibrator<63> (SQ6) / <15> (SQ4) with drop ratio 0.01
-            // (sq6_quant.hpp:17-31, sq4_quant.hpp:17-31):
-            // chunk j of find_minmax_perdim gives the top-th smallest and largest
-            // of items [j*n, (j+1)*n), top = int64(float(n) * 0.01f) + 1
+            // AffinePerDimCalibrator<255> (SQ8, drop 0), <63> (SQ6) and <15>
+            // (SQ4, both drop 0.01) through find_minmax_perdim
+            // (glass/quant/utils.hpp:52-70, sq8_quant.hpp, sq6_quant.hpp:17-31,
+            // sq4_quant.hpp:17-31), which bins flattened item i into
+            // "dimension" i / n: segment j is items [j*n, (j+1)*n), not
+            // column j.  Padding dims keep min = dif = 0.
+            const int64_t top = s.quant == QK_SQ8 ? 1 : (int64_t)((float)n * 0.01f) + 1;
+            if (int r = device_order_stats(s.stream, data_dev, s.dim, n, n, false, top, lo, hi)) return r;
             s.sq8_mins.assign(s.dim_align, 0.0f);
             s.sq8_difs.assign(s.dim_align, 0.0f);
-            const int64_t top = (int64_t)((float)n * 0.01f) + 1;
-            std::vector<float> v;
             for (int32_t j = 0; j < s.dim; ++j) {
-                v.assign(data + (int64_t)j * n, data + (int64_t)(j + 1) * n);
-                auto mm = kth_extremes(v, top);
-                s.sq8_mins[j] = mm.first;
-                s.sq8_difs[j] = mm.second - mm.first;
+                s.sq8_mins[j] = lo[j];
+                s.sq8_difs[j] = hi[j] - lo[j];
             }
             break;
         }
@@ -951,7 +982,6 @@ int glass_hip_searcher_set_data(glass_hip_searcher *s, const float *data, int32_
     quant_layout(s->quant, dim, &s->dim_align, &s->code_size);
     int32_t r_row_bytes = 0;
     if (s->refine != QK_NONE) quant_layout(s->refine, dim, &s->r_dim_align, &r_row_bytes);
-    if (int r = train_quant(*s, data, n)) return r;
     // graph
     const glass_hip_graph &g = s->graph;
     HIP_TRY(s->d_adj.alloc(g.adj.size()));
@@ -971,6 +1001,7 @@ int glass_hip_searcher_set_data(glass_hip_searcher *s, const float *data, int32_
     DevBuf<float> raw;
     HIP_TRY(raw.alloc((size_t)n * dim));
     HIP_TRY(hipMemcpy(raw.p, data, sizeof(float) * (size_t)n * dim, hipMemcpyHostToDevice));
+    if (int r = train_quant(*s, raw.p, n)) return r;
     if (s->quant == QK_SQ8 || s->quant == QK_SQ6 || s->quant == QK_SQ4) {
         HIP_TRY(s->d_mins.alloc(s->dim_align));
         HIP_TRY(s->d_difs.alloc(s->dim_align));
