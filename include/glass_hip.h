@@ -71,7 +71,8 @@ int glass_hip_build_graph(const char *index_type, const float *data, int32_t n, 
 
 /* create_searcher(graph, metric, quantizer, refine_quant): metric "L2"/"IP";
  * quantizer "SQ8U", "SQ4U", "SQ2U", "E5M2" (alias "FP8_E5M2"), "SQ8", "SQ6",
- * "SQ4", "FP32", "FP16", "BF16" (graph_searcher.hpp:263-381); refine_quant "", "FP32" or
+ * "SQ4", "PQ8" (dim divisible by 4), "FP32", "FP16", "BF16"
+ * (graph_searcher.hpp:263-381); refine_quant "", "FP32" or
  * "FP16" (make_refiner, graph_searcher.hpp:240-254).  The graph's contents move into the searcher
  * and `g` is left empty (python/bindings.cc:94-98). */
 int glass_hip_searcher_create(glass_hip_graph *g, const char *metric, const char *quantizer, const char *refine_quant,
@@ -152,8 +153,9 @@ int glass_hip_searcher_info(const glass_hip_searcher *s, int32_t *code_size, int
                             int32_t *degree, int32_t *n, float *refine_factor);
 /* Copy the device-resident codes (n*code_size bytes) back to the host. */
 int glass_hip_searcher_get_codes(const glass_hip_searcher *s, uint8_t *codes);
-/* Calibration parameters: [min, dif] (affine), [max] (symmetric) or
- * [mins[dim_align], difs[dim_align]] (SQ8 per-dim); returns count via *n. */
+/* Calibration parameters: [min, dif] (affine), [max] (symmetric),
+ * [mins[dim_align], difs[dim_align]] (SQ8 / SQ6 / SQ4 per-dim) or the PQ8
+ * codebook [dim/4][256][4]; returns count via *n. */
 int glass_hip_searcher_get_calibration(const glass_hip_searcher *s, float *params, int32_t *n);
 /* Synchronize the searcher's stream. */
 int glass_hip_searcher_sync(glass_hip_searcher *s);

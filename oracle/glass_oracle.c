@@ -23,6 +23,7 @@
  */
 #include "glass_oracle.h"
 
+#include <float.h>
 #include <math.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -157,6 +158,12 @@ int orq_quant_init(orq_quant *q, int32_t kind, int32_t metric, int32_t dim) {
         case ORQ_SQ6: q->align = 8; q->nbits = 6; break; /* sq6_quant.hpp:13-15 */
         case ORQ_SQ4: q->align = 16; q->nbits = 4; break; /* sq4_quant.hpp:13-15 */
         case ORQ_BF16: q->align = 16; q->nbits = 16; break;
+        case ORQ_PQ8: /* ProductQuant(dim, sd = 4): Quantizer<metric, 1, 8> over nsq = dim / 4 (product_quant.hpp:10-24) */
+            if (dim % 4 != 0) return -1;
+            q->align = 1;
+            q->nbits = 8;
+            q->dim_align = q->code_size = dim / 4;
+            return 0;
         default: return -1;
     }
     /* Tensor ctor (glass/storage/tensor.hpp:30-35) */
@@ -168,11 +175,269 @@ int orq_quant_init(orq_quant *q, int32_t kind, int32_t metric, int32_t dim) {
 void orq_quant_free(orq_quant *q) {
     free(q->mins);
     free(q->difs);
+    free(q->codebook);
     q->mins = q->difs = NULL;
+    q->codebook = NULL;
+}
+
+/* ------------------------------------------------------------------ */
+/* PQ8: ProductQuant<metric> (glass/quant/product_quant.hpp) with K = 256
+ * centroids per 4-dim sub-space, trained by glass::Clustering
+ * (glass/algorithms/clustering.hpp: k-means++ seeding, 10 Lloyd epochs).
+ *
+ * Random numbers come from libstdc++ 11 (the library the reference is
+ * compiled against): std::mt19937 default-seeded (5489), GenRandom
+ * (glass/utils.hpp:13-27), std::uniform_int_distribution<int> (Lemire's
+ * nearly divisionless method with 64-bit products, bits/uniform_int_dist.h
+ * _S_nd) and std::generate_canonical<double, 53> (two draws,
+ * bits/random.tcc).  Float steps follow the x86 code GCC -Ofast emits for the
+ * reference (compiled here from its sources with its own flags):
+ *   4-dim L2 / dot (helpa::l2_fp32_fp32 / dot_fp32_fp32 with d = 4, the
+ *   scalar tail vectorised 4-wide): t_i = x_i - y_i, s_i = t_i * t_i,
+ *   (s0 + s2) + (s1 + s3); dot: -((p0 + p2) + (p1 + p3))
+ *   k-means++ total (double): 8 floats per step into 4 double lanes,
+ *   a_l += d[8b+l] + d[8b+4+l], then (a1 + a3) + (a0 + a2), a 4-wide step
+ *   (d1 + d3) + (d0 + d2), then the rest one by one
+ *   new centroid = sum * (1.0f / float(count))   (reciprocal math)
+ *   LUT sum: see orq_pq_lut_sum. */
+
+typedef struct {
+    uint32_t mt[624];
+    int32_t i;
+} mt19937_t;
+
+static void mt_seed(mt19937_t *g, uint32_t seed) {
+    g->mt[0] = seed;
+    for (int i = 1; i < 624; ++i) g->mt[i] = 1812433253u * (g->mt[i - 1] ^ (g->mt[i - 1] >> 30)) + (uint32_t)i;
+    g->i = 624;
+}
+
+static uint32_t mt_next(mt19937_t *g) {
+    if (g->i >= 624) {
+        for (int k = 0; k < 624; ++k) {
+            const uint32_t y = (g->mt[k] & 0x80000000u) | (g->mt[(k + 1) % 624] & 0x7fffffffu);
+            g->mt[k] = g->mt[(k + 397) % 624] ^ (y >> 1) ^ ((y & 1u) ? 0x9908b0dfu : 0u);
+        }
+        g->i = 0;
+    }
+    uint32_t y = g->mt[g->i++];
+    y ^= y >> 11;
+    y ^= (y << 7) & 0x9d2c5680u;
+    y ^= (y << 15) & 0xefc60000u;
+    y ^= y >> 18;
+    return y;
+}
+
+void orq_mt19937_outputs(int32_t count, uint32_t *out) {
+    mt19937_t g;
+    mt_seed(&g, 5489u);
+    for (int32_t i = 0; i < count; ++i) out[i] = mt_next(&g);
+}
+
+/* uniform_int_distribution<int>(a, b) */
+static int32_t uniform_int(mt19937_t *g, int32_t a, int32_t b) {
+    const uint32_t range = (uint32_t)((int64_t)b - a) + 1u;
+    uint64_t product = (uint64_t)mt_next(g) * range;
+    uint32_t low = (uint32_t)product;
+    if (low < range) {
+        const uint32_t threshold = (0u - range) % range;
+        while (low < threshold) {
+            product = (uint64_t)mt_next(g) * range;
+            low = (uint32_t)product;
+        }
+    }
+    return a + (int32_t)(product >> 32);
+}
+
+/* uniform_real_distribution<double>(0, 1) = generate_canonical<double, 53> */
+static double canonical(mt19937_t *g) {
+    const double x1 = (double)mt_next(g);
+    const double x2 = (double)mt_next(g);
+    const double r = fma(x2, 4294967296.0, x1) * 0x1p-64;
+    return r >= 1.0 ? nextafter(1.0, 0.0) : r;
+}
+
+/* GenRandom (glass/utils.hpp:13-27) */
+static void gen_random(mt19937_t *g, int32_t *addr, int32_t size, int32_t N) {
+    for (int32_t i = 0; i < size; ++i) addr[i] = (int32_t)(mt_next(g) % (uint32_t)(N - size));
+    for (int32_t i = 1; i < size; ++i) { /* std::sort */
+        int32_t v = addr[i], j = i - 1;
+        while (j >= 0 && addr[j] > v) {
+            addr[j + 1] = addr[j];
+            --j;
+        }
+        addr[j + 1] = v;
+    }
+    for (int32_t i = 1; i < size; ++i)
+        if (addr[i] <= addr[i - 1]) addr[i] = addr[i - 1] + 1;
+    const int32_t off = (int32_t)(mt_next(g) % (uint32_t)N);
+    for (int32_t i = 0; i < size; ++i) addr[i] = (addr[i] + off) % N;
+}
+
+static float l2_4(const float *x, const float *y) {
+    float s[4];
+    for (int i = 0; i < 4; ++i) {
+        const float t = x[i] - y[i];
+        s[i] = t * t;
+    }
+    return (s[0] + s[2]) + (s[1] + s[3]);
+}
+
+static float dot_4(const float *x, const float *y) {
+    float p[4];
+    for (int i = 0; i < 4; ++i) p[i] = x[i] * y[i];
+    return -((p[0] + p[2]) + (p[1] + p[3]));
+}
+
+static double kpp_total(const float *d, int32_t n) {
+    double S = 0.0;
+    int32_t i = 0;
+    if (n >= 8) {
+        double a[4] = {0.0, 0.0, 0.0, 0.0};
+        for (; i + 8 <= n; i += 8)
+            for (int l = 0; l < 4; ++l) a[l] = a[l] + ((double)d[i + 4 + l] + (double)d[i + l]);
+        S = (a[1] + a[3]) + (a[0] + a[2]);
+    }
+    if (n - i >= 4) {
+        S = S + (((double)d[i + 1] + (double)d[i + 3]) + ((double)d[i] + (double)d[i + 2]));
+        i += 4;
+    }
+    for (; i < n; ++i) S = S + (double)d[i];
+    return S;
+}
+
+/* Clustering::fit on n rows of 4 floats with k clusters (kmeans++ init). */
+void orq_kmeans_fit4(const float *rows, int32_t n, int32_t k, int32_t epochs, float *medoids) {
+    const int32_t m = n - 1 < 100000 ? n - 1 : 100000; /* kSampleNum */
+    int32_t *samples = (int32_t *)malloc(sizeof(int32_t) * (size_t)(m > 0 ? m : 1));
+    mt19937_t g;
+    mt_seed(&g, 5489u);
+    gen_random(&g, samples, m, n);
+    float *tr = (float *)malloc(sizeof(float) * 4 * (size_t)(m > 0 ? m : 1));
+    for (int32_t i = 0; i < m; ++i) memcpy(tr + 4 * (int64_t)i, rows + 4 * (int64_t)samples[i], 16);
+    memset(medoids, 0, sizeof(float) * 4 * (size_t)k);
+    /* init_kmeanspp (clustering.hpp:138-196) */
+    mt19937_t r;
+    mt_seed(&r, 5489u);
+    float *dist = (float *)malloc(sizeof(float) * (size_t)(m > 0 ? m : 1));
+    int32_t *picked = (int32_t *)malloc(sizeof(int32_t) * (size_t)k);
+    const int32_t init_id = uniform_int(&r, 0, m - 1);
+    int32_t npicked = 1;
+    picked[0] = init_id;
+    memcpy(medoids, tr + 4 * (int64_t)init_id, 16);
+    for (int32_t i = 0; i < m; ++i) dist[i] = l2_4(tr + 4 * (int64_t)init_id, tr + 4 * (int64_t)i);
+    int sum_flag = 0;
+    while (npicked < k) {
+        double dart = canonical(&r);
+        const double sum = kpp_total(dist, m);
+        if (sum < 1e-6) sum_flag = 1;
+        dart *= sum;
+        double prefix = 0.0;
+        int32_t pivot = 0;
+        for (int32_t i = 0; i < m; ++i) {
+            pivot = i;
+            const double next = prefix + (double)dist[i];
+            if (dart >= prefix && dart < next) break;
+            prefix = next;
+        }
+        int dup = 0;
+        for (int32_t j = 0; j < npicked; ++j) dup |= picked[j] == pivot;
+        if (dup && !sum_flag) continue;
+        picked[npicked] = pivot;
+        memcpy(medoids + 4 * (int64_t)npicked, tr + 4 * (int64_t)pivot, 16);
+        for (int32_t i = 0; i < m; ++i) {
+            const float v = l2_4(tr + 4 * (int64_t)pivot, tr + 4 * (int64_t)i);
+            dist[i] = v < dist[i] ? v : dist[i]; /* std::min */
+        }
+        npicked++;
+    }
+    /* Lloyd epochs: compute_dist + find_index + update_mediods (clustering.hpp:198-256) */
+    int32_t *idx = (int32_t *)malloc(sizeof(int32_t) * (size_t)(m > 0 ? m : 1));
+    int32_t *cnt = (int32_t *)malloc(sizeof(int32_t) * (size_t)k);
+    float *nm = (float *)malloc(sizeof(float) * 4 * (size_t)k);
+    for (int32_t e = 0; e < epochs; ++e) {
+        for (int32_t i = 0; i < m; ++i) {
+            float best = HUGE_VALF;
+            int32_t bid = -1;
+            for (int32_t j = 0; j < k; ++j) {
+                const float v = l2_4(tr + 4 * (int64_t)i, medoids + 4 * (int64_t)j);
+                if (v < best) {
+                    best = v;
+                    bid = j;
+                }
+            }
+            idx[i] = bid;
+        }
+        memset(nm, 0, sizeof(float) * 4 * (size_t)k);
+        memset(cnt, 0, sizeof(int32_t) * (size_t)k);
+        for (int32_t i = 0; i < m; ++i) {
+            cnt[idx[i]]++;
+            for (int j = 0; j < 4; ++j) nm[4 * (int64_t)idx[i] + j] += tr[4 * (int64_t)i + j];
+        }
+        for (int32_t c = 0; c < k; ++c) {
+            if (cnt[c] == 0) continue;
+            const float rcp = 1.0f / (float)cnt[c];
+            for (int j = 0; j < 4; ++j) nm[4 * (int64_t)c + j] *= rcp;
+        }
+        memcpy(medoids, nm, sizeof(float) * 4 * (size_t)k);
+    }
+    free(samples);
+    free(tr);
+    free(dist);
+    free(picked);
+    free(idx);
+    free(cnt);
+    free(nm);
+}
+
+/* ProductQuant::dist_func, sum over nsq LUT entries as GCC -Ofast vectorises
+ * it (AVX2 gathers): blocks of 32 into 8 lanes, acc_l += (G(l) + G(8+l)) +
+ * (G(16+l) + G(24+l)), then h_i = acc_i + acc_(i+4), (h0 + h2) + (h1 + h3);
+ * one block of 16 into 4 lanes, v_i = (G(i) + G(4+i)) + (G(8+i) + G(12+i)),
+ * r += (v1 + v3) + (v0 + v2); then r += G(s) one by one. */
+float orq_pq_lut_sum(const float *lut, const uint8_t *code, int32_t nsq) {
+#define PQ_G(s) lut[(int64_t)(s) * 256 + code[(s)]]
+    float r = 0.0f;
+    int32_t c = 0;
+    if (nsq >= 32) {
+        float acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        for (; c + 32 <= nsq; c += 32)
+            for (int l = 0; l < 8; ++l)
+                acc[l] = acc[l] + ((PQ_G(c + l) + PQ_G(c + 8 + l)) + (PQ_G(c + 16 + l) + PQ_G(c + 24 + l)));
+        float h[4];
+        for (int i = 0; i < 4; ++i) h[i] = acc[i] + acc[i + 4];
+        r = (h[0] + h[2]) + (h[1] + h[3]);
+    }
+    if (nsq - c >= 16) {
+        float v[4];
+        for (int i = 0; i < 4; ++i) v[i] = (PQ_G(c + i) + PQ_G(c + 4 + i)) + (PQ_G(c + 8 + i) + PQ_G(c + 12 + i));
+        r = r + ((v[1] + v[3]) + (v[0] + v[2]));
+        c += 16;
+    }
+    for (; c < nsq; ++c) r = r + PQ_G(c);
+    return r;
+#undef PQ_G
+}
+
+/* ProductQuant::train: one Clustering per 4-dim sub-space (product_quant.hpp:26-41) */
+static void pq_train(orq_quant *q, const float *data, int64_t n) {
+    const int32_t nsq = q->dim / 4;
+    q->codebook = (float *)calloc((size_t)nsq * 256 * 4, sizeof(float));
+#pragma omp parallel for schedule(dynamic)
+    for (int32_t sq = 0; sq < nsq; ++sq) {
+        float *rows = (float *)malloc(sizeof(float) * 4 * (size_t)n);
+        for (int64_t i = 0; i < n; ++i) memcpy(rows + 4 * i, data + i * q->dim + 4 * sq, 16);
+        orq_kmeans_fit4(rows, (int32_t)n, 256, 10, q->codebook + (int64_t)sq * 256 * 4);
+        free(rows);
+    }
 }
 
 int orq_quant_train(orq_quant *q, const float *data, int64_t n) {
     int64_t nd = n * (int64_t)q->dim;
+    if (q->kind == ORQ_PQ8) {
+        pq_train(q, data, n);
+        return 0;
+    }
     float mn, mx;
     switch (q->kind) {
         case ORQ_SQ8U:
@@ -259,6 +524,18 @@ void orq_encode(const orq_quant *q, const float *x, uint8_t *out) {
     memset(out, 0, (size_t)q->code_size);
     int32_t d = q->dim;
     switch (q->kind) {
+        case ORQ_PQ8: /* ProductQuant::encode (product_quant.hpp:52-64): first nearest centroid */
+            for (int32_t sq = 0; sq < q->code_size; ++sq) {
+                float best = FLT_MAX;
+                for (int32_t k = 0; k < 256; ++k) {
+                    const float v = l2_4(x + 4 * sq, q->codebook + ((int64_t)sq * 256 + k) * 4);
+                    if (v < best) {
+                        best = v;
+                        out[sq] = (uint8_t)k;
+                    }
+                }
+            }
+            break;
         case ORQ_SQ8U: /* sq8u_quant.hpp:38-42 */
             for (int32_t j = 0; j < d; ++j) {
                 int32_t c = q->metric == ORQ_L2 ? affine_code(x[j], q->min, q->dif, 127) : sym_code(x[j], q->max, 127);
@@ -342,12 +619,20 @@ void orq_encode_rows(const orq_quant *q, const float *data, int64_t n, uint8_t *
  * into a zeroed dim_align float buffer. */
 int64_t orq_query_size(const orq_quant *q) {
     if (q->kind == ORQ_SQ8U || q->kind == ORQ_SQ4U || q->kind == ORQ_SQ2U) return q->code_size;
+    if (q->kind == ORQ_PQ8) return (int64_t)q->code_size * 256 * 4; /* the LUT */
     return (int64_t)q->dim_align * 4;
 }
 
 void orq_prepare_query(const orq_quant *q, const float *x, uint8_t *qbuf) {
     if (q->kind == ORQ_SQ8U || q->kind == ORQ_SQ4U || q->kind == ORQ_SQ2U) {
         orq_encode(q, x, qbuf);
+    } else if (q->kind == ORQ_PQ8) { /* ProductQuant::get_lut (product_quant.hpp:72-82) */
+        float *lut = (float *)qbuf;
+        for (int32_t sq = 0; sq < q->code_size; ++sq)
+            for (int32_t k = 0; k < 256; ++k) {
+                const float *c = q->codebook + ((int64_t)sq * 256 + k) * 4;
+                lut[(int64_t)sq * 256 + k] = q->metric == ORQ_L2 ? l2_4(x + 4 * sq, c) : dot_4(x + 4 * sq, c);
+            }
     } else {
         memset(qbuf, 0, (size_t)q->dim_align * 4);
         memcpy(qbuf, x, sizeof(float) * (size_t)q->dim);
@@ -385,6 +670,8 @@ static float reduce16_helpa(const float a[16]) {
 double orq_distance_ctx(const orq_quant *q, const uint8_t *qbuf, const uint8_t *code, int32_t ctx) {
     int32_t da = q->dim_align;
     switch (q->kind) {
+        case ORQ_PQ8: /* ProductQuant::dist_func (product_quant.hpp:84-90) over the query's LUT */
+            return (double)orq_pq_lut_sum((const float *)qbuf, code, q->code_size);
         case ORQ_SQ8U: {
             const int8_t *x = (const int8_t *)qbuf, *y = (const int8_t *)code;
             int64_t s = 0;
@@ -796,7 +1083,7 @@ void orq_search(const orq_graph *g, const orq_quant *q, const uint8_t *codes, co
 }
 
 float orq_refine_factor(int32_t kind) {
-    if (kind == ORQ_SQ8U || kind == ORQ_SQ8 || kind == ORQ_SQ4U) return 1.5f;
+    if (kind == ORQ_SQ8U || kind == ORQ_SQ8 || kind == ORQ_SQ4U || kind == ORQ_PQ8) return 1.5f;
     if (kind == ORQ_SQ2U) return 3.0f;
     return 1.0f;
 }

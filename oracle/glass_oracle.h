@@ -16,13 +16,14 @@ extern "C" {
 #endif
 
 /* quantizer kinds (names as in glass/quant/quant.hpp:23-43) */
-enum { ORQ_FP32 = 0, ORQ_FP16 = 1, ORQ_BF16 = 2, ORQ_E5M2 = 3, ORQ_SQ8U = 4, ORQ_SQ8 = 5, ORQ_SQ6 = 6, ORQ_SQ4U = 7, ORQ_SQ4 = 9, ORQ_SQ2U = 11 };
+enum { ORQ_FP32 = 0, ORQ_FP16 = 1, ORQ_BF16 = 2, ORQ_E5M2 = 3, ORQ_SQ8U = 4, ORQ_SQ8 = 5, ORQ_SQ6 = 6, ORQ_SQ4U = 7, ORQ_SQ4 = 9, ORQ_SQ2U = 11, ORQ_PQ8 = 13 };
 enum { ORQ_L2 = 0, ORQ_IP = 1 };
 
 typedef struct {
     int32_t kind, metric, dim, dim_align, code_size, nbits, align;
     float min, dif, max; /* affine (min, dif) or symmetric (max) calibration */
     float *mins, *difs;  /* per-dim calibration, dim_align entries (SQ8) */
+    float *codebook;     /* PQ8: nsq x 256 x 4 centroids (nsq = dim_align = dim / 4) */
 } orq_quant;
 
 typedef struct {
@@ -94,6 +95,13 @@ void orq_search_many(const orq_graph *g, const orq_quant *q, const uint8_t *code
  * Writes the final pool (ids with checked bit 31), its size and cur_. */
 int orq_pool_run(int32_t cap, int32_t ef, int32_t nops, const int32_t *ops, const double *dists, int32_t *popped,
                  int32_t *out_ids, double *out_dists, int32_t *out_size, int32_t *out_cur);
+
+/* PQ8 pieces, exposed for tests: std::mt19937 (seed 5489) outputs, the
+ * k-means fit of one 4-dim sub-space (glass/algorithms/clustering.hpp) and
+ * the LUT sum (ProductQuant::dist_func). */
+void orq_mt19937_outputs(int32_t count, uint32_t *out);
+void orq_kmeans_fit4(const float *rows, int32_t n, int32_t k, int32_t epochs, float *medoids);
+float orq_pq_lut_sum(const float *lut, const uint8_t *code, int32_t nsq);
 
 /* fp32 -> fp16 bits, round to nearest even (F16C semantics). */
 uint16_t orq_f32_to_f16(float f);

@@ -15,7 +15,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboracle.so")
 
-KINDS = {"FP32": 0, "FP16": 1, "BF16": 2, "E5M2": 3, "SQ8U": 4, "SQ8": 5, "SQ6": 6, "SQ4U": 7, "SQ4": 9, "SQ2U": 11}
+KINDS = {"FP32": 0, "FP16": 1, "BF16": 2, "E5M2": 3, "SQ8U": 4, "SQ8": 5, "SQ6": 6, "SQ4U": 7, "SQ4": 9, "SQ2U": 11, "PQ8": 13}
 METRICS = {"L2": 0, "IP": 1}
 
 
@@ -23,7 +23,7 @@ class _Quant(C.Structure):
     _fields_ = [("kind", C.c_int32), ("metric", C.c_int32), ("dim", C.c_int32), ("dim_align", C.c_int32),
                 ("code_size", C.c_int32), ("nbits", C.c_int32), ("align", C.c_int32), ("min", C.c_float),
                 ("dif", C.c_float), ("max", C.c_float), ("mins", C.POINTER(C.c_float)),
-                ("difs", C.POINTER(C.c_float))]
+                ("difs", C.POINTER(C.c_float)), ("codebook", C.POINTER(C.c_float))]
 
 
 class _Graph(C.Structure):
@@ -72,6 +72,10 @@ def lib():
                                     C.c_int32, C.c_int32, C.c_int32, vp, vp, C.POINTER(_Counters)]
     L.orq_refine_factor.argtypes = [C.c_int32]
     L.orq_refine_factor.restype = C.c_float
+    L.orq_mt19937_outputs.argtypes = [C.c_int32, vp]
+    L.orq_kmeans_fit4.argtypes = [vp, C.c_int32, C.c_int32, C.c_int32, vp]
+    L.orq_pq_lut_sum.argtypes = [vp, vp, C.c_int32]
+    L.orq_pq_lut_sum.restype = C.c_float
     L.orq_search_many.argtypes = [C.POINTER(_Graph), C.POINTER(_Quant), vp, C.POINTER(_Quant), vp, C.c_float, vp,
                                   C.c_int64, C.c_int32, C.c_int32, C.c_int32, vp, vp, vp, vp, C.c_int32]
     L.orq_pool_run.argtypes = [C.c_int32, C.c_int32, C.c_int32, vp, vp, vp, vp, vp, vp, vp]
@@ -123,6 +127,19 @@ class Quant:
             out[:da] = np.ctypeslib.as_array(self.s.mins, (da,))
             out[da:2 * da] = np.ctypeslib.as_array(self.s.difs, (da,))
         return out
+
+    def codebook(self) -> np.ndarray:
+        """PQ8 centroids, [nsq, 256, 4]."""
+        nsq = self.s.code_size
+        return np.ctypeslib.as_array(self.s.codebook, (nsq * 256 * 4,)).reshape(nsq, 256, 4).copy()
+
+    def lut(self, x: np.ndarray) -> np.ndarray:
+        """PQ8 query lookup table (ProductQuant::get_lut), [nsq, 256]."""
+        nsq = self.s.code_size
+        qb = np.zeros(nsq * 256, np.float32)
+        xx = np.ascontiguousarray(x, np.float32)
+        self.L.orq_prepare_query(C.byref(self.s), _p(xx), _p(qb))
+        return qb.reshape(nsq, 256)
 
     def train(self, data: np.ndarray):
         data = np.ascontiguousarray(data, np.float32)
@@ -256,3 +273,26 @@ def f32_to_f16(x: float) -> int:
 
 def f16_to_f32(h: int) -> float:
     return float(lib().orq_f16_to_f32(int(h)))
+
+
+def mt19937_outputs(count: int) -> np.ndarray:
+    """The first `count` outputs of a default-seeded std::mt19937 (seed 5489)."""
+    out = np.zeros(count, np.uint32)
+    lib().orq_mt19937_outputs(count, _p(out))
+    return out
+
+
+def kmeans_fit4(rows: np.ndarray, k: int = 256, epochs: int = 10) -> np.ndarray:
+    """glass::Clustering(k, epochs).fit over rows of 4 floats; [k, 4] centroids."""
+    r = np.ascontiguousarray(rows, np.float32)
+    out = np.zeros((k, 4), np.float32)
+    lib().orq_kmeans_fit4(_p(r), r.shape[0], k, epochs, _p(out))
+    return out
+
+
+def pq_lut_sum(lut: np.ndarray, code: np.ndarray) -> float:
+    """ProductQuant::dist_func over one code row, in the reference's order."""
+    lt = np.ascontiguousarray(lut, np.float32)
+    c = np.ascontiguousarray(code, np.uint8)
+    return float(lib().orq_pq_lut_sum(_p(lt), _p(c), c.size))
+

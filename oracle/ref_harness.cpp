@@ -87,9 +87,44 @@ int quant_dispatch(const std::string &quant, const float *data, int n, int d, co
     return -1;
 }
 
+// ProductQuant (glass/quant/product_quant.hpp): codebooks, codes, per-query
+// lookup tables and computer distances.
+template <glass::Metric M>
+int pq_probe(const float *data, int n, int d, const float *qs, int nq, const int *ids, int nids, float *codebook_out,
+             uint8_t *codes_out, float *lut_out, double *dist_out) {
+    glass::ProductQuant<M> pq(d);
+    pq.train(data, n);
+    pq.add(data, n);
+    const int nsq = pq.nsq, sd = pq.sd, K = glass::ProductQuant<M>::K;
+    if (codebook_out) {
+        for (int sq = 0; sq < nsq; ++sq)
+            for (int k = 0; k < K; ++k)
+                std::memcpy(codebook_out + ((int64_t)sq * K + k) * sd, pq.codebook[sq].get(k), sizeof(float) * sd);
+    }
+    if (codes_out) {
+        for (int i = 0; i < n; ++i) std::memcpy(codes_out + (int64_t)i * nsq, pq.get_code(i), nsq);
+    }
+    for (int qi = 0; qi < nq; ++qi) {
+        if (lut_out) pq.get_lut(qs + (int64_t)qi * d, lut_out + (int64_t)qi * nsq * K);
+        if (dist_out) {
+            auto computer = pq.get_computer(qs + (int64_t)qi * d);
+            for (int j = 0; j < nids; ++j) dist_out[(int64_t)qi * nids + j] = (double)computer(ids[j]);
+        }
+    }
+    return 0;
+}
+
 }  // namespace
 
 extern "C" {
+
+int ref_pq_probe(const char *metric, const float *data, int n, int d, const float *qs, int nq, const int *ids,
+                 int nids, float *codebook_out, uint8_t *codes_out, float *lut_out, double *dist_out) {
+    if (d % 4 != 0) return -1;
+    if (std::string(metric) == "IP")
+        return pq_probe<glass::Metric::IP>(data, n, d, qs, nq, ids, nids, codebook_out, codes_out, lut_out, dist_out);
+    return pq_probe<glass::Metric::L2>(data, n, d, qs, nq, ids, nids, codebook_out, codes_out, lut_out, dist_out);
+}
 
 int ref_set_num_threads(int t) {
     if (t > 0) omp_set_num_threads(t);

@@ -149,3 +149,27 @@ def quant_probe(quant: str, metric: str, data: np.ndarray, queries: np.ndarray |
     if rc != 0:
         raise ValueError(f"unknown quantizer {quant}")
     return codes, params, dist, code_size, dim_align
+
+
+def pq_probe(metric: str, data: np.ndarray, queries: np.ndarray, ids: np.ndarray):
+    """Train the reference ProductQuant (glass/quant/product_quant.hpp, 4 dims
+    per sub-quantizer, 256 centroids) on data; return (codebook[nsq, 256, 4],
+    codes[n, nsq], lut[nq, nsq, 256], dists[nq, nids] float64)."""
+    lib = load(False)
+    data = np.ascontiguousarray(data, dtype=np.float32)
+    qs = np.ascontiguousarray(queries, dtype=np.float32)
+    idv = np.ascontiguousarray(ids, dtype=np.int32)
+    n, d = data.shape
+    nsq = d // 4
+    cb = np.zeros((nsq, 256, 4), np.float32)
+    codes = np.zeros((n, nsq), np.uint8)
+    lut = np.zeros((qs.shape[0], nsq, 256), np.float32)
+    dist = np.zeros((qs.shape[0], len(idv)), np.float64)
+    vp = C.c_void_p
+    lib.ref_pq_probe.argtypes = [C.c_char_p, vp, C.c_int, C.c_int, vp, C.c_int, vp, C.c_int, vp, vp, vp, vp]
+    rc = lib.ref_pq_probe(metric.encode(), data.ctypes.data, n, d, qs.ctypes.data, qs.shape[0], idv.ctypes.data,
+                          len(idv), cb.ctypes.data, codes.ctypes.data, lut.ctypes.data, dist.ctypes.data)
+    if rc != 0:
+        raise ValueError("dim must be a multiple of 4")
+    return cb, codes, lut, dist
+

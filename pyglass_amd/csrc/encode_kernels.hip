@@ -104,6 +104,33 @@ __global__ void encode_kernel(const float *data, int64_t n, int32_t dim, int32_t
     }
 }
 
+// ProductQuant::encode (glass/quant/product_quant.hpp:52-64): for each 4-dim
+// sub-space the first centroid of least L2 distance, the distance rounded as
+// helpa::l2_fp32_fp32 with d = 4 (t_i = x_i - c_i, (t0^2 + t2^2) + (t1^2 + t3^2)).
+// One thread per (row, sub-space).
+__global__ void pq_encode_kernel(const float *data, int64_t n, int32_t nsq, const float *codebook, uint8_t *codes) {
+    const int64_t total = n * (int64_t)nsq;
+    for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+         idx += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t row = idx / nsq;
+        const int32_t sq = (int32_t)(idx - row * nsq);
+        const float4 x = *(const float4 *)(data + row * (int64_t)nsq * 4 + 4 * sq);
+        const float4 *cb = (const float4 *)(codebook + (int64_t)sq * 256 * 4);
+        float best = 3.402823466e+38f;
+        uint32_t code = 0;
+        for (int k = 0; k < 256; ++k) {
+            const float4 c = cb[k];
+            const float t0 = x.x - c.x, t1 = x.y - c.y, t2 = x.z - c.z, t3 = x.w - c.w;
+            const float v = (t0 * t0 + t2 * t2) + (t1 * t1 + t3 * t3);
+            if (v < best) {
+                best = v;
+                code = (uint32_t)k;
+            }
+        }
+        codes[idx] = (uint8_t)code;
+    }
+}
+
 __global__ void pad_rows_kernel(const float *data, int64_t n, int32_t dim, int32_t dim_align, float *out) {
     const int64_t total = n * (int64_t)dim_align;
     for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
@@ -186,6 +213,14 @@ hipError_t launch_encode(const float *data_dev, int64_t n, int32_t dim, int32_t 
     const int64_t total = n * (int64_t)code_size;
     hipLaunchKernelGGL(encode_kernel, dim3(grid_for(total)), dim3(256), 0, stream, data_dev, n, dim, code_size, quant,
                        metric, mn, dif, mx, mins, difs, codes_dev);
+    return hipGetLastError();
+}
+
+hipError_t launch_pq_encode(const float *data_dev, int64_t n, int32_t nsq, const float *codebook, uint8_t *codes_dev,
+                            hipStream_t stream) {
+    const int64_t total = n * (int64_t)nsq;
+    hipLaunchKernelGGL(pq_encode_kernel, dim3(grid_for(total)), dim3(256), 0, stream, data_dev, n, nsq, codebook,
+                       codes_dev);
     return hipGetLastError();
 }
 

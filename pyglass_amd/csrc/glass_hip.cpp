@@ -38,6 +38,8 @@ hipError_t launch_pad_rows(const float *data_dev, int64_t n, int32_t dim, int32_
 hipError_t launch_fill_i32(int32_t *p, int64_t n, int32_t v, hipStream_t stream);
 hipError_t launch_radix_hist(const float *data, int32_t nseg, int64_t seg_len, int64_t seg_stride, int32_t absval,
                              int32_t shift, const uint32_t *prefix, unsigned long long *hist, hipStream_t stream);
+hipError_t launch_pq_encode(const float *data_dev, int64_t n, int32_t nsq, const float *codebook, uint8_t *codes_dev,
+                            hipStream_t stream);
 hipError_t launch_radix_pick(int32_t ntargets, int32_t shift, const unsigned long long *hist, uint32_t *prefix,
                              unsigned long long *rank, hipStream_t stream);
 }  // namespace glass_hip
@@ -108,6 +110,7 @@ int32_t parse_quant(const std::string &s) {
     if (s == "SQ8") return QK_SQ8;
     if (s == "SQ6") return QK_SQ6;
     if (s == "SQ4") return QK_SQ4;
+    if (s == "PQ8") return QK_PQ8;
     if (s == "FP32") return QK_FP32;
     if (s == "FP16") return QK_FP16;
     if (s == "BF16") return QK_BF16;
@@ -131,13 +134,17 @@ void quant_layout(int32_t kind, int32_t dim, int32_t *dim_align, int32_t *code_s
         case QK_FP16: align = 16; nbits = 16; break;
         case QK_BF16: align = 16; nbits = 16; break;
     }
+    if (kind == QK_PQ8) {  // ProductQuant(dim, sd = 4): Quantizer<metric, 1, 8> over dim / 4 sub-spaces
+        *dim_align = *code_size = dim / 4;
+        return;
+    }
     *dim_align = (dim + align - 1) / align * align;  // Tensor ctor, glass/storage/tensor.hpp:30-35
     *code_size = *dim_align * nbits / 8;
 }
 
 // get_refine_factor (glass/searcher/graph_searcher.hpp:227-238)
 float refine_factor_of(int32_t kind) {
-    if (kind == QK_SQ8U || kind == QK_SQ8 || kind == QK_SQ4U) return 1.5f;
+    if (kind == QK_SQ8U || kind == QK_SQ8 || kind == QK_SQ4U || kind == QK_PQ8) return 1.5f;
     if (kind == QK_SQ2U) return 3.0f;
     return 1.0f;
 }
@@ -172,6 +179,7 @@ struct glass_hip_searcher {
     int32_t nb = 0;
     float qmin = 0, qdif = 0, qmax = 0;
     std::vector<float> sq8_mins, sq8_difs;
+    std::vector<float> pq_codebook;  // PQ8: nsq x 256 x 4
     int32_t ef = 32;
     double last_avg_dist_cmps = 0.0;
     bool has_data = false;
@@ -209,7 +217,8 @@ struct glass_hip_searcher {
     DevBuf<int64_t> d_up_off;
     DevBuf<uint8_t> d_codes;
     DevBuf<uint8_t> d_rerank;  // re-rank rows, fp32 or fp16 (refine)
-    DevBuf<float> d_mins, d_difs;
+    DevBuf<float> d_mins, d_difs, d_codebook;
+    int64_t lut_words = 0;  // PQ8: per-slot LUT words ahead of the visited bitset
     // per-slot state
     DevBuf<uint32_t> d_visited;
     int64_t vis_stride = 0;
@@ -265,6 +274,7 @@ struct glass_hip_searcher {
         ix.qmax = qmax;
         ix.sq8_mins = d_mins.p;
         ix.sq8_difs = d_difs.p;
+        ix.pq_codebook = d_codebook.p;
         ix.rerank = refine != QK_NONE ? (const void *)d_rerank.p : nullptr;
         ix.r_dim_align = refine != QK_NONE ? r_dim_align : 0;
         ix.rerank_kind = refine;
@@ -517,6 +527,135 @@ int device_order_stats(hipStream_t stream, const float *data, int32_t nseg, int6
     return GLASS_HIP_OK;
 }
 
+// ---------------------------------------------------------------------------
+// PQ8 training: ProductQuant::train (glass/quant/product_quant.hpp:26-41), one
+// glass::Clustering (glass/algorithms/clustering.hpp) of 256 centroids per
+// 4-dim sub-space: GenRandom sample of min(n - 1, 100000) rows, k-means++
+// seeding, 10 Lloyd epochs.  The random draws are libstdc++'s own
+// (std::mt19937, uniform_int_distribution, uniform_real_distribution), as in
+// the reference; the float steps are rounded as GCC -Ofast compiles the
+// reference:
+//   4-dim L2: t_i = x_i - y_i, (t0^2 + t2^2) + (t1^2 + t3^2)
+//   k-means++ total in double: 8 floats per step into 4 lanes, a_l += d[8b+l]
+//   + d[8b+4+l], (a1 + a3) + (a0 + a2), then one 4-wide step
+//   (d1 + d3) + (d0 + d2), then one by one
+//   centroid = sum * (1.0f / float(count))
+void gen_random(std::mt19937 &rng, int32_t *addr, int32_t size, int32_t N);
+
+namespace pq {
+
+inline float l2_4(const float *x, const float *y) {
+    const float t0 = x[0] - y[0], t1 = x[1] - y[1], t2 = x[2] - y[2], t3 = x[3] - y[3];
+    return (t0 * t0 + t2 * t2) + (t1 * t1 + t3 * t3);
+}
+
+double kpp_total(const float *d, int32_t n) {
+    double S = 0.0;
+    int32_t i = 0;
+    if (n >= 8) {
+        double a[4] = {0.0, 0.0, 0.0, 0.0};
+        for (; i + 8 <= n; i += 8)
+            for (int l = 0; l < 4; ++l) a[l] = a[l] + ((double)d[i + 4 + l] + (double)d[i + l]);
+        S = (a[1] + a[3]) + (a[0] + a[2]);
+    }
+    if (n - i >= 4) {
+        S = S + (((double)d[i + 1] + (double)d[i + 3]) + ((double)d[i] + (double)d[i + 2]));
+        i += 4;
+    }
+    for (; i < n; ++i) S = S + (double)d[i];
+    return S;
+}
+
+// Clustering::fit over n rows of 4 floats; writes k x 4 centroids.
+void fit4(const float *rows, int32_t n, int32_t k, int32_t epochs, float *medoids) {
+    const int32_t m = std::min(n - 1, 100000);  // kSampleNum
+    std::vector<int32_t> samples(std::max(m, 1));
+    std::mt19937 rng;
+    gen_random(rng, samples.data(), m, n);
+    std::vector<float> tr(4 * (size_t)std::max(m, 1));
+    for (int32_t i = 0; i < m; ++i) std::memcpy(&tr[4 * (size_t)i], rows + 4 * (int64_t)samples[i], 16);
+    std::fill(medoids, medoids + 4 * (size_t)k, 0.0f);
+    // init_kmeanspp (clustering.hpp:138-196)
+    std::mt19937 r;
+    std::uniform_real_distribution<> dart_dist(0, 1);
+    std::uniform_int_distribution<int32_t> first(0, m - 1);
+    std::vector<float> dist(std::max(m, 1));
+    std::vector<int32_t> picked;
+    const int32_t init_id = first(r);
+    picked.push_back(init_id);
+    std::memcpy(medoids, &tr[4 * (size_t)init_id], 16);
+    for (int32_t i = 0; i < m; ++i) dist[i] = l2_4(&tr[4 * (size_t)init_id], &tr[4 * (size_t)i]);
+    bool sum_flag = false;
+    while ((int32_t)picked.size() < k) {
+        double dart = dart_dist(r);
+        const double sum = kpp_total(dist.data(), m);
+        if (sum < 1e-6) sum_flag = true;
+        dart *= sum;
+        double prefix = 0.0;
+        int32_t pivot = 0;
+        for (int32_t i = 0; i < m; ++i) {
+            pivot = i;
+            const double next = prefix + (double)dist[i];
+            if (dart >= prefix && dart < next) break;
+            prefix = next;
+        }
+        if (std::find(picked.begin(), picked.end(), pivot) != picked.end() && !sum_flag) continue;
+        std::memcpy(medoids + 4 * picked.size(), &tr[4 * (size_t)pivot], 16);
+        picked.push_back(pivot);
+        for (int32_t i = 0; i < m; ++i) dist[i] = std::min(dist[i], l2_4(&tr[4 * (size_t)pivot], &tr[4 * (size_t)i]));
+    }
+    // Lloyd epochs: compute_dist + find_index + update_mediods (clustering.hpp:198-256)
+    std::vector<int32_t> idx(std::max(m, 1)), cnt(k);
+    std::vector<float> nm(4 * (size_t)k);
+    for (int32_t e = 0; e < epochs; ++e) {
+        for (int32_t i = 0; i < m; ++i) {
+            float best = HUGE_VALF;
+            int32_t bid = -1;
+            for (int32_t j = 0; j < k; ++j) {
+                const float v = l2_4(&tr[4 * (size_t)i], medoids + 4 * (size_t)j);
+                if (v < best) {
+                    best = v;
+                    bid = j;
+                }
+            }
+            idx[i] = bid;
+        }
+        std::fill(nm.begin(), nm.end(), 0.0f);
+        std::fill(cnt.begin(), cnt.end(), 0);
+        for (int32_t i = 0; i < m; ++i) {
+            cnt[idx[i]]++;
+            for (int j = 0; j < 4; ++j) nm[4 * (size_t)idx[i] + j] += tr[4 * (size_t)i + j];
+        }
+        for (int32_t c = 0; c < k; ++c) {
+            if (cnt[c] == 0) continue;
+            const float rcp = 1.0f / (float)cnt[c];
+            for (int j = 0; j < 4; ++j) nm[4 * (size_t)c + j] *= rcp;
+        }
+        std::copy(nm.begin(), nm.end(), medoids);
+    }
+}
+
+// One Clustering per sub-space, sub-spaces spread over host threads.
+void train(const float *data, int64_t n, int32_t dim, std::vector<float> &codebook) {
+    const int32_t nsq = dim / 4;
+    codebook.assign((size_t)nsq * 256 * 4, 0.0f);
+    const int32_t nt = std::max(1, std::min<int32_t>(nsq, (int32_t)std::thread::hardware_concurrency()));
+    std::atomic<int32_t> next{0};
+    auto work = [&]() {
+        std::vector<float> rows(4 * (size_t)n);
+        for (int32_t sq; (sq = next.fetch_add(1)) < nsq;) {
+            for (int64_t i = 0; i < n; ++i) std::memcpy(&rows[4 * (size_t)i], data + i * dim + 4 * sq, 16);
+            fit4(rows.data(), (int32_t)n, 256, 10, &codebook[(size_t)sq * 256 * 4]);
+        }
+    };
+    std::vector<std::thread> th;
+    for (int32_t t = 1; t < nt; ++t) th.emplace_back(work);
+    work();
+    for (auto &t : th) t.join();
+}
+
+}  // namespace pq
+
 // Calibration (train) of the search quantizer from the rows already in HBM.
 int train_quant(glass_hip_searcher &s, const float *data_dev, int64_t n) {
     const int64_t nd = n * (int64_t)s.dim;
@@ -610,7 +749,7 @@ int32_t grid_blocks(const glass_hip_searcher &s, const LaunchConfig &lc, int32_t
 
 int ensure_slots(glass_hip_searcher &s, int32_t slots) {
     if (slots <= s.nslots && s.d_visited.p) return GLASS_HIP_OK;
-    s.vis_stride = ((int64_t)(s.graph.N + 31) / 32 + 3) / 4 * 4;
+    s.vis_stride = s.lut_words + ((int64_t)(s.graph.N + 31) / 32 + 3) / 4 * 4;
     HIP_TRY(s.d_visited.alloc((size_t)slots * (size_t)s.vis_stride));
     s.nslots = slots;
     return GLASS_HIP_OK;
@@ -684,6 +823,7 @@ int run_batch(glass_hip_searcher &s, const float *qs_dev, int32_t nq, const int3
     SlotState ss{};
     ss.visited = bs ? bs->visited : s.d_visited.p;
     ss.vis_stride = s.vis_stride;
+    ss.lut_words = s.lut_words;
     ss.nslots = bs ? bs->max_slots : s.nslots;
     ss.work_counter = counter;
     const DeviceIndex ix = s.index();
@@ -977,9 +1117,13 @@ int glass_hip_searcher_set_data(glass_hip_searcher *s, const float *data, int32_
     if (n != s->graph.N) return fail(GLASS_HIP_EINVAL, "data has %d rows but the graph has %d nodes", n, s->graph.N);
     std::lock_guard<std::mutex> lk(s->mu);
     if (int r = ensure_stream(*s)) return r;
+    if (s->quant == QK_PQ8 && dim % 4 != 0)
+        return fail(GLASS_HIP_EINVAL, "PQ8 needs a dimension divisible by 4 (got %d)", dim);
     s->nb = n;
     s->dim = dim;
     quant_layout(s->quant, dim, &s->dim_align, &s->code_size);
+    s->lut_words = s->quant == QK_PQ8 ? (int64_t)s->code_size * 256 : 0;
+    s->nslots = 0;  // the slot stride depends on lut_words and N: recomputed at the next launch
     int32_t r_row_bytes = 0;
     if (s->refine != QK_NONE) quant_layout(s->refine, dim, &s->r_dim_align, &r_row_bytes);
     // graph
@@ -1011,6 +1155,12 @@ int glass_hip_searcher_set_data(glass_hip_searcher *s, const float *data, int32_
     HIP_TRY(s->d_codes.alloc((size_t)n * s->code_size));
     if (s->quant == QK_FP32) {
         HIP_TRY(launch_pad_rows(raw.p, n, dim, s->dim_align, (float *)s->d_codes.p, s->stream));
+    } else if (s->quant == QK_PQ8) {
+        // k-means on the host, as ProductQuant::train; nearest-centroid codes on the device
+        pq::train(data, n, dim, s->pq_codebook);
+        HIP_TRY(s->d_codebook.alloc(s->pq_codebook.size()));
+        HIP_TRY(hipMemcpy(s->d_codebook.p, s->pq_codebook.data(), 4 * s->pq_codebook.size(), hipMemcpyHostToDevice));
+        HIP_TRY(launch_pq_encode(raw.p, n, s->code_size, s->d_codebook.p, s->d_codes.p, s->stream));
     } else {
         HIP_TRY(launch_encode(raw.p, n, dim, s->code_size, s->quant, s->metric, s->qmin, s->qdif, s->qmax,
                               s->d_mins.p, s->d_difs.p, s->d_codes.p, s->stream));
@@ -1271,7 +1421,10 @@ int glass_hip_searcher_get_codes(const glass_hip_searcher *s, uint8_t *codes) {
 
 int glass_hip_searcher_get_calibration(const glass_hip_searcher *s, float *params, int32_t *n) {
     if (!s || !n) return fail(GLASS_HIP_EINVAL, "null argument");
-    if (s->quant == QK_SQ8 || s->quant == QK_SQ6 || s->quant == QK_SQ4) {
+    if (s->quant == QK_PQ8) {
+        *n = (int32_t)s->pq_codebook.size();
+        if (params) std::copy(s->pq_codebook.begin(), s->pq_codebook.end(), params);
+    } else if (s->quant == QK_SQ8 || s->quant == QK_SQ6 || s->quant == QK_SQ4) {
         *n = 2 * s->dim_align;
         if (params) {
             std::copy(s->sq8_mins.begin(), s->sq8_mins.end(), params);

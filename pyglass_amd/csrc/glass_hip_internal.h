@@ -10,7 +10,7 @@ namespace glass_hip {
 
 // Quantizer kinds; the numbering follows glass/quant/quant.hpp:23 (QuantizerType).
 enum QuantKind : int32_t { QK_FP32 = 0, QK_FP16 = 1, QK_BF16 = 2, QK_E5M2 = 3, QK_SQ8U = 4, QK_SQ8 = 5,
-                           QK_SQ6 = 6, QK_SQ4U = 7, QK_SQ4 = 9, QK_SQ2U = 11, QK_NONE = -1 };
+                           QK_SQ6 = 6, QK_SQ4U = 7, QK_SQ4 = 9, QK_SQ2U = 11, QK_PQ8 = 13, QK_NONE = -1 };
 enum MetricKind : int32_t { MK_L2 = 0, MK_IP = 1 };
 
 // Search-mode semantics of one query (glass/searcher/graph_searcher.hpp):
@@ -34,6 +34,7 @@ struct DeviceIndex {
     float qmin, qdif, qmax;  // affine / symmetric calibration
     const float *sq8_mins;   // dim_align (SQ8 / SQ6 / SQ4 per-dim)
     const float *sq8_difs;
+    const float *pq_codebook; // PQ8: nsq x 256 x 4 centroids (nsq = code_size = dim / 4)
     // re-rank rows (Refiner<FP32Quantizer> or Refiner<FP16Quantizer>), or
     // null: N x r_dim_align fp32 (rerank_kind QK_FP32) or fp16 (QK_FP16)
     const void *rerank;
@@ -70,6 +71,9 @@ struct LaunchConfig {
 struct SlotState {
     uint32_t *visited;       // nslots x vis_stride words
     int64_t vis_stride;      // words per slot (multiple of 4)
+    int64_t lut_words;       // PQ8: the slot's first lut_words words hold the query's LUT (multiple of 4); the
+                             // visited bitset follows
+
     int32_t nslots;
     int32_t *work_counter;   // device counter for the persistent queue
 };

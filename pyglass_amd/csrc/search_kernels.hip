@@ -137,7 +137,11 @@ enum Variant : int {
     V_SQ6_IP = 21,
     V_SQ4_L2 = 22,   // glass::L2SqrSQ4_ext / IPSQ4_ext (glass/simd/avx512.hpp:106-144)
     V_SQ4_IP = 23,
+    V_PQ8_L2 = 24,   // ProductQuant::dist_func over the query's LUT (glass/quant/product_quant.hpp:66-90)
+    V_PQ8_IP = 25,
 };
+
+constexpr bool is_pq_variant(int v) { return v == V_PQ8_L2 || v == V_PQ8_IP; }
 
 // Or-ed into the search mode for initialize_search's evaluations: the
 // compiled reference's SQ6 kernel rounds differently there than in Search's
@@ -715,8 +719,68 @@ __device__ void rerank(const DeviceIndex &ix, const WaveLds &w, int m) {
     }
 }
 
+// PQ8 lookup table (ProductQuant::get_lut, glass/quant/product_quant.hpp:66-76):
+// entry (sq, k) is helpa::l2_fp32_fp32 (L2) or dot_fp32_fp32 (IP) of the
+// query's sub-vector sq and centroid k with d = 4, which GCC -Ofast compiles
+// to one 4-wide step: (t0^2 + t2^2) + (t1^2 + t3^2) with t_i = x_i - c_i, or
+// -((p0 + p2) + (p1 + p3)) with p_i = x_i * c_i.  Written by the wave into
+// its slot's scratch in HBM (nsq x 256 floats).
 template <int V>
-__device__ void float_distances(const DeviceIndex &ix, const WaveLds &w, int m, int32_t mode) {
+__device__ void build_lut(const DeviceIndex &ix, const WaveLds &w, float *lut) {
+    const int n = ix.code_size * 256;
+    for (int e = lane_id(); e < n; e += 64) {
+        const float4 c = *(const float4 *)(ix.pq_codebook + (int64_t)e * 4);
+        const float4 x = *(const float4 *)(w.qf + 4 * (e >> 8));
+        float v;
+        if (V == V_PQ8_L2) {
+            const float t0 = x.x - c.x, t1 = x.y - c.y, t2 = x.z - c.z, t3 = x.w - c.w;
+            v = (t0 * t0 + t2 * t2) + (t1 * t1 + t3 * t3);
+        } else {
+            v = -((x.x * c.x + x.z * c.z) + (x.y * c.y + x.w * c.w));
+        }
+        lut[e] = v;
+    }
+}
+
+// PQ8 distance: sum over the row's nsq codes of lut[sq][code], in the order
+// GCC -Ofast vectorises ProductQuant::dist_func (AVX2 gathers): blocks of 32
+// into 8 accumulators, acc_l += (G(l) + G(8+l)) + (G(16+l) + G(24+l)), reduced
+// as h_i = acc_i + acc_(i+4), (h0 + h2) + (h1 + h3); one block of 16 into 4
+// lanes, v_i = (G(i) + G(4+i)) + (G(8+i) + G(12+i)), added as
+// (v1 + v3) + (v0 + v2); then the rest one by one.  Lane p of a 4-lane group
+// owns acc_(2p) and acc_(2p+1) and v_p; the reductions are the DPP steps of
+// the SQ6 kernel.
+template <int V>
+__device__ __forceinline__ float pq_row4(const DeviceIndex &ix, const float *lut, const uint8_t *row, int p) {
+    const int nsq = ix.code_size;
+    auto G = [&](int sq) { return lut[(sq << 8) + row[sq]]; };
+    float r = 0.0f;
+    int c = 0;
+    if (nsq >= 32) {
+        float a0 = 0.0f, a1 = 0.0f;
+        for (; c + 32 <= nsq; c += 32) {
+            const int l0 = c + 2 * p, l1 = l0 + 1;
+            a0 = a0 + ((G(l0) + G(l0 + 8)) + (G(l0 + 16) + G(l0 + 24)));
+            a1 = a1 + ((G(l1) + G(l1 + 8)) + (G(l1 + 16) + G(l1 + 24)));
+        }
+        a0 = a0 + dpp_xor2(a0);
+        a1 = a1 + dpp_xor2(a1);
+        a0 = a0 + dpp_xor1(a0);
+        a1 = a1 + dpp_xor1(a1);
+        r = a0 + a1;
+    }
+    if (nsq - c >= 16) {
+        const float v = (G(c + p) + G(c + 4 + p)) + (G(c + 8 + p) + G(c + 12 + p));
+        const float wv = v + dpp_xor2(v);  // lane 0: v0 + v2, lane 1: v1 + v3
+        r = r + (dpp_quad_bcast<1>(wv) + dpp_quad_bcast<0>(wv));
+        c += 16;
+    }
+    for (; c < nsq; ++c) r = r + G(c);
+    return r;
+}
+
+template <int V>
+__device__ void float_distances(const DeviceIndex &ix, const WaveLds &w, int m, int32_t mode, const float *lut) {
     if (m <= 0) return;
     const int l = lane_id();
     const int p = l & 3, cj = l >> 2;
@@ -730,6 +794,8 @@ __device__ void float_distances(const DeviceIndex &ix, const WaveLds &w, int m, 
             r = sq6_row4<V>(ix, w, row, p, mode, ix.dim_align);
         } else if (V == V_SQ4_L2 || V == V_SQ4_IP) {
             r = sq4_row4<V>(ix, w, row, p, ix.dim_align);
+        } else if (is_pq_variant(V)) {
+            r = pq_row4<V>(ix, lut, row, p);  // IP entries are already negated dot products
         } else {
             r = float_row4<V>(ix, w, row, p, mode, ix.dim_align);
         }
@@ -1127,7 +1193,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void s
     const size_t wbytes = wave_lds_bytes(ix, lc.cap_max, lc.vis_hash);
     const WaveLds w = carve(lds_raw + (size_t)wv * wbytes, ix, lc.cap_max, lc.vis_hash);
     const int slot = blockIdx.x * lc.waves_per_block + wv;
-    uint32_t *vis = ss.visited + (int64_t)slot * ss.vis_stride;
+    // the slot's HBM scratch: PQ8 lookup table (lut_words), then the visited bitset
+    uint32_t *const slot_mem = ss.visited + (int64_t)slot * ss.vis_stride;
+    float *const lut = (float *)slot_mem;
+    uint32_t *const vis = slot_mem + ss.lut_words;
     const int ql = qf_len(ix);
 
     // The next query's index, and its row when ql <= 256 (xn, four floats
@@ -1189,6 +1258,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void s
                 w.qm[2 * ql + j] = df;
             }
         }
+        if (is_pq_variant(V)) {
+            build_lut<V>(ix, w, lut);
+            __threadfence_block();  // the wave reads the table back from the same CU's L1
+            wave_sync();
+        }
         IntQuery<V, DPL> iq;
         if (is_int_variant(V)) {
             encode_query<V>(ix, w);
@@ -1197,13 +1271,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void s
         }
         // empty visited set (Bitset::reset, neighbor.hpp:67-74)
         VisitedSet vs;
-        vs.reset(w.vt, lc.vis_hash, vis, ss.vis_stride);
+        vs.reset(w.vt, lc.vis_hash, vis, ss.vis_stride - ss.lut_words);
 
         auto dists = [&](int m, int32_t mode) {
             if (is_int_variant(V)) {
                 iq.distances(ix, w, m);
             } else {
-                float_distances<V>(ix, w, m, mode);
+                float_distances<V>(ix, w, m, mode, lut);
             }
             wave_sync();
         };
@@ -1578,6 +1652,7 @@ int kernel_vgprs(const DeviceIndex &ix, const LaunchConfig &lc) {
         case QK_BF16: return l2 ? vgprs_one<V_BF16_L2, 4>(ix, lc) : vgprs_one<V_BF16_IP, 4>(ix, lc);
         case QK_SQ6: return l2 ? vgprs_one<V_SQ6_L2, 4>(ix, lc) : vgprs_one<V_SQ6_IP, 4>(ix, lc);
         case QK_SQ4: return l2 ? vgprs_one<V_SQ4_L2, 4>(ix, lc) : vgprs_one<V_SQ4_IP, 4>(ix, lc);
+        case QK_PQ8: return l2 ? vgprs_one<V_PQ8_L2, 4>(ix, lc) : vgprs_one<V_PQ8_IP, 4>(ix, lc);
         default: return 0;
     }
 }
@@ -1595,6 +1670,7 @@ int max_blocks_per_cu(const DeviceIndex &ix, const LaunchConfig &lc) {
         case QK_BF16: return l2 ? occupancy_one<V_BF16_L2, 4>(ix, lc) : occupancy_one<V_BF16_IP, 4>(ix, lc);
         case QK_SQ6: return l2 ? occupancy_one<V_SQ6_L2, 4>(ix, lc) : occupancy_one<V_SQ6_IP, 4>(ix, lc);
         case QK_SQ4: return l2 ? occupancy_one<V_SQ4_L2, 4>(ix, lc) : occupancy_one<V_SQ4_IP, 4>(ix, lc);
+        case QK_PQ8: return l2 ? occupancy_one<V_PQ8_L2, 4>(ix, lc) : occupancy_one<V_PQ8_IP, 4>(ix, lc);
         default: return 0;
     }
 }
@@ -1632,6 +1708,7 @@ hipError_t launch_search(const DeviceIndex &ix, const QueryBatch &qb, const Laun
         case QK_BF16: return l2 ? launch_one<V_BF16_L2, 4>(ix, qb, lc, ss, stream) : launch_one<V_BF16_IP, 4>(ix, qb, lc, ss, stream);
         case QK_SQ6: return l2 ? launch_one<V_SQ6_L2, 4>(ix, qb, lc, ss, stream) : launch_one<V_SQ6_IP, 4>(ix, qb, lc, ss, stream);
         case QK_SQ4: return l2 ? launch_one<V_SQ4_L2, 4>(ix, qb, lc, ss, stream) : launch_one<V_SQ4_IP, 4>(ix, qb, lc, ss, stream);
+        case QK_PQ8: return l2 ? launch_one<V_PQ8_L2, 4>(ix, qb, lc, ss, stream) : launch_one<V_PQ8_IP, 4>(ix, qb, lc, ss, stream);
         default: return hipErrorInvalidValue;
     }
 }

@@ -121,4 +121,4 @@ def test_errors_are_raised_not_crashes(tmp_path):
         pg.Searcher(g, np.zeros((4, 8), np.float32), "COS", "SQ8U")
     g = pg.Graph.from_arrays(np.full((4, 2), -1, np.int32), [0])
     with pytest.raises(pg.GlassError, match="quantizer"):
-        pg.Searcher(g, np.zeros((4, 8), np.float32), "L2", "PQ8")
+        pg.Searcher(g, np.zeros((4, 8), np.float32), "L2", "SQ8P")

@@ -338,3 +338,120 @@ def test_perdim_search_matches_reference(szp, dsetsp, golden_dir, quant, gname, 
             sd = np.stack([s.search(QQ[i], k)[1] for i in range(nq)])
             np.testing.assert_array_equal(si, z[f"{key}/single_ids"], err_msg=key)
             np.testing.assert_array_equal(sd, z[f"{key}/single_dists"], err_msg=key)
+
+
+# PQ8 (tests/golden/make_golden_pq.py)
+
+@pytest.fixture(scope="module")
+def qpq(golden_dir):
+    return dict(np.load(os.path.join(golden_dir, "quant_pq8.npz")))
+
+
+@pytest.fixture(scope="module")
+def spq(golden_dir):
+    return np.load(os.path.join(golden_dir, "search_pq8.npz"))
+
+
+def test_mt19937_known_answer():
+    """std::mt19937 default-seeded: first output 3499211612, and the 10000th
+    4123659995 (the value the C++ standard requires, [rand.predef])."""
+    out = O.mt19937_outputs(10000)
+    assert out[0] == 3499211612
+    assert out[-1] == 4123659995
+
+
+def test_kmeans_fit4_invariants():
+    """Lloyd fixed point on separable data: 4 well separated blobs, k = 4;
+    every centroid ends at the (float) mean of its blob, each blob found
+    once, whatever the seeding picked."""
+    rng = np.random.default_rng(3)
+    centers = np.array([[0, 0, 0, 0], [10, 0, 0, 0], [0, 10, 0, 0], [0, 0, 10, 10]], np.float32)
+    X = np.concatenate([c + rng.standard_normal((50, 4)).astype(np.float32) * 0.1 for c in centers])
+    cb = O.kmeans_fit4(X, k=4, epochs=10)
+    # the fit samples n - 1 rows: compare against the blob means within tolerance
+    order = [int(np.argmin(((cb - c) ** 2).sum(1))) for c in centers]
+    assert sorted(order) == [0, 1, 2, 3]
+    for c, j in zip(centers, order):
+        np.testing.assert_allclose(cb[j], c, atol=0.1)
+
+
+def test_pq_lut_sum_orders():
+    """The LUT sum against the float64 sum (tolerance) on all block shapes
+    (nsq = 8: tail only; 25: 16-block + tail; 32: one 32-block; 57: 32 + 16 +
+    9), and against the serial float sum, which differs from the blocked one
+    for some rows (so the blocked order is what the golden data pins)."""
+    rng = np.random.default_rng(5)
+    differs = 0
+    for nsq in [8, 25, 32, 57]:
+        lut = rng.standard_normal((nsq, 256)).astype(np.float32) * 100
+        for _ in range(50):
+            code = rng.integers(0, 256, nsq).astype(np.uint8)
+            vals = lut[np.arange(nsq), code]
+            got = O.pq_lut_sum(lut, code)
+            np.testing.assert_allclose(got, vals.astype(np.float64).sum(), rtol=1e-5, atol=1e-3)
+            serial = np.float32(0)
+            for v in vals:
+                serial = np.float32(serial + v)
+            differs += got != serial
+    assert differs > 0
+
+
+@pytest.mark.parametrize("d", [128, 100])
+@pytest.mark.parametrize("metric", ["L2", "IP"])
+def test_pq8_quantizer_matches_reference(qpq, dsets16, metric, d):
+    X, Q, _ = dsets16["c1_hnsw_l2" if d == 128 else "g100_hnsw_l2"]
+    assert sha(X) == str(qpq[f"{d}/data_sha"])
+    key = f"PQ8_{metric}_{d}"
+    oq = O.Quant("PQ8", metric, d).fit(X)
+    cb = oq.codebook()
+    np.testing.assert_array_equal(cb[0], qpq[f"{key}/codebook_head"])
+    assert sha(cb) == str(qpq[f"{key}/codebook_sha"])
+    np.testing.assert_array_equal(oq.codes[:8], qpq[f"{key}/codes_head"])
+    assert sha(oq.codes) == str(qpq[f"{key}/codes_sha"])
+    lut = np.stack([oq.lut(Q[i]) for i in range(4)])
+    np.testing.assert_array_equal(lut[0, 0], qpq[f"{key}/lut_head"])
+    assert sha(lut) == str(qpq[f"{key}/lut_sha"])
+    ids = qpq[f"{d}/ids"]
+    got = np.stack([oq.distances(Q[i], ids) for i in range(16)])
+    np.testing.assert_array_equal(got, qpq[f"{key}/dist"])
+
+
+def test_pq8_lut_is_the_subvector_distance(dsets16):
+    """LUT entry (sq, k) vs the float64 L2 / -dot of the query's sub-vector sq
+    and centroid k; codes are the nearest centroid of each sub-vector."""
+    X, Q, _ = dsets16["g100_hnsw_l2"]
+    for metric in ["L2", "IP"]:
+        oq = O.Quant("PQ8", metric, 100).fit(X)
+        cb = oq.codebook().astype(np.float64)
+        x = Q[0].astype(np.float64).reshape(25, 1, 4)
+        want = ((x - cb) ** 2).sum(-1) if metric == "L2" else -(x * cb).sum(-1)
+        np.testing.assert_allclose(oq.lut(Q[0]), want, rtol=1e-5, atol=1e-5)
+    xr = X[:50].astype(np.float64).reshape(50, 25, 1, 4)
+    near = ((xr - cb[None]) ** 2).sum(-1).argmin(-1)
+    assert (near == oq.codes[:50]).mean() > 0.99
+
+
+CASESPQ = [(g, "none") for g in ["c1_hnsw_l2", "ip_hnsw", "nsg_l2", "g100_hnsw_l2"]] + [("c1_hnsw_l2", "FP32")]
+
+
+@pytest.mark.parametrize("gname,refine", CASESPQ)
+def test_pq8_search_matches_reference(spq, dsetsp, golden_dir, gname, refine):
+    """Ids, distances and dist-cmp counts of Search and SearchBatch; PQ8 re-ranks
+    int(1.5 k) candidates with FP32 rows (get_refine_factor)."""
+    XX, QQ, metric = dsetsp[gname]
+    assert sha(XX) == str(spq[f"{gname}/data_sha"])
+    g = O.Graph(os.path.join(golden_dir, f"{gname}.glass"))
+    s = O.Searcher(g, XX, metric, "PQ8", "" if refine == "none" else refine)
+    nq = 64
+    for ef in [16, 40, 96]:
+        for k in [1, 10, 50]:
+            key = f"{gname}/PQ8/{refine}/{ef}/{k}"
+            s.set_ef(ef)
+            I, D = s.batch_search(QQ[:nq], k)
+            np.testing.assert_array_equal(I, spq[f"{key}/batch_ids"], err_msg=key)
+            np.testing.assert_array_equal(D, spq[f"{key}/batch_dists"], err_msg=key)
+            assert s.get_last_search_dist_cmps() == float(spq[f"{key}/batch_dist_cmps"]), key
+            si = np.stack([s.search(QQ[i], k)[0] for i in range(nq)])
+            sd = np.stack([s.search(QQ[i], k)[1] for i in range(nq)])
+            np.testing.assert_array_equal(si, spq[f"{key}/single_ids"], err_msg=key)
+            np.testing.assert_array_equal(sd, spq[f"{key}/single_dists"], err_msg=key)
