@@ -195,6 +195,35 @@ int glass_hip_server_stats(glass_hip_server *sv, int64_t *batches, int64_t *quer
 /* Stops the dispatcher after the batches in flight; queued requests fail. */
 void glass_hip_server_free(glass_hip_server *sv);
 
+/* ---- Quantizer trainers (python/bindings.cc:206-229) -------------------- */
+
+/* quant_<name>(dim).train(data): calibrates `quantizer` ("SQ8U", "SQ8",
+ * "FP32", "FP16", "BF16", "E5M2", and every other searcher quantizer) on n
+ * rows of dim floats and returns its parameters in the layout of
+ * glass_hip_searcher_get_calibration (none for the float formats).  With
+ * params == NULL only *nparams is set.  The reference's trainers use the L2
+ * metric; metric NULL means "L2". */
+int glass_hip_quant_train(const char *quantizer, const char *metric, const float *data, int32_t n, int32_t dim,
+                          float *params, int32_t *nparams);
+
+/* ---- Clustering (glass/algorithms/clustering.hpp, python/bindings.cc:155-203) */
+
+typedef struct glass_hip_clustering glass_hip_clustering;
+/* Clustering(n_cluster, epochs, init): init "random" (the Python default) or
+ * "kmeans++". */
+int glass_hip_clustering_create(int32_t n_cluster, int32_t epochs, const char *init, glass_hip_clustering **out);
+void glass_hip_clustering_free(glass_hip_clustering *c);
+/* fit(data): k-means over min(n - 1, 100000) sampled rows; distance sweeps on
+ * the GPU, seeding and centroid updates on the host in the reference's order. */
+int glass_hip_clustering_fit(glass_hip_clustering *c, const float *data, int32_t n, int32_t dim);
+/* predict(data): each row's nearest centroid (first on ties). */
+int glass_hip_clustering_predict(const glass_hip_clustering *c, const float *data, int32_t n, int32_t dim,
+                                 int32_t *ids);
+/* transform(data, inplace=True): subtracts each row's nearest centroid in place. */
+int glass_hip_clustering_transform(const glass_hip_clustering *c, float *data, int32_t n, int32_t dim);
+/* The fitted centroids (n_cluster x dim floats); dim is 0 before fit. */
+int glass_hip_clustering_centroids(const glass_hip_clustering *c, float *out, int32_t *n_cluster, int32_t *dim);
+
 #ifdef __cplusplus
 }
 #endif

@@ -306,88 +306,170 @@ static double kpp_total(const float *d, int32_t n) {
     return S;
 }
 
-/* Clustering::fit on n rows of 4 floats with k clusters (kmeans++ init). */
-void orq_kmeans_fit4(const float *rows, int32_t n, int32_t k, int32_t epochs, float *medoids) {
+static float reduce16_helpa(const float a[16]);
+
+/* helpa::l2_fp32_fp32 for any d (x86/l2_impl.hpp:14-18,50-71 with
+ * ref/l2_ref.hpp:7-15), as GCC -Ofast compiles it: the 16-lane AVX-512 body
+ * with fma accumulators and helpa's (c0+c2)+(c1+c3) tree; the scalar tail
+ * vectorised as one 8-wide step (squares rounded, u_i = s_(i+4) + s_i, then
+ * (u1 + u3) + (u0 + u2)), one 4-wide step ((s1 + s3) + (s0 + s2), added),
+ * then fused multiply-adds; body + tail. */
+static float l2_helpa(const float *x, const float *y, int32_t d) {
+    const int32_t da = d / 16 * 16;
+    float A = 0.0f;
+    if (da > 0) {
+        float a[16] = {0};
+        for (int32_t i = 0; i < da; i += 16)
+            for (int l = 0; l < 16; ++l) {
+                const float t = x[i + l] - y[i + l];
+                a[l] = fmaf(t, t, a[l]);
+            }
+        A = reduce16_helpa(a);
+    }
+    float T = 0.0f;
+    int32_t i = da;
+    if (d - i >= 8) {
+        float sq[8], u[4];
+        for (int l = 0; l < 8; ++l) {
+            const float t = x[i + l] - y[i + l];
+            sq[l] = t * t;
+        }
+        for (int l = 0; l < 4; ++l) u[l] = sq[l + 4] + sq[l];
+        T = (u[1] + u[3]) + (u[0] + u[2]);
+        i += 8;
+    }
+    if (d - i >= 4) {
+        float sq[4];
+        for (int l = 0; l < 4; ++l) {
+            const float t = x[i + l] - y[i + l];
+            sq[l] = t * t;
+        }
+        T = T + ((sq[1] + sq[3]) + (sq[0] + sq[2]));
+        i += 4;
+    }
+    for (; i < d; ++i) {
+        const float t = x[i] - y[i];
+        T = fmaf(t, t, T);
+    }
+    return A + T;
+}
+
+/* find_index over compute_dist (clustering.hpp:198-232): the first centroid
+ * at least distance; ties keep the lower index. */
+static int32_t nearest(const float *x, const float *medoids, int32_t k, int32_t d) {
+    float best = HUGE_VALF;
+    int32_t bid = -1;
+    for (int32_t j = 0; j < k; ++j) {
+        const float v = l2_helpa(x, medoids + (int64_t)j * d, d);
+        if (v < best) {
+            best = v;
+            bid = j;
+        }
+    }
+    return bid;
+}
+
+/* Clustering::fit(data, n, dim) (clustering.hpp:73-100) with k clusters,
+ * kmeans++ (init_kmeanspp, clustering.hpp:138-196) or random
+ * (init_random, clustering.hpp:127-136) seeding, then `epochs` Lloyd steps. */
+void orq_clustering_fit(const float *rows, int32_t n, int32_t d, int32_t k, int32_t epochs, int32_t kmeanspp,
+                        float *medoids) {
     const int32_t m = n - 1 < 100000 ? n - 1 : 100000; /* kSampleNum */
     int32_t *samples = (int32_t *)malloc(sizeof(int32_t) * (size_t)(m > 0 ? m : 1));
     mt19937_t g;
     mt_seed(&g, 5489u);
     gen_random(&g, samples, m, n);
-    float *tr = (float *)malloc(sizeof(float) * 4 * (size_t)(m > 0 ? m : 1));
-    for (int32_t i = 0; i < m; ++i) memcpy(tr + 4 * (int64_t)i, rows + 4 * (int64_t)samples[i], 16);
-    memset(medoids, 0, sizeof(float) * 4 * (size_t)k);
-    /* init_kmeanspp (clustering.hpp:138-196) */
-    mt19937_t r;
-    mt_seed(&r, 5489u);
-    float *dist = (float *)malloc(sizeof(float) * (size_t)(m > 0 ? m : 1));
-    int32_t *picked = (int32_t *)malloc(sizeof(int32_t) * (size_t)k);
-    const int32_t init_id = uniform_int(&r, 0, m - 1);
-    int32_t npicked = 1;
-    picked[0] = init_id;
-    memcpy(medoids, tr + 4 * (int64_t)init_id, 16);
-    for (int32_t i = 0; i < m; ++i) dist[i] = l2_4(tr + 4 * (int64_t)init_id, tr + 4 * (int64_t)i);
-    int sum_flag = 0;
-    while (npicked < k) {
-        double dart = canonical(&r);
-        const double sum = kpp_total(dist, m);
-        if (sum < 1e-6) sum_flag = 1;
-        dart *= sum;
-        double prefix = 0.0;
-        int32_t pivot = 0;
-        for (int32_t i = 0; i < m; ++i) {
-            pivot = i;
-            const double next = prefix + (double)dist[i];
-            if (dart >= prefix && dart < next) break;
-            prefix = next;
+    float *tr = (float *)malloc(sizeof(float) * (size_t)d * (size_t)(m > 0 ? m : 1));
+    for (int32_t i = 0; i < m; ++i) memcpy(tr + (int64_t)d * i, rows + (int64_t)d * samples[i], sizeof(float) * d);
+    memset(medoids, 0, sizeof(float) * (size_t)d * (size_t)k);
+    if (kmeanspp) {
+        mt19937_t r;
+        mt_seed(&r, 5489u);
+        float *dist = (float *)malloc(sizeof(float) * (size_t)(m > 0 ? m : 1));
+        int32_t *picked = (int32_t *)malloc(sizeof(int32_t) * (size_t)k);
+        const int32_t init_id = uniform_int(&r, 0, m - 1);
+        int32_t npicked = 1;
+        picked[0] = init_id;
+        memcpy(medoids, tr + (int64_t)d * init_id, sizeof(float) * d);
+        for (int32_t i = 0; i < m; ++i) dist[i] = l2_helpa(tr + (int64_t)d * init_id, tr + (int64_t)d * i, d);
+        int sum_flag = 0;
+        while (npicked < k) {
+            double dart = canonical(&r);
+            const double sum = kpp_total(dist, m);
+            if (sum < 1e-6) sum_flag = 1;
+            dart *= sum;
+            double prefix = 0.0;
+            int32_t pivot = 0;
+            for (int32_t i = 0; i < m; ++i) {
+                pivot = i;
+                const double next = prefix + (double)dist[i];
+                if (dart >= prefix && dart < next) break;
+                prefix = next;
+            }
+            int dup = 0;
+            for (int32_t j = 0; j < npicked; ++j) dup |= picked[j] == pivot;
+            if (dup && !sum_flag) continue;
+            picked[npicked] = pivot;
+            memcpy(medoids + (int64_t)d * npicked, tr + (int64_t)d * pivot, sizeof(float) * d);
+            for (int32_t i = 0; i < m; ++i) {
+                const float v = l2_helpa(tr + (int64_t)d * pivot, tr + (int64_t)d * i, d);
+                dist[i] = v < dist[i] ? v : dist[i]; /* std::min */
+            }
+            npicked++;
         }
-        int dup = 0;
-        for (int32_t j = 0; j < npicked; ++j) dup |= picked[j] == pivot;
-        if (dup && !sum_flag) continue;
-        picked[npicked] = pivot;
-        memcpy(medoids + 4 * (int64_t)npicked, tr + 4 * (int64_t)pivot, 16);
-        for (int32_t i = 0; i < m; ++i) {
-            const float v = l2_4(tr + 4 * (int64_t)pivot, tr + 4 * (int64_t)i);
-            dist[i] = v < dist[i] ? v : dist[i]; /* std::min */
-        }
-        npicked++;
+        free(dist);
+        free(picked);
+    } else {
+        mt19937_t r;
+        mt_seed(&r, 5489u);
+        int32_t *idx0 = (int32_t *)malloc(sizeof(int32_t) * (size_t)k);
+        gen_random(&r, idx0, k, m);
+        for (int32_t i = 0; i < k; ++i) memcpy(medoids + (int64_t)d * i, tr + (int64_t)d * idx0[i], sizeof(float) * d);
+        free(idx0);
     }
-    /* Lloyd epochs: compute_dist + find_index + update_mediods (clustering.hpp:198-256) */
     int32_t *idx = (int32_t *)malloc(sizeof(int32_t) * (size_t)(m > 0 ? m : 1));
     int32_t *cnt = (int32_t *)malloc(sizeof(int32_t) * (size_t)k);
-    float *nm = (float *)malloc(sizeof(float) * 4 * (size_t)k);
+    float *nm = (float *)malloc(sizeof(float) * (size_t)d * (size_t)k);
     for (int32_t e = 0; e < epochs; ++e) {
-        for (int32_t i = 0; i < m; ++i) {
-            float best = HUGE_VALF;
-            int32_t bid = -1;
-            for (int32_t j = 0; j < k; ++j) {
-                const float v = l2_4(tr + 4 * (int64_t)i, medoids + 4 * (int64_t)j);
-                if (v < best) {
-                    best = v;
-                    bid = j;
-                }
-            }
-            idx[i] = bid;
-        }
-        memset(nm, 0, sizeof(float) * 4 * (size_t)k);
+#pragma omp parallel for schedule(static)
+        for (int32_t i = 0; i < m; ++i) idx[i] = nearest(tr + (int64_t)d * i, medoids, k, d);
+        memset(nm, 0, sizeof(float) * (size_t)d * (size_t)k);
         memset(cnt, 0, sizeof(int32_t) * (size_t)k);
         for (int32_t i = 0; i < m; ++i) {
             cnt[idx[i]]++;
-            for (int j = 0; j < 4; ++j) nm[4 * (int64_t)idx[i] + j] += tr[4 * (int64_t)i + j];
+            for (int32_t j = 0; j < d; ++j) nm[(int64_t)d * idx[i] + j] += tr[(int64_t)d * i + j];
         }
         for (int32_t c = 0; c < k; ++c) {
             if (cnt[c] == 0) continue;
             const float rcp = 1.0f / (float)cnt[c];
-            for (int j = 0; j < 4; ++j) nm[4 * (int64_t)c + j] *= rcp;
+            for (int32_t j = 0; j < d; ++j) nm[(int64_t)d * c + j] *= rcp;
         }
-        memcpy(medoids, nm, sizeof(float) * 4 * (size_t)k);
+        memcpy(medoids, nm, sizeof(float) * (size_t)d * (size_t)k);
     }
     free(samples);
     free(tr);
-    free(dist);
-    free(picked);
     free(idx);
     free(cnt);
     free(nm);
+}
+
+/* Clustering::predict / transform (clustering.hpp:102-129) */
+void orq_clustering_predict(const float *medoids, int32_t k, int32_t d, const float *x, int32_t n, int32_t *ids) {
+#pragma omp parallel for schedule(static)
+    for (int32_t i = 0; i < n; ++i) ids[i] = nearest(x + (int64_t)d * i, medoids, k, d);
+}
+
+void orq_clustering_transform(const float *medoids, int32_t k, int32_t d, float *x, int32_t n) {
+#pragma omp parallel for schedule(static)
+    for (int32_t i = 0; i < n; ++i) {
+        const float *c = medoids + (int64_t)d * nearest(x + (int64_t)d * i, medoids, k, d);
+        for (int32_t j = 0; j < d; ++j) x[(int64_t)d * i + j] -= c[j];
+    }
+}
+
+/* the 4-dim fit ProductQuant::train runs per sub-space */
+void orq_kmeans_fit4(const float *rows, int32_t n, int32_t k, int32_t epochs, float *medoids) {
+    orq_clustering_fit(rows, n, 4, k, epochs, 1, medoids);
 }
 
 /* ProductQuant::dist_func, sum over nsq LUT entries as GCC -Ofast vectorises

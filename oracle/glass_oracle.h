@@ -101,6 +101,11 @@ int orq_pool_run(int32_t cap, int32_t ef, int32_t nops, const int32_t *ops, cons
  * the LUT sum (ProductQuant::dist_func). */
 void orq_mt19937_outputs(int32_t count, uint32_t *out);
 void orq_kmeans_fit4(const float *rows, int32_t n, int32_t k, int32_t epochs, float *medoids);
+/* glass::Clustering (glass/algorithms/clustering.hpp) over rows of d floats */
+void orq_clustering_fit(const float *rows, int32_t n, int32_t d, int32_t k, int32_t epochs, int32_t kmeanspp,
+                        float *medoids);
+void orq_clustering_predict(const float *medoids, int32_t k, int32_t d, const float *x, int32_t n, int32_t *ids);
+void orq_clustering_transform(const float *medoids, int32_t k, int32_t d, float *x, int32_t n);
 float orq_pq_lut_sum(const float *lut, const uint8_t *code, int32_t nsq);
 
 /* fp32 -> fp16 bits, round to nearest even (F16C semantics). */

@@ -75,6 +75,9 @@ def lib():
     L.orq_mt19937_outputs.argtypes = [C.c_int32, vp]
     L.orq_kmeans_fit4.argtypes = [vp, C.c_int32, C.c_int32, C.c_int32, vp]
     L.orq_pq_lut_sum.argtypes = [vp, vp, C.c_int32]
+    L.orq_clustering_fit.argtypes = [vp, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, vp]
+    L.orq_clustering_predict.argtypes = [vp, C.c_int32, C.c_int32, vp, C.c_int32, vp]
+    L.orq_clustering_transform.argtypes = [vp, C.c_int32, C.c_int32, vp, C.c_int32]
     L.orq_pq_lut_sum.restype = C.c_float
     L.orq_search_many.argtypes = [C.POINTER(_Graph), C.POINTER(_Quant), vp, C.POINTER(_Quant), vp, C.c_float, vp,
                                   C.c_int64, C.c_int32, C.c_int32, C.c_int32, vp, vp, vp, vp, C.c_int32]
@@ -295,4 +298,32 @@ def pq_lut_sum(lut: np.ndarray, code: np.ndarray) -> float:
     lt = np.ascontiguousarray(lut, np.float32)
     c = np.ascontiguousarray(code, np.uint8)
     return float(lib().orq_pq_lut_sum(_p(lt), _p(c), c.size))
+
+
+class Clustering:
+    """glass::Clustering as the Python binding drives it (python/bindings.cc:155-203)."""
+
+    def __init__(self, n_cluster: int, epochs: int = 10, init: str = "random"):
+        if init not in ("random", "kmeans++"):
+            raise ValueError(init)
+        self.k, self.epochs, self.kpp = n_cluster, epochs, init == "kmeans++"
+        self.centroids = None
+
+    def fit(self, data):
+        x = np.ascontiguousarray(data, np.float32)
+        self.centroids = np.zeros((self.k, x.shape[1]), np.float32)
+        lib().orq_clustering_fit(_p(x), x.shape[0], x.shape[1], self.k, self.epochs, int(self.kpp),
+                                 _p(self.centroids))
+        return self
+
+    def predict(self, data) -> np.ndarray:
+        x = np.ascontiguousarray(data, np.float32)
+        ids = np.zeros(x.shape[0], np.int32)
+        lib().orq_clustering_predict(_p(self.centroids), self.k, x.shape[1], _p(x), x.shape[0], _p(ids))
+        return ids
+
+    def transform(self, data: np.ndarray) -> np.ndarray:
+        x = np.ascontiguousarray(data, np.float32).copy()
+        lib().orq_clustering_transform(_p(self.centroids), self.k, x.shape[1], _p(x), x.shape[0])
+        return x
 

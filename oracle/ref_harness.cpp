@@ -118,6 +118,21 @@ int pq_probe(const float *data, int n, int d, const float *qs, int nq, const int
 
 extern "C" {
 
+// glass::Clustering as python/bindings.cc:155-203 drives it: fit on data,
+// centroids out, predict and in-place transform of `pred` (clustering.hpp).
+int ref_clustering_probe(const float *data, int n, int d, int n_cluster, int epochs, const char *init,
+                         float *centroids_out, float *pred, int npred, int *ids_out) {
+    glass::Clustering c(n_cluster, epochs, init);
+    c.fit(data, n, d);
+    for (int i = 0; i < n_cluster; ++i) std::memcpy(centroids_out + (int64_t)i * d, c.medoids.get(i), sizeof(float) * d);
+    if (npred > 0) {
+        auto ids = c.predict(pred, npred, d);
+        std::memcpy(ids_out, ids.data(), sizeof(int) * npred);
+        c.transform(pred, npred, d);
+    }
+    return 0;
+}
+
 int ref_pq_probe(const char *metric, const float *data, int n, int d, const float *qs, int nq, const int *ids,
                  int nids, float *codebook_out, uint8_t *codes_out, float *lut_out, double *dist_out) {
     if (d % 4 != 0) return -1;

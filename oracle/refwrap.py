@@ -173,3 +173,19 @@ def pq_probe(metric: str, data: np.ndarray, queries: np.ndarray, ids: np.ndarray
         raise ValueError("dim must be a multiple of 4")
     return cb, codes, lut, dist
 
+
+def clustering_probe(data: np.ndarray, n_cluster: int, epochs: int, init: str, pred: np.ndarray):
+    """Fit the reference glass::Clustering; return (centroids[n_cluster, d],
+    predict(pred) ids, pred after the in-place transform)."""
+    lib = load(False)
+    data = np.ascontiguousarray(data, dtype=np.float32)
+    pr = np.ascontiguousarray(pred, dtype=np.float32).copy()
+    n, d = data.shape
+    cent = np.zeros((n_cluster, d), np.float32)
+    ids = np.zeros(pr.shape[0], np.int32)
+    vp = C.c_void_p
+    lib.ref_clustering_probe.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_char_p, vp, vp, C.c_int, vp]
+    lib.ref_clustering_probe(data.ctypes.data, n, d, n_cluster, epochs, init.encode(), cent.ctypes.data,
+                             pr.ctypes.data, pr.shape[0], ids.ctypes.data)
+    return cent, ids, pr
+

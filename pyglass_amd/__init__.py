@@ -16,7 +16,8 @@ import numpy as np
 from . import _lib
 from ._lib import GlassError
 
-__all__ = ["Graph", "Searcher", "Server", "Index", "build_graph", "GlassError", "set_num_threads", "device_count", "version"]
+__all__ = ["Graph", "Searcher", "Server", "Index", "build_graph", "Clustering", "GlassError", "set_num_threads",
+           "device_count", "version", "quant_fp32", "quant_bf16", "quant_fp16", "quant_e5m2", "quant_sq8", "quant_sq8u"]
 
 _NUM_THREADS = 0
 
@@ -389,3 +390,99 @@ class Server:
         b, q, r = C.c_int64(), C.c_int64(), C.c_int64()
         _lib.check(self._L.glass_hip_server_stats(self._h, C.byref(b), C.byref(q), C.byref(r)))
         return {"batches": b.value, "queries": q.value, "requests": r.value}
+
+
+class Clustering:
+    """glass.Clustering (python/bindings.cc:155-203; glass/algorithms/clustering.hpp):
+    k-means with `init` "random" (the binding's default) or "kmeans++"."""
+
+    def __init__(self, n_cluster: int, epochs: int = 10, init: str = "random"):
+        self._L = _lib.load()
+        h = C.c_void_p()
+        _lib.check(self._L.glass_hip_clustering_create(int(n_cluster), int(epochs), init.encode(), C.byref(h)))
+        self._h = h
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            self._L.glass_hip_clustering_free(h)
+            self._h = None
+
+    def fit(self, data):
+        x = _as_f32(data)
+        _lib.check(self._L.glass_hip_clustering_fit(self._h, _p(x), x.shape[0], x.shape[1]))
+
+    def predict(self, data) -> np.ndarray:
+        x = _as_f32(data)
+        ids = np.empty(x.shape[0], np.int32)
+        _lib.check(self._L.glass_hip_clustering_predict(self._h, _p(x), x.shape[0], x.shape[1], _p(ids)))
+        return ids
+
+    def transform(self, data, inplace: bool = False):
+        """With inplace=True subtracts each row's nearest centroid from `data`
+        (a float32 C-contiguous array is changed in place); returns None, as
+        the binding does (it does nothing without inplace)."""
+        if not inplace:
+            return None
+        x = data if isinstance(data, np.ndarray) and data.dtype == np.float32 and data.flags.c_contiguous \
+            else _as_f32(data)
+        _lib.check(self._L.glass_hip_clustering_transform(self._h, _p(x), x.shape[0], x.shape[1]))
+        return None
+
+    def centroids(self) -> np.ndarray:
+        """The fitted centroids, [n_cluster, dim] (not in the binding)."""
+        k, d = C.c_int32(), C.c_int32()
+        _lib.check(self._L.glass_hip_clustering_centroids(self._h, None, C.byref(k), C.byref(d)))
+        out = np.zeros((k.value, d.value), np.float32)
+        _lib.check(self._L.glass_hip_clustering_centroids(self._h, _p(out), C.byref(k), C.byref(d)))
+        return out
+
+
+class _QuantTrainer:
+    """quant_<name>(dim).train(data) (python/bindings.cc:206-229): trains the
+    quantizer on data with the L2 metric.  `params()` (not in the binding)
+    returns what training found, in the layout of Searcher.calibration()."""
+
+    _quant = ""
+
+    def __init__(self, dim: int):
+        self.dim = int(dim)
+        self._params = np.zeros(0, np.float32)
+
+    def train(self, data):
+        L = _lib.load()
+        x = _as_f32(data).reshape(-1, self.dim)
+        n = C.c_int32()
+        _lib.check(L.glass_hip_quant_train(self._quant.encode(), b"L2", None, 0, self.dim, None, C.byref(n)))
+        out = np.zeros(max(1, n.value), np.float32)
+        _lib.check(L.glass_hip_quant_train(self._quant.encode(), b"L2", _p(x), x.shape[0], self.dim, _p(out),
+                                           C.byref(n)))
+        self._params = out[: n.value]
+
+    def params(self) -> np.ndarray:
+        return self._params
+
+
+class quant_fp32(_QuantTrainer):
+    _quant = "FP32"
+
+
+class quant_bf16(_QuantTrainer):
+    _quant = "BF16"
+
+
+class quant_fp16(_QuantTrainer):
+    _quant = "FP16"
+
+
+class quant_e5m2(_QuantTrainer):
+    _quant = "E5M2"
+
+
+class quant_sq8(_QuantTrainer):
+    _quant = "SQ8"
+
+
+class quant_sq8u(_QuantTrainer):
+    _quant = "SQ8U"
+

@@ -53,6 +53,13 @@ SIGNATURES = {
     "glass_hip_searcher_get_codes": (C.c_int, [_vp, _vp]),
     "glass_hip_searcher_get_calibration": (C.c_int, [_vp, _vp, _pi32]),
     "glass_hip_searcher_sync": (C.c_int, [_vp]),
+    "glass_hip_quant_train": (C.c_int, [C.c_char_p, C.c_char_p, _vp, _i32, _i32, _vp, _pi32]),
+    "glass_hip_clustering_create": (C.c_int, [_i32, _i32, C.c_char_p, C.POINTER(_vp)]),
+    "glass_hip_clustering_free": (None, [_vp]),
+    "glass_hip_clustering_fit": (C.c_int, [_vp, _vp, _i32, _i32]),
+    "glass_hip_clustering_predict": (C.c_int, [_vp, _vp, _i32, _i32, _vp]),
+    "glass_hip_clustering_transform": (C.c_int, [_vp, _vp, _i32, _i32]),
+    "glass_hip_clustering_centroids": (C.c_int, [_vp, _vp, _pi32, _pi32]),
     "glass_hip_server_create": (C.c_int, [_vp, _i32, _i32, _i32, C.POINTER(_vp)]),
     "glass_hip_server_submit": (C.c_int, [_vp, _vp, _i32, _i32, _i32, C.POINTER(C.c_int64)]),
     "glass_hip_server_poll": (C.c_int, [_vp, C.c_int64, _vp, _vp, _pi32]),

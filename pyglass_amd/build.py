@@ -16,7 +16,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libglass_hip.so")
 
-SOURCES = ["search_kernels.hip", "encode_kernels.hip", "glass_hip.cpp"]
+SOURCES = ["search_kernels.hip", "encode_kernels.hip", "glass_hip.cpp", "clustering.cpp"]
 HOST_SOURCES = ["graph_build.cpp"]
 ARCH = os.environ.get("GLASS_HIP_ARCH", "gfx950")
 

@@ -131,6 +131,91 @@ __global__ void pq_encode_kernel(const float *data, int64_t n, int32_t nsq, cons
     }
 }
 
+// helpa::l2_fp32_fp32 for any d (x86/l2_impl.hpp:14-18,50-71 with
+// ref/l2_ref.hpp:7-15), rounded as GCC -Ofast compiles it for the reference:
+// the 16-lane body accumulates fma(t, t, a_l) for elements l, l+16, ... and
+// reduces b_i = a_(i+8) + a_i, c_i = b_(i+4) + b_i, (c0 + c2) + (c1 + c3);
+// the tail of d mod 16 elements takes one 8-wide step (squares rounded,
+// u_i = s_(i+4) + s_i, (u1 + u3) + (u0 + u2)), one 4-wide step
+// ((s1 + s3) + (s0 + s2), added), then fused multiply-adds; body + tail.
+__device__ float l2_helpa(const float *x, const float *y, int32_t d) {
+    const int32_t da = d / 16 * 16;
+    float A = 0.0f;
+    if (da > 0) {
+        float a[16];
+#pragma unroll
+        for (int l = 0; l < 16; ++l) a[l] = 0.0f;
+        for (int32_t i = 0; i < da; i += 16) {
+#pragma unroll
+            for (int l = 0; l < 16; ++l) {
+                const float t = x[i + l] - y[i + l];
+                a[l] = fmaf(t, t, a[l]);
+            }
+        }
+        float b[8], c[4];
+#pragma unroll
+        for (int l = 0; l < 8; ++l) b[l] = a[l + 8] + a[l];
+#pragma unroll
+        for (int l = 0; l < 4; ++l) c[l] = b[l + 4] + b[l];
+        A = (c[0] + c[2]) + (c[1] + c[3]);
+    }
+    float T = 0.0f;
+    int32_t i = da;
+    if (d - i >= 8) {
+        float u[4];
+#pragma unroll
+        for (int l = 0; l < 4; ++l) {
+            const float t0 = x[i + l] - y[i + l], t1 = x[i + 4 + l] - y[i + 4 + l];
+            u[l] = t1 * t1 + t0 * t0;
+        }
+        T = (u[1] + u[3]) + (u[0] + u[2]);
+        i += 8;
+    }
+    if (d - i >= 4) {
+        float q[4];
+#pragma unroll
+        for (int l = 0; l < 4; ++l) {
+            const float t = x[i + l] - y[i + l];
+            q[l] = t * t;
+        }
+        T = T + ((q[1] + q[3]) + (q[0] + q[2]));
+        i += 4;
+    }
+    for (; i < d; ++i) {
+        const float t = x[i] - y[i];
+        T = fmaf(t, t, T);
+    }
+    return A + T;
+}
+
+// Clustering's compute_dist + find_index (glass/algorithms/clustering.hpp:198-232):
+// each row's first nearest centroid.  One thread per row; the lanes of a wave
+// read the same centroid at the same time.
+__global__ void cluster_assign_kernel(const float *x, int32_t n, int32_t d, const float *cent, int32_t k,
+                                      int32_t *ids) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float *xi = x + i * d;
+    float best = __int_as_float(0x7f800000);
+    int32_t bid = -1;
+    for (int32_t j = 0; j < k; ++j) {
+        const float v = l2_helpa(xi, cent + (int64_t)j * d, d);
+        if (v < best) {
+            best = v;
+            bid = j;
+        }
+    }
+    ids[i] = bid;
+}
+
+// k-means++ seeding (clustering.hpp:138-196): distance of every row to one
+// picked row, as the symmetric computer evaluates it (picked row first).
+__global__ void cluster_dist_kernel(const float *x, int32_t n, int32_t d, const float *pivot, float *out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    out[i] = l2_helpa(pivot, x + i * d, d);
+}
+
 __global__ void pad_rows_kernel(const float *data, int64_t n, int32_t dim, int32_t dim_align, float *out) {
     const int64_t total = n * (int64_t)dim_align;
     for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
@@ -221,6 +306,18 @@ hipError_t launch_pq_encode(const float *data_dev, int64_t n, int32_t nsq, const
     const int64_t total = n * (int64_t)nsq;
     hipLaunchKernelGGL(pq_encode_kernel, dim3(grid_for(total)), dim3(256), 0, stream, data_dev, n, nsq, codebook,
                        codes_dev);
+    return hipGetLastError();
+}
+
+hipError_t launch_cluster_assign(const float *x, int32_t n, int32_t d, const float *cent, int32_t k, int32_t *ids,
+                                 hipStream_t stream) {
+    hipLaunchKernelGGL(cluster_assign_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, x, n, d, cent, k, ids);
+    return hipGetLastError();
+}
+
+hipError_t launch_cluster_dist(const float *x, int32_t n, int32_t d, const float *pivot, float *out,
+                               hipStream_t stream) {
+    hipLaunchKernelGGL(cluster_dist_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, x, n, d, pivot, out);
     return hipGetLastError();
 }
 

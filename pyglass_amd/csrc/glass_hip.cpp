@@ -151,6 +151,20 @@ float refine_factor_of(int32_t kind) {
 
 }  // namespace
 
+int glass_hip::set_error(int code, const char *msg) { return fail(code, "%s", msg); }
+
+// GenRandom (glass/utils.hpp:13-27) with a default-seeded mt19937: the rows
+// GraphSearcher::SetData samples for Optimize, and Clustering's samples.
+void glass_hip::gen_random(std::mt19937 &rng, int32_t *addr, int32_t size, int32_t N) {
+    for (int32_t i = 0; i < size; ++i) addr[i] = (int32_t)(rng() % (uint32_t)(N - size));
+    std::sort(addr, addr + size);
+    for (int32_t i = 1; i < size; ++i) {
+        if (addr[i] <= addr[i - 1]) addr[i] = addr[i - 1] + 1;
+    }
+    int32_t off = (int32_t)(rng() % (uint32_t)N);
+    for (int32_t i = 0; i < size; ++i) addr[i] = (addr[i] + off) % N;
+}
+
 struct glass_hip_graph {
     int32_t N = 0, K = 0;
     std::vector<int32_t> adj;
@@ -527,115 +541,12 @@ int device_order_stats(hipStream_t stream, const float *data, int32_t nseg, int6
     return GLASS_HIP_OK;
 }
 
-// ---------------------------------------------------------------------------
 // PQ8 training: ProductQuant::train (glass/quant/product_quant.hpp:26-41), one
-// glass::Clustering (glass/algorithms/clustering.hpp) of 256 centroids per
-// 4-dim sub-space: GenRandom sample of min(n - 1, 100000) rows, k-means++
-// seeding, 10 Lloyd epochs.  The random draws are libstdc++'s own
-// (std::mt19937, uniform_int_distribution, uniform_real_distribution), as in
-// the reference; the float steps are rounded as GCC -Ofast compiles the
-// reference:
-//   4-dim L2: t_i = x_i - y_i, (t0^2 + t2^2) + (t1^2 + t3^2)
-//   k-means++ total in double: 8 floats per step into 4 lanes, a_l += d[8b+l]
-//   + d[8b+4+l], (a1 + a3) + (a0 + a2), then one 4-wide step
-//   (d1 + d3) + (d0 + d2), then one by one
-//   centroid = sum * (1.0f / float(count))
-void gen_random(std::mt19937 &rng, int32_t *addr, int32_t size, int32_t N);
-
+// glass::Clustering of 256 centroids (kmeans++ seeding, 10 Lloyd steps) per
+// 4-dim sub-space on the host k-means backend, sub-spaces spread over host
+// threads.
 namespace pq {
 
-inline float l2_4(const float *x, const float *y) {
-    const float t0 = x[0] - y[0], t1 = x[1] - y[1], t2 = x[2] - y[2], t3 = x[3] - y[3];
-    return (t0 * t0 + t2 * t2) + (t1 * t1 + t3 * t3);
-}
-
-double kpp_total(const float *d, int32_t n) {
-    double S = 0.0;
-    int32_t i = 0;
-    if (n >= 8) {
-        double a[4] = {0.0, 0.0, 0.0, 0.0};
-        for (; i + 8 <= n; i += 8)
-            for (int l = 0; l < 4; ++l) a[l] = a[l] + ((double)d[i + 4 + l] + (double)d[i + l]);
-        S = (a[1] + a[3]) + (a[0] + a[2]);
-    }
-    if (n - i >= 4) {
-        S = S + (((double)d[i + 1] + (double)d[i + 3]) + ((double)d[i] + (double)d[i + 2]));
-        i += 4;
-    }
-    for (; i < n; ++i) S = S + (double)d[i];
-    return S;
-}
-
-// Clustering::fit over n rows of 4 floats; writes k x 4 centroids.
-void fit4(const float *rows, int32_t n, int32_t k, int32_t epochs, float *medoids) {
-    const int32_t m = std::min(n - 1, 100000);  // kSampleNum
-    std::vector<int32_t> samples(std::max(m, 1));
-    std::mt19937 rng;
-    gen_random(rng, samples.data(), m, n);
-    std::vector<float> tr(4 * (size_t)std::max(m, 1));
-    for (int32_t i = 0; i < m; ++i) std::memcpy(&tr[4 * (size_t)i], rows + 4 * (int64_t)samples[i], 16);
-    std::fill(medoids, medoids + 4 * (size_t)k, 0.0f);
-    // init_kmeanspp (clustering.hpp:138-196)
-    std::mt19937 r;
-    std::uniform_real_distribution<> dart_dist(0, 1);
-    std::uniform_int_distribution<int32_t> first(0, m - 1);
-    std::vector<float> dist(std::max(m, 1));
-    std::vector<int32_t> picked;
-    const int32_t init_id = first(r);
-    picked.push_back(init_id);
-    std::memcpy(medoids, &tr[4 * (size_t)init_id], 16);
-    for (int32_t i = 0; i < m; ++i) dist[i] = l2_4(&tr[4 * (size_t)init_id], &tr[4 * (size_t)i]);
-    bool sum_flag = false;
-    while ((int32_t)picked.size() < k) {
-        double dart = dart_dist(r);
-        const double sum = kpp_total(dist.data(), m);
-        if (sum < 1e-6) sum_flag = true;
-        dart *= sum;
-        double prefix = 0.0;
-        int32_t pivot = 0;
-        for (int32_t i = 0; i < m; ++i) {
-            pivot = i;
-            const double next = prefix + (double)dist[i];
-            if (dart >= prefix && dart < next) break;
-            prefix = next;
-        }
-        if (std::find(picked.begin(), picked.end(), pivot) != picked.end() && !sum_flag) continue;
-        std::memcpy(medoids + 4 * picked.size(), &tr[4 * (size_t)pivot], 16);
-        picked.push_back(pivot);
-        for (int32_t i = 0; i < m; ++i) dist[i] = std::min(dist[i], l2_4(&tr[4 * (size_t)pivot], &tr[4 * (size_t)i]));
-    }
-    // Lloyd epochs: compute_dist + find_index + update_mediods (clustering.hpp:198-256)
-    std::vector<int32_t> idx(std::max(m, 1)), cnt(k);
-    std::vector<float> nm(4 * (size_t)k);
-    for (int32_t e = 0; e < epochs; ++e) {
-        for (int32_t i = 0; i < m; ++i) {
-            float best = HUGE_VALF;
-            int32_t bid = -1;
-            for (int32_t j = 0; j < k; ++j) {
-                const float v = l2_4(&tr[4 * (size_t)i], medoids + 4 * (size_t)j);
-                if (v < best) {
-                    best = v;
-                    bid = j;
-                }
-            }
-            idx[i] = bid;
-        }
-        std::fill(nm.begin(), nm.end(), 0.0f);
-        std::fill(cnt.begin(), cnt.end(), 0);
-        for (int32_t i = 0; i < m; ++i) {
-            cnt[idx[i]]++;
-            for (int j = 0; j < 4; ++j) nm[4 * (size_t)idx[i] + j] += tr[4 * (size_t)i + j];
-        }
-        for (int32_t c = 0; c < k; ++c) {
-            if (cnt[c] == 0) continue;
-            const float rcp = 1.0f / (float)cnt[c];
-            for (int j = 0; j < 4; ++j) nm[4 * (size_t)c + j] *= rcp;
-        }
-        std::copy(nm.begin(), nm.end(), medoids);
-    }
-}
-
-// One Clustering per sub-space, sub-spaces spread over host threads.
 void train(const float *data, int64_t n, int32_t dim, std::vector<float> &codebook) {
     const int32_t nsq = dim / 4;
     codebook.assign((size_t)nsq * 256 * 4, 0.0f);
@@ -645,7 +556,8 @@ void train(const float *data, int64_t n, int32_t dim, std::vector<float> &codebo
         std::vector<float> rows(4 * (size_t)n);
         for (int32_t sq; (sq = next.fetch_add(1)) < nsq;) {
             for (int64_t i = 0; i < n; ++i) std::memcpy(&rows[4 * (size_t)i], data + i * dim + 4 * sq, 16);
-            fit4(rows.data(), (int32_t)n, 256, 10, &codebook[(size_t)sq * 256 * 4]);
+            km::fit(rows.data(), (int32_t)n, 4, 256, 10, true, &codebook[(size_t)sq * 256 * 4], km::host_backend,
+                    nullptr);
         }
     };
     std::vector<std::thread> th;
@@ -707,18 +619,6 @@ int train_quant(glass_hip_searcher &s, const float *data_dev, int64_t n) {
             break;
     }
     return GLASS_HIP_OK;
-}
-
-// GenRandom (glass/utils.hpp:13-27) with a default-seeded mt19937: the rows
-// GraphSearcher::SetData samples for Optimize.
-void gen_random(std::mt19937 &rng, int32_t *addr, int32_t size, int32_t N) {
-    for (int32_t i = 0; i < size; ++i) addr[i] = (int32_t)(rng() % (uint32_t)(N - size));
-    std::sort(addr, addr + size);
-    for (int32_t i = 1; i < size; ++i) {
-        if (addr[i] <= addr[i - 1]) addr[i] = addr[i - 1] + 1;
-    }
-    int32_t off = (int32_t)(rng() % (uint32_t)N);
-    for (int32_t i = 0; i < size; ++i) addr[i] = (addr[i] + off) % N;
 }
 
 int ensure_stream(glass_hip_searcher &s) {
@@ -1443,6 +1343,38 @@ int glass_hip_searcher_get_calibration(const glass_hip_searcher *s, float *param
         *n = 0;
     }
     return GLASS_HIP_OK;
+}
+
+int glass_hip_quant_train(const char *quantizer, const char *metric, const float *data, int32_t n, int32_t dim,
+                          float *params, int32_t *nparams) {
+    if (!quantizer || !nparams || dim <= 0) return fail(GLASS_HIP_EINVAL, "bad quantizer arguments");
+    const int32_t q = parse_quant(quantizer);
+    if (q == QK_NONE) return fail(GLASS_HIP_EINVAL, "unknown quantizer %s", quantizer);
+    if (q == QK_PQ8 && dim % 4 != 0) return fail(GLASS_HIP_EINVAL, "PQ8 needs a dimension divisible by 4");
+    glass_hip_searcher t;
+    t.quant = q;
+    t.metric = (metric && std::string(metric) == "IP") ? MK_IP : MK_L2;
+    t.dim = dim;
+    quant_layout(q, dim, &t.dim_align, &t.code_size);
+    if (!params) {  // size query: the layout of glass_hip_searcher_get_calibration
+        if (q == QK_PQ8) *nparams = (dim / 4) * 256 * 4;
+        else if (q == QK_SQ8 || q == QK_SQ6 || q == QK_SQ4) *nparams = 2 * t.dim_align;
+        else if (q == QK_SQ8U && t.metric == MK_IP) *nparams = 1;
+        else if (q == QK_SQ8U || q == QK_SQ4U || q == QK_SQ2U) *nparams = 2;
+        else *nparams = 0;
+        return GLASS_HIP_OK;
+    }
+    if (!data || n <= 0) return fail(GLASS_HIP_EINVAL, "no training data");
+    if (q == QK_PQ8) {
+        pq::train(data, n, dim, t.pq_codebook);
+    } else if (q != QK_FP32 && q != QK_FP16 && q != QK_BF16 && q != QK_E5M2) {
+        if (int r = ensure_stream(t)) return r;
+        DevBuf<float> raw;
+        HIP_TRY(raw.alloc((size_t)n * dim));
+        HIP_TRY(hipMemcpy(raw.p, data, sizeof(float) * (size_t)n * dim, hipMemcpyHostToDevice));
+        if (int r = train_quant(t, raw.p, n)) return r;
+    }
+    return glass_hip_searcher_get_calibration(&t, params, nparams);
 }
 
 int glass_hip_searcher_sync(glass_hip_searcher *s) {

@@ -6,6 +6,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <random>
+
 namespace glass_hip {
 
 // Quantizer kinds; the numbering follows glass/quant/quant.hpp:23 (QuantizerType).
@@ -95,5 +97,43 @@ int kernel_vgprs(const DeviceIndex &ix, const LaunchConfig &lc);
 // code_size/4 with at most 16 dwords per lane).
 int32_t default_team(const DeviceIndex &ix);
 bool team_valid(const DeviceIndex &ix, int32_t team);
+
+// Sets the calling thread's last-error text (glass_hip_last_error) and returns code.
+int set_error(int code, const char *msg);
+
+// GenRandom (glass/utils.hpp:13-27): size sorted, distinct, rotated draws in [0, N).
+void gen_random(std::mt19937 &rng, int32_t *addr, int32_t size, int32_t N);
+
+// k-means as glass::Clustering::fit runs it (glass/algorithms/clustering.hpp),
+// over sample rows the backend evaluates distances for.
+namespace km {
+
+// helpa::l2_fp32_fp32 in the reference's compiled order, on the host.
+float l2(const float *x, const float *y, int32_t d);
+
+struct Backend {
+    virtual ~Backend() = default;
+    // out[i] = l2(row, sample i) for every sample (k-means++ seeding)
+    virtual int dist_to(const float *row, float *out) = 0;
+    // ids[i] = first nearest of the k centroids to sample i (a Lloyd step)
+    virtual int assign(const float *cent, int32_t k, int32_t *ids) = 0;
+};
+
+// Clustering::fit on n rows of d floats: samples min(n - 1, 100000) rows with
+// GenRandom, seeds k centroids (kmeans++ or random), runs `epochs` Lloyd
+// steps.  `make` builds the backend over the sampled rows.  Returns 0 or an
+// error code (the backend's).
+int fit(const float *rows, int32_t n, int32_t d, int32_t k, int32_t epochs, bool kmeanspp, float *centroids,
+        Backend *(*make)(const float *samples, int32_t m, int32_t d, void *ctx), void *ctx);
+
+// A backend evaluating on the host (small d, e.g. PQ8's 4-dim sub-spaces).
+Backend *host_backend(const float *samples, int32_t m, int32_t d, void *ctx);
+
+}  // namespace km
+
+hipError_t launch_cluster_assign(const float *x, int32_t n, int32_t d, const float *cent, int32_t k, int32_t *ids,
+                                 hipStream_t stream);
+hipError_t launch_cluster_dist(const float *x, int32_t n, int32_t d, const float *pivot, float *out,
+                               hipStream_t stream);
 
 }  // namespace glass_hip

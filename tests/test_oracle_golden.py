@@ -455,3 +455,44 @@ def test_pq8_search_matches_reference(spq, dsetsp, golden_dir, gname, refine):
             sd = np.stack([s.search(QQ[i], k)[1] for i in range(nq)])
             np.testing.assert_array_equal(si, spq[f"{key}/single_ids"], err_msg=key)
             np.testing.assert_array_equal(sd, spq[f"{key}/single_dists"], err_msg=key)
+
+
+# Clustering (tests/golden/make_golden_clustering.py)
+
+CL_DIMS = [4, 7, 12, 16, 30, 100, 128]
+
+
+@pytest.fixture(scope="module")
+def zcl(golden_dir):
+    return dict(np.load(os.path.join(golden_dir, "clustering.npz")))
+
+
+@pytest.mark.parametrize("init", ["random", "kmeans++"])
+@pytest.mark.parametrize("d", CL_DIMS)
+def test_clustering_matches_reference(zcl, d, init):
+    X = synth.gaussian(1500, d, synth.BASE_SEED)
+    P = synth.gaussian(200, d, synth.QUERY_SEED)
+    assert sha(X) == str(zcl[f"{d}/data_sha"])
+    c = O.Clustering(64, 10, init).fit(X)
+    key = f"{d}/{init}"
+    np.testing.assert_array_equal(c.centroids, zcl[f"{key}/centroids"])
+    np.testing.assert_array_equal(c.predict(P), zcl[f"{key}/predict"])
+    t = c.transform(P)
+    np.testing.assert_array_equal(t[:4], zcl[f"{key}/transform_head"])
+    assert sha(t) == str(zcl[f"{key}/transform_sha"])
+
+
+@pytest.mark.parametrize("d", [7, 30, 128])
+def test_clustering_is_lloyd_fixed_point_like(d):
+    """After the epochs each centroid is near the float64 mean of the sample
+    rows nearest to it (one more Lloyd step moves it little), and predict
+    agrees with a float64 brute-force nearest-centroid search."""
+    X = synth.gaussian(1500, d, synth.BASE_SEED)
+    c = O.Clustering(16, 20, "kmeans++").fit(X)
+    ids = c.predict(X)
+    d64 = ((X.astype(np.float64)[:, None, :] - c.centroids.astype(np.float64)[None]) ** 2).sum(-1)
+    assert (ids == d64.argmin(1)).mean() > 0.999
+    for j in range(16):
+        rows = X[ids == j].astype(np.float64)
+        if len(rows) > 20:
+            assert np.abs(rows.mean(0) - c.centroids[j]).max() < 0.2
