@@ -18,6 +18,11 @@ graph and data.
 
 Usage: python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu] [--small]
 Prints one JSON line.
+
+`--quant Q[:REFINE]` measures another quantizer (FP16, BF16, SQ6, SQ4, PQ8,
+...) on the same workload with the reference's default re-rank depth: QPS at
+recall 0.90 / 0.95 over the ef sweep, the kernel's algorithmic-bytes roofline,
+and the reference CPU searcher with the same quantizer beside it.
 """
 
 from __future__ import annotations
@@ -360,6 +365,85 @@ def cpu_baseline(graph_path, X, Q, gt, k, efs, factor, budget_s=12.0):
     return {"points": pts, "threads": nthreads, "ids": res_ids}
 
 
+def quant_mode(spec, gpath, X, Q, gt, k, args, pg, torch, dev):
+    """One quantizer on the headline workload (see the module docstring)."""
+    quant, _, refine = spec.partition(":")
+    nq = Q.shape[0]
+    t = time.time()
+    s = pg.Searcher(pg.Graph(gpath), X, "L2", quant, refine)
+    setup_s = time.time() - t
+    s.optimize()
+    info = s.info()
+    factor = info["refine_factor"]
+    log(f"quant {spec}: set_data {setup_s:.1f}s, info {info}")
+    q_dev = torch.from_numpy(Q).to(dev)
+    ids_dev = torch.empty((nq, k), dtype=torch.int32, device=dev)
+    d_dev = torch.empty((nq, k), dtype=torch.float32, device=dev)
+    cnt_dev = torch.empty((nq, 3), dtype=torch.int32, device=dev)
+    stream = torch.cuda.Stream(device=dev)
+    pts = gpu_sweep(s, q_dev, nq, k, gt, ids_dev, d_dev, cnt_dev, args, torch, stream, factor, EFS)
+    curve = [(p["recall"], p["qps"]) for p in pts]
+    q90, q95 = qps_at_recall(curve, 0.90), qps_at_recall(curve, 0.95)
+    op = next((p for p in pts if p["recall"] >= TARGET), max(pts, key=lambda p: p["recall"]))
+    # algorithmic bytes per launch: code rows, adjacency rows, upper-layer
+    # lists, re-rank rows, query and results; PQ8 adds its lookup table
+    # (written once per query, one 4-byte entry read per code byte)
+    s_code, r_bytes, R = info["code_size"], info["rerank_row_bytes"], info["degree"]
+    cnt = op["cnt"]
+    rq = int(k * factor) if refine else 0
+    bq = cnt[:, 0] * s_code + cnt[:, 1] * 4 * R + cnt[:, 2] * 4 + rq * r_bytes + 4 * X.shape[1] + 8 * k
+    if quant == "PQ8":
+        bq = bq + s_code * 256 * 4 + cnt[:, 0] * s_code * 4
+    alg = float(bq.sum())
+    achieved = alg / (op["kernel_ms"] / 1e3) / 1e9
+    cpu = None
+    if not args.no_cpu:
+        from oracle import refwrap as R_
+
+        if R_.available(portable=True):
+            with stdout_to_stderr():
+                nthreads = os.cpu_count() or 1
+                R_.set_num_threads(nthreads, portable=True)
+                r = R_.RefSearcher(gpath, X, "L2", quant, refine, portable=True)
+                r.optimize()
+                cpts, eq, spent = [], [], 0.0
+                for p in pts:
+                    r.set_ef(p["ef"])
+                    best = None
+                    for run in range(3):
+                        t0 = time.perf_counter()
+                        ids, _ = r.batch_search(Q, k)
+                        dt = time.perf_counter() - t0
+                        spent += dt
+                        best = dt if best is None else min(best, dt)
+                        if run == 0:
+                            eq.append(float(np.mean(np.all(ids == p["ids"], axis=1))))
+                            rec = recall_at(ids, gt, k)
+                        if spent > 10.0:
+                            break
+                    cpts.append((rec, nq / best))
+                    log(f"cpu {spec}: ef={p['ef']} recall={rec:.4f} qps={nq / best:.0f}")
+                    if spent > 10.0:
+                        break
+                del r
+            cpu = {"qps_at_0.90": qps_at_recall(cpts, 0.90), "qps_at_0.95": qps_at_recall(cpts, 0.95),
+                   "points": cpts, "cores": nthreads, "kind": "reference", "ids_equal_gpu": min(eq) if eq else None}
+    out = {
+        "quantizer": quant, "refine": refine or None, "refine_factor": factor,
+        "workload": f"{X.shape[0] // 1000}k x {X.shape[1]} uint8-valued Gaussian mixture, HNSW R=32 L=200, L2, "
+                    f"k={k}, {nq} queries per step",
+        "qps_at_0.90": q90, "qps_at_0.95": q95, "max_recall": max(r for r, _ in curve),
+        "operating_point": {kk: v for kk, v in op.items() if kk not in ("ids", "cnt")},
+        "set_data_s": setup_s,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "algorithmic_bytes_per_launch": alg,
+                     "kernel_ms": op["kernel_ms"]},
+        "sweep": [{kk: v for kk, v in p.items() if kk not in ("ids", "cnt")} for p in pts],
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -378,6 +462,8 @@ def main():
                          "(profiler runs perturb the autotuner's timings)")
     ap.add_argument("--point", type=str, default=None,
                     help="FACTOR:EF - time only this operating point (for profiler runs)")
+    ap.add_argument("--quant", type=str, default=None,
+                    help="Q[:REFINE] - measure quantizer Q (with optional re-rank quantizer) instead of the headline")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU only; --gpus must be 1 "
@@ -404,6 +490,10 @@ def main():
     gt = exact_knn_gpu(X, Q, k, dev)
     nchk, nbad = fp64_check(X, Q, gt, k, dev, n=100)
     log(f"gt: exact kNN in {time.time() - t:.1f}s, fp64 recheck {nbad}/{nchk} mismatches")
+
+    if args.quant:
+        quant_mode(args.quant, gpath, X, Q, gt, k, args, pg, torch, dev)
+        return
 
     t = time.time()
     s = pg.Searcher(pg.Graph(gpath), X, "L2", "SQ4U", "FP32")
