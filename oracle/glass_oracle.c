@@ -20,6 +20,12 @@
  *                                of the compiled reference Searcher
  *   HNSW initializer             hand-built 3-level toy graph + golden vectors
  *   refiner                      total order (dist, id) + golden vectors
+ *   FP16 / BF16 / SQ6 / SQ4      golden codes, calibration and distances of the
+ *                                compiled reference; numpy encoders; float64
+ *                                closed forms of the decoded values
+ *   PQ8 + k-means (Clustering)   golden codebooks, codes, LUTs, centroids; the
+ *                                standard std::mt19937 known answer; float64
+ *                                sub-vector distances and LUT sums
  */
 #include "glass_oracle.h"
 
