@@ -19,6 +19,10 @@ graph and data.
 Usage: python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu] [--small]
 Prints one JSON line.
 
+`--clustering` times glass.Clustering (k-means++, 1,024 centroids, 10 epochs)
+fit and predict on the base rows, and the searcher's set-up (calibration and
+encoding on the device), each beside the compiled reference on the host cores.
+
 `--quant Q[:REFINE]` measures another quantizer (FP16, BF16, SQ6, SQ4, PQ8,
 ...) on the same workload with the reference's default re-rank depth: QPS at
 recall 0.90 / 0.95 over the ef sweep, the kernel's algorithmic-bytes roofline,
@@ -444,6 +448,44 @@ def quant_mode(spec, gpath, X, Q, gt, k, args, pg, torch, dev):
     print(json.dumps(out))
 
 
+def clustering_mode(gpath, X, Q, args, pg):
+    """k-means and set-up timings (see the module docstring)."""
+    from oracle import refwrap as R_
+
+    k, epochs = 1024, 10
+    out = {"workload": f"{X.shape[0] // 1000}k x {X.shape[1]} uint8-valued Gaussian mixture", "n_cluster": k,
+           "epochs": epochs, "init": "kmeans++"}
+    c = pg.Clustering(k, epochs, "kmeans++")
+    t = time.perf_counter()
+    c.fit(X)
+    out["gpu_fit_s"] = time.perf_counter() - t
+    t = time.perf_counter()
+    ids = c.predict(X)
+    out["gpu_predict_rows_per_s"] = X.shape[0] / (time.perf_counter() - t)
+    t = time.perf_counter()
+    s = pg.Searcher(pg.Graph(gpath), X, "L2", "SQ4U", "FP32")
+    out["gpu_set_data_s"] = time.perf_counter() - t
+    log(f"clustering: gpu fit {out['gpu_fit_s']:.2f}s, predict {out['gpu_predict_rows_per_s']:.0f} rows/s, "
+        f"set_data {out['gpu_set_data_s']:.2f}s")
+    del s
+    if not args.no_cpu and R_.available(portable=True):
+        with stdout_to_stderr():
+            R_.set_num_threads(os.cpu_count() or 1, portable=True)
+            t = time.perf_counter()
+            cent, rids, _ = R_.clustering_probe(X, k, epochs, "kmeans++", X[:100000], portable=True)
+            out["cpu_fit_and_predict_100k_s"] = time.perf_counter() - t
+            t = time.perf_counter()
+            r = R_.RefSearcher(gpath, X, "L2", "SQ4U", "FP32", portable=True)
+            out["cpu_set_data_s"] = time.perf_counter() - t
+            del r
+        out["centroids_equal_reference"] = bool(np.array_equal(cent, c.centroids()))
+        out["predict_equal_reference"] = float(np.mean(rids == ids[:100000]))
+        out["cpu_cores"] = os.cpu_count()
+        log(f"clustering: cpu fit+predict(100k) {out['cpu_fit_and_predict_100k_s']:.2f}s, "
+            f"set_data {out['cpu_set_data_s']:.2f}s, equal {out['centroids_equal_reference']}")
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -462,6 +504,8 @@ def main():
                          "(profiler runs perturb the autotuner's timings)")
     ap.add_argument("--point", type=str, default=None,
                     help="FACTOR:EF - time only this operating point (for profiler runs)")
+    ap.add_argument("--clustering", action="store_true",
+                    help="time Clustering fit / predict and the searcher set-up against the reference")
     ap.add_argument("--quant", type=str, default=None,
                     help="Q[:REFINE] - measure quantizer Q (with optional re-rank quantizer) instead of the headline")
     args = ap.parse_args()
@@ -493,6 +537,9 @@ def main():
 
     if args.quant:
         quant_mode(args.quant, gpath, X, Q, gt, k, args, pg, torch, dev)
+        return
+    if args.clustering:
+        clustering_mode(gpath, X, Q, args, pg)
         return
 
     t = time.time()

@@ -174,10 +174,11 @@ def pq_probe(metric: str, data: np.ndarray, queries: np.ndarray, ids: np.ndarray
     return cb, codes, lut, dist
 
 
-def clustering_probe(data: np.ndarray, n_cluster: int, epochs: int, init: str, pred: np.ndarray):
+def clustering_probe(data: np.ndarray, n_cluster: int, epochs: int, init: str, pred: np.ndarray,
+                     portable: bool = False):
     """Fit the reference glass::Clustering; return (centroids[n_cluster, d],
     predict(pred) ids, pred after the in-place transform)."""
-    lib = load(False)
+    lib = load(portable)
     data = np.ascontiguousarray(data, dtype=np.float32)
     pr = np.ascontiguousarray(pred, dtype=np.float32).copy()
     n, d = data.shape
