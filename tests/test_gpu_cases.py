@@ -218,15 +218,18 @@ def test_neighbour_gather_modes_match_oracle(c1, quant, metric, team, gather):
         assert s.get_last_search_dist_cmps() == o.get_last_search_dist_cmps()
 
 
-def test_graph_replay_matches_eager(c1):
+@pytest.mark.parametrize("quant,refine", [("SQ4U", "FP32"), ("PQ8", "FP32"), ("SQ6", ""), ("FP16", "")])
+def test_graph_replay_matches_eager(c1, quant, refine):
     # search_batch_dev captures (counter-reset kernel + search kernel) after
     # the first eager call and replays the graph while the arguments repeat;
     # every replay is checked against the eager result, with the outputs
-    # cleared before it and eager searches + synchronisations in between
+    # cleared before it and eager searches + synchronisations in between.
+    # PQ8 rebuilds each query's lookup table in the slot scratch inside the
+    # replayed launch.
     import torch
 
     X, Q, path = c1
-    s = pg.Searcher(pg.Graph(path), X, "L2", "SQ4U", "FP32")
+    s = pg.Searcher(pg.Graph(path), X, "L2", quant, refine)
     dev = torch.device("cuda:0")
     st = torch.cuda.Stream(device=dev)
     q = torch.from_numpy(Q[:500].copy()).to(dev)
@@ -236,7 +239,7 @@ def test_graph_replay_matches_eager(c1):
         s.set_ef(ef)
         I_ref, D_ref = s.batch_search(Q[:500], 10)
         for rep in range(5):
-            print(f"graph replay ef={ef} rep={rep}", flush=True)
+            print(f"graph replay {quant} ef={ef} rep={rep}", flush=True)
             with torch.cuda.stream(st):
                 ids.fill_(-7)
                 dis.fill_(0.0)
