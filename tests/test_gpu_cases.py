@@ -71,8 +71,12 @@ def test_short_pool_pads_like_oracle(tmp_path):
 
 
 @pytest.mark.parametrize("n", [240, 600])
-@pytest.mark.parametrize("quant,metric,refine", [("SQ4U", "L2", "FP32"), ("SQ8U", "IP", ""), ("SQ8", "L2", "")])
+@pytest.mark.parametrize("quant,metric,refine", [("SQ4U", "L2", "FP32"), ("SQ8U", "IP", ""), ("SQ8", "L2", ""),
+                                                 ("SQ6", "L2", ""), ("SQ4", "IP", ""), ("PQ8", "L2", "FP32"),
+                                                 ("BF16", "IP", "")])
 def test_ef_at_least_n_matches_oracle(n, quant, metric, refine):
+    # PQ8 at d = 64: 16 sub-spaces (the 16-block of the LUT sum); with n = 240
+    # its k-means has fewer sample rows than centroids (k-means++ repeats picks)
     # n=240 keeps the pool in registers (capacity <= 256); n=600 takes the LDS merge
     X = mixture(n, 64, 31)
     Q = mixture(40, 64, 32)
@@ -105,7 +109,8 @@ def c1(golden_dir):
     return X, Q, path
 
 
-@pytest.mark.parametrize("quant,refine", [("SQ8U", ""), ("SQ4U", "FP32"), ("E5M2", "")])
+@pytest.mark.parametrize("quant,refine", [("SQ8U", ""), ("SQ4U", "FP32"), ("E5M2", ""), ("PQ8", "FP32"), ("SQ6", ""),
+                                         ("BF16", "")])
 def test_per_query_ef_k_matches_oracle(c1, quant, refine):
     X, Q, path = c1
     nq = 300
@@ -125,15 +130,17 @@ def test_per_query_ef_k_matches_oracle(c1, quant, refine):
         assert cnt[i, 1] == o.last_counts[1][0]
 
 
+@pytest.mark.parametrize("quant", ["SQ8U", "PQ8"])
 @pytest.mark.parametrize("entries", [256, 512])
-def test_visited_overflow_to_global_bitset(c1, entries):
+def test_visited_overflow_to_global_bitset(c1, entries, quant):
     # a small LDS table forces most queries past half occupancy onto the
-    # slot's global bitset part-way through the search
+    # slot's global bitset part-way through the search; PQ8's lookup table
+    # sits in front of that bitset in the same slot scratch
     X, Q, path = c1
-    s = pg.Searcher(pg.Graph(path), X, "L2", "SQ8U")
-    o = O.Searcher(O.Graph(path), X, "L2", "SQ8U")
+    s = pg.Searcher(pg.Graph(path), X, "L2", quant)
+    o = O.Searcher(O.Graph(path), X, "L2", quant)
     s.set_visited_capacity(entries)
-    print(f"visited overflow entries={entries}", flush=True)
+    print(f"visited overflow {quant} entries={entries}", flush=True)
     for ef in [64, 200]:
         s.set_ef(ef)
         o.set_ef(ef)
