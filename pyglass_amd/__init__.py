@@ -447,6 +447,8 @@ class _QuantTrainer:
 
     def __init__(self, dim: int):
         self.dim = int(dim)
+        if self.dim <= 0:
+            raise GlassError(f"dim must be positive (got {dim})")
         self._params = np.zeros(0, np.float32)
 
     def train(self, data):

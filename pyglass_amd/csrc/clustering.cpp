@@ -306,9 +306,9 @@ void glass_hip_clustering_free(glass_hip_clustering *c) { delete c; }
 
 int glass_hip_clustering_fit(glass_hip_clustering *c, const float *data, int32_t n, int32_t dim) {
     if (!c || !data || dim <= 0) return set_error(GLASS_HIP_EINVAL, "bad clustering input");
-    // the fit samples n - 1 rows; random seeding draws k distinct ones of them
-    if (n < 2 || (!c->kmeanspp && n - 1 < c->k))
-        return set_error(GLASS_HIP_EINVAL, "too few rows for the number of clusters");
+    // the fit samples min(n - 1, 100000) rows; random seeding draws k distinct ones of them
+    if (n < 2 || (!c->kmeanspp && std::min(n - 1, 100000) < c->k))
+        return set_error(GLASS_HIP_EINVAL, "too few sampled rows for the number of clusters");
     std::vector<float> cent((size_t)c->k * dim);
     DeviceCtx ctx{c->k};
     const int r = km::fit(data, n, dim, c->k, c->epochs, c->kmeanspp, cent.data(), device_backend, &ctx);

@@ -122,3 +122,19 @@ def test_errors_are_raised_not_crashes(tmp_path):
     g = pg.Graph.from_arrays(np.full((4, 2), -1, np.int32), [0])
     with pytest.raises(pg.GlassError, match="quantizer"):
         pg.Searcher(g, np.zeros((4, 8), np.float32), "L2", "SQ8P")
+
+
+def test_clustering_argument_errors_without_a_gpu():
+    # checked before any device work
+    with pytest.raises(pg.GlassError, match="init"):
+        pg.Clustering(8, 10, "other")
+    c = pg.Clustering(8, 10, "random")
+    with pytest.raises(pg.GlassError, match="not fitted"):
+        c.predict(np.zeros((4, 3), np.float32))
+    with pytest.raises(pg.GlassError, match="too few"):
+        c.fit(np.zeros((5, 3), np.float32))  # random seeding draws k of the n - 1 sampled rows
+    big = pg.Clustering(100001, 1, "random")
+    with pytest.raises(pg.GlassError, match="too few"):
+        big.fit(np.zeros((200000, 1), np.float32))  # the sample is capped at 100,000 rows
+    with pytest.raises(pg.GlassError, match="dim"):
+        pg.quant_sq8(0)
