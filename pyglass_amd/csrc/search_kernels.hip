@@ -1078,6 +1078,31 @@ __device__ uint32_t pool_merge_reg(const DeviceIndex &ix, const WaveLds &w, Pool
     const uint64_t am = cm & (ps.size < ps.cap ? ~0ull : ballot(kj < last));  // the lanes' act bits
     int new_size = ps.size, new_cur = cur0, np0 = 0x7fffffff;
     uint32_t id0 = 0;
+    if (am) {
+        // the next node before the rank loop, so that its adjacency row is
+        // requested as early as possible: the rank-0 candidate (least key,
+        // then least lane) lands at the pool's upper bound of its key (new
+        // keys go after equal old keys)
+        const uint32_t key0 = uni(wave_min_u32(((am >> l) & 1) ? kj : kKeyMax));
+        const int i0 = __builtin_ctzll(am & ballot(kj == key0));
+        int ub0 = 0;
+#pragma unroll
+        for (int c = 0; c < NCH; ++c) ub0 += __builtin_popcountll(ballot(pkc[c] <= key0));
+        id0 = (uint32_t)__builtin_amdgcn_readlane((int)ij, i0);
+        np0 = ub0 < ps.cap ? ub0 : 0x7fffffff;
+        const int ma = __builtin_popcountll(am);
+        new_size = ps.size + ma < ps.cap ? ps.size + ma : ps.cap;
+        new_cur = cur0 < ps.size ? cur0 + __builtin_popcountll(am & ballot(kj < kc)) : 0x7fffffff;
+        new_cur = new_cur < np0 ? new_cur : np0;
+        new_cur = new_cur < new_size ? new_cur : new_size;
+    }
+    // the next pop takes entry new_cur: the rank-0 candidate or the old first
+    // unchecked entry
+    uint32_t next = kIdMask;
+    new_size = uni(new_size);
+    new_cur = uni(new_cur);
+    if (new_cur < new_size && new_cur < ps.ef) next = uni((new_cur == np0) ? id0 : idc);
+    if (next != kIdMask) pred_row = nj < ix.K ? ix.adj[(int64_t)next * ix.K + nj] : -1;
     int r = 0, ub = 0;
     // le[c]: active candidates whose key is >= pool entry 64c + l; the
     // entry's shift in the merge is the number of the others, ma - le[c]
@@ -1086,8 +1111,8 @@ __device__ uint32_t pool_merge_reg(const DeviceIndex &ix, const WaveLds &w, Pool
     for (int c = 0; c < NCH; ++c) le[c] = 0;
     if (am) {
         // rank among the active candidates by (key, insertion order) -- one
-        // 64-bit compare of (key, lane) pairs -- and upper bound in the pool
-        // (new keys go after equal old keys)
+        // 64-bit compare of (key, lane) pairs -- and each candidate's upper
+        // bound in the pool
         const uint64_t kl = ((uint64_t)kj << 32) | (uint32_t)l;
         uint64_t mm = am;
         while (mm) {
@@ -1106,24 +1131,7 @@ __device__ uint32_t pool_merge_reg(const DeviceIndex &ix, const WaveLds &w, Pool
             }
             ub = (l == i) ? cnt : ub;
         }
-        // masks of plain compares, combined on the scalar unit
-        const int i0 = __builtin_ctzll(am & ballot(r == 0));
-        const int ub0 = __builtin_amdgcn_readlane(ub, i0);
-        id0 = (uint32_t)__builtin_amdgcn_readlane((int)ij, i0);
-        np0 = ub0 < ps.cap ? ub0 : 0x7fffffff;
-        const int ma = __builtin_popcountll(am);
-        new_size = ps.size + ma < ps.cap ? ps.size + ma : ps.cap;
-        new_cur = cur0 < ps.size ? cur0 + __builtin_popcountll(am & ballot(kj < kc)) : 0x7fffffff;
-        new_cur = new_cur < np0 ? new_cur : np0;
-        new_cur = new_cur < new_size ? new_cur : new_size;
     }
-    // the next pop takes entry new_cur: the rank-0 candidate or the old first
-    // unchecked entry
-    uint32_t next = kIdMask;
-    new_size = uni(new_size);
-    new_cur = uni(new_cur);
-    if (new_cur < new_size && new_cur < ps.ef) next = uni((new_cur == np0) ? id0 : idc);
-    if (next != kIdMask) pred_row = nj < ix.K ? ix.adj[(int64_t)next * ix.K + nj] : -1;
     if (am) {
         const int lo = np0 < ps.cap ? np0 : ps.cap;  // entries before the first insertion do not move
         // shift of each pool entry = number of candidates with a smaller key
