@@ -44,7 +44,7 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-EFS = [16, 24, 32, 48, 64, 96, 128, 192, 256]
+EFS = [16, 24, 32, 48, 56, 64, 96, 128, 192, 256]
 FACTORS = [4.0, 8.0]
 TARGET = 0.95
 HBM_PEAK_GBS = 8000.0
@@ -504,11 +504,15 @@ def main():
                          "(profiler runs perturb the autotuner's timings)")
     ap.add_argument("--point", type=str, default=None,
                     help="FACTOR:EF - time only this operating point (for profiler runs)")
+    ap.add_argument("--factors", type=str, default=None,
+                    help="comma-separated re-rank factors to sweep (default: %s)" % ",".join(str(f) for f in FACTORS))
     ap.add_argument("--clustering", action="store_true",
                     help="time Clustering fit / predict and the searcher set-up against the reference")
     ap.add_argument("--quant", type=str, default=None,
                     help="Q[:REFINE] - measure quantizer Q (with optional re-rank quantizer) instead of the headline")
     args = ap.parse_args()
+    if args.factors:
+        FACTORS[:] = [float(f) for f in args.factors.split(",")]
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU only; --gpus must be 1 "
                                    "(queries are independent: multi-GPU is query sharding, see DESIGN.md)"}))
