@@ -515,7 +515,7 @@ def main():
         FACTORS[:] = [float(f) for f in args.factors.split(",")]
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU only; --gpus must be 1 "
-                                   "(queries are independent: multi-GPU is query sharding, see DESIGN.md)"}))
+                                   "(queries are independent: more GPUs would each search a share of the queries)"}))
         sys.exit(2)
 
     import torch
