@@ -55,7 +55,9 @@ def test_codebook_and_codes_match_oracle(dsets, golden_dir, metric, gname):
 
 
 CASES = [(g, "") for g in ["c1_hnsw_l2", "ip_hnsw", "nsg_l2", "g100_hnsw_l2"]] + [("c1_hnsw_l2", "FP32"),
-                                                                                    ("g100_hnsw_l2", "FP16")]
+                                                                                    ("g100_hnsw_l2", "FP16"),
+                                                                                    ("ip_hnsw", "FP32"),
+                                                                                    ("nsg_l2", "FP16")]
 
 
 @pytest.mark.parametrize("gname,refine", CASES)
