@@ -504,6 +504,8 @@ def main():
                          "(profiler runs perturb the autotuner's timings)")
     ap.add_argument("--point", type=str, default=None,
                     help="FACTOR:EF - time only this operating point (for profiler runs)")
+    ap.add_argument("--efs", type=str, default=None,
+                    help="comma-separated ef values to sweep (default: %s)" % ",".join(str(e) for e in EFS))
     ap.add_argument("--factors", type=str, default=None,
                     help="comma-separated re-rank factors to sweep (default: %s)" % ",".join(str(f) for f in FACTORS))
     ap.add_argument("--clustering", action="store_true",
@@ -513,6 +515,8 @@ def main():
     args = ap.parse_args()
     if args.factors:
         FACTORS[:] = [float(f) for f in args.factors.split(",")]
+    if args.efs:
+        EFS[:] = [int(e) for e in args.efs.split(",")]
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU only; --gpus must be 1 "
                                    "(queries are independent: more GPUs would each search a share of the queries)"}))
