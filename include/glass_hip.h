@@ -26,9 +26,10 @@ enum {
     GLASS_HIP_OK = 0,
     GLASS_HIP_EINVAL = -1,   /* bad argument (unknown metric/quantizer, shape, id range) */
     GLASS_HIP_EIO = -2,      /* graph file unreadable or malformed */
-    GLASS_HIP_ENOMEM = -3,   /* device allocation failed */
+    GLASS_HIP_ENOMEM = -3,   /* device or host allocation failed */
     GLASS_HIP_EDEVICE = -4,  /* HIP runtime / kernel launch error, or no device */
-    GLASS_HIP_ESTATE = -5    /* call out of order (e.g. search before set_data) */
+    GLASS_HIP_ESTATE = -5,   /* call out of order (e.g. search before set_data) */
+    GLASS_HIP_EINTERNAL = -6 /* unexpected C++ exception inside the library */
 };
 
 /* Message of the last failing call on this thread. */

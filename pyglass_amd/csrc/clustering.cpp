@@ -178,6 +178,7 @@ int fit(const float *rows, int32_t n, int32_t d, int32_t k, int32_t epochs, bool
         std::fill(nm.begin(), nm.end(), 0.0f);
         std::fill(cnt.begin(), cnt.end(), 0);
         for (int32_t i = 0; i < m; ++i) {
+            if (idx[i] < 0) continue;  // no centroid compares less than +inf (NaN rows)
             cnt[idx[i]]++;
             float *to = &nm[(size_t)d * idx[i]];
             const float *from = &tr[(size_t)d * i];
@@ -290,63 +291,84 @@ struct glass_hip_clustering {
 extern "C" {
 
 int glass_hip_clustering_create(int32_t n_cluster, int32_t epochs, const char *init, glass_hip_clustering **out) {
-    if (!out || n_cluster < 1 || epochs < 0) return set_error(GLASS_HIP_EINVAL, "bad clustering arguments");
-    const std::string in = init ? init : "random";
-    if (in != "random" && in != "kmeans++")
-        return set_error(GLASS_HIP_EINVAL, "init must be \"random\" or \"kmeans++\"");
-    auto *c = new glass_hip_clustering;
-    c->k = n_cluster;
-    c->epochs = epochs;
-    c->kmeanspp = in == "kmeans++";
-    *out = c;
-    return GLASS_HIP_OK;
+    try {
+        if (!out || n_cluster < 1 || epochs < 0) return set_error(GLASS_HIP_EINVAL, "bad clustering arguments");
+        const std::string in = init ? init : "random";
+        if (in != "random" && in != "kmeans++")
+            return set_error(GLASS_HIP_EINVAL, "init must be \"random\" or \"kmeans++\"");
+        auto *c = new glass_hip_clustering;
+        c->k = n_cluster;
+        c->epochs = epochs;
+        c->kmeanspp = in == "kmeans++";
+        *out = c;
+        return GLASS_HIP_OK;
+    } catch (...) {
+        return exception_status();
+    }
 }
 
 void glass_hip_clustering_free(glass_hip_clustering *c) { delete c; }
 
 int glass_hip_clustering_fit(glass_hip_clustering *c, const float *data, int32_t n, int32_t dim) {
-    if (!c || !data || dim <= 0) return set_error(GLASS_HIP_EINVAL, "bad clustering input");
-    // the fit samples min(n - 1, 100000) rows; random seeding draws k distinct ones of them
-    if (n < 2 || (!c->kmeanspp && std::min(n - 1, 100000) < c->k))
-        return set_error(GLASS_HIP_EINVAL, "too few sampled rows for the number of clusters");
-    std::vector<float> cent((size_t)c->k * dim);
-    DeviceCtx ctx{c->k};
-    const int r = km::fit(data, n, dim, c->k, c->epochs, c->kmeanspp, cent.data(), device_backend, &ctx);
-    if (r) return ctx.error ? ctx.error : r;
-    c->centroids.swap(cent);
-    c->dim = dim;
-    return GLASS_HIP_OK;
+    try {
+        if (!c || !data || dim <= 0) return set_error(GLASS_HIP_EINVAL, "bad clustering input");
+        // the fit samples min(n - 1, 100000) rows; random seeding draws k distinct ones of them
+        if (n < 2 || (!c->kmeanspp && std::min(n - 1, 100000) < c->k))
+            return set_error(GLASS_HIP_EINVAL, "too few sampled rows for the number of clusters");
+        std::vector<float> cent((size_t)c->k * dim);
+        DeviceCtx ctx{c->k};
+        const int r = km::fit(data, n, dim, c->k, c->epochs, c->kmeanspp, cent.data(), device_backend, &ctx);
+        if (r) return ctx.error ? ctx.error : r;
+        c->centroids.swap(cent);
+        c->dim = dim;
+        return GLASS_HIP_OK;
+    } catch (...) {
+        return exception_status();
+    }
 }
 
 int glass_hip_clustering_predict(const glass_hip_clustering *c, const float *data, int32_t n, int32_t dim,
                                  int32_t *ids) {
-    if (!c || !data || !ids || n < 0) return set_error(GLASS_HIP_EINVAL, "bad clustering input");
-    if (c->dim == 0) return set_error(GLASS_HIP_ESTATE, "clustering not fitted");
-    if (dim != c->dim) {
-        char msg[96];
-        std::snprintf(msg, sizeof(msg), "data has dim %d, the centroids %d", dim, c->dim);
-        return set_error(GLASS_HIP_EINVAL, msg);
+    try {
+        if (!c || !data || !ids || n < 0) return set_error(GLASS_HIP_EINVAL, "bad clustering input");
+        if (c->dim == 0) return set_error(GLASS_HIP_ESTATE, "clustering not fitted");
+        if (dim != c->dim) {
+            char msg[96];
+            std::snprintf(msg, sizeof(msg), "data has dim %d, the centroids %d", dim, c->dim);
+            return set_error(GLASS_HIP_EINVAL, msg);
+        }
+        if (n == 0) return GLASS_HIP_OK;
+        return predict_device(c->centroids.data(), c->k, dim, data, n, ids);
+    } catch (...) {
+        return exception_status();
     }
-    if (n == 0) return GLASS_HIP_OK;
-    return predict_device(c->centroids.data(), c->k, dim, data, n, ids);
 }
 
 int glass_hip_clustering_transform(const glass_hip_clustering *c, float *data, int32_t n, int32_t dim) {
-    std::vector<int32_t> ids(std::max(n, 1));
-    if (int r = glass_hip_clustering_predict(c, data, n, dim, ids.data())) return r;
-    for (int32_t i = 0; i < n; ++i) {
-        const float *ctr = &c->centroids[(size_t)dim * ids[i]];
-        for (int32_t j = 0; j < dim; ++j) data[(int64_t)dim * i + j] -= ctr[j];
+    try {
+        std::vector<int32_t> ids(std::max(n, 1));
+        if (int r = glass_hip_clustering_predict(c, data, n, dim, ids.data())) return r;
+        for (int32_t i = 0; i < n; ++i) {
+            if (ids[i] < 0) continue;  // a row with no nearest centroid is left as it is
+            const float *ctr = &c->centroids[(size_t)dim * ids[i]];
+            for (int32_t j = 0; j < dim; ++j) data[(int64_t)dim * i + j] -= ctr[j];
+        }
+        return GLASS_HIP_OK;
+    } catch (...) {
+        return exception_status();
     }
-    return GLASS_HIP_OK;
 }
 
 int glass_hip_clustering_centroids(const glass_hip_clustering *c, float *out, int32_t *n_cluster, int32_t *dim) {
-    if (!c) return set_error(GLASS_HIP_EINVAL, "null clustering");
-    if (n_cluster) *n_cluster = c->k;
-    if (dim) *dim = c->dim;
-    if (out && c->dim) std::copy(c->centroids.begin(), c->centroids.end(), out);
-    return GLASS_HIP_OK;
+    try {
+        if (!c) return set_error(GLASS_HIP_EINVAL, "null clustering");
+        if (n_cluster) *n_cluster = c->k;
+        if (dim) *dim = c->dim;
+        if (out && c->dim) std::copy(c->centroids.begin(), c->centroids.end(), out);
+        return GLASS_HIP_OK;
+    } catch (...) {
+        return exception_status();
+    }
 }
 
 }  // extern "C"

@@ -19,8 +19,10 @@
 #include <fstream>
 #include <memory>
 #include <mutex>
+#include <new>
 #include <numeric>
 #include <random>
+#include <stdexcept>
 #include <string>
 #include <thread>
 #include <unordered_map>
@@ -152,6 +154,20 @@ float refine_factor_of(int32_t kind) {
 }  // namespace
 
 int glass_hip::set_error(int code, const char *msg) { return fail(code, "%s", msg); }
+
+int glass_hip::exception_status() noexcept {
+    try {
+        throw;
+    } catch (const std::bad_alloc &) {
+        return fail(GLASS_HIP_ENOMEM, "host allocation failed");
+    } catch (const std::length_error &e) {
+        return fail(GLASS_HIP_EINVAL, "size too large: %s", e.what());
+    } catch (const std::exception &e) {
+        return fail(GLASS_HIP_EINTERNAL, "internal error: %s", e.what());
+    } catch (...) {
+        return fail(GLASS_HIP_EINTERNAL, "internal error: unknown exception");
+    }
+}
 
 // GenRandom (glass/utils.hpp:13-27) with a default-seeded mt19937: the rows
 // GraphSearcher::SetData samples for Optimize, and Clustering's samples.
@@ -859,366 +875,430 @@ int glass_hip_device_count(void) {
 }
 
 int glass_hip_graph_load(const char *path, const char *format, glass_hip_graph **out) {
-    if (!path || !out) return fail(GLASS_HIP_EINVAL, "null argument");
-    std::string fmt = format ? format : "glass";
-    auto g = std::make_unique<glass_hip_graph>();
-    int r;
-    if (fmt == "glass") r = load_glass(path, *g);
-    else if (fmt == "nsg") r = load_nsg(path, *g);
-    else if (fmt == "diskann") r = load_diskann(path, *g);
-    else if (fmt == "hnswlib") r = load_hnswlib(path, *g);
-    else return fail(GLASS_HIP_EINVAL, "unknown graph format '%s'", fmt.c_str());
-    if (r) return r;
-    *out = g.release();
-    return GLASS_HIP_OK;
+    try {
+        if (!path || !out) return fail(GLASS_HIP_EINVAL, "null argument");
+        std::string fmt = format ? format : "glass";
+        auto g = std::make_unique<glass_hip_graph>();
+        int r;
+        if (fmt == "glass") r = load_glass(path, *g);
+        else if (fmt == "nsg") r = load_nsg(path, *g);
+        else if (fmt == "diskann") r = load_diskann(path, *g);
+        else if (fmt == "hnswlib") r = load_hnswlib(path, *g);
+        else return fail(GLASS_HIP_EINVAL, "unknown graph format '%s'", fmt.c_str());
+        if (r) return r;
+        *out = g.release();
+        return GLASS_HIP_OK;
+    } catch (...) {
+        return exception_status();
+    }
 }
 
 int glass_hip_graph_save(const glass_hip_graph *g, const char *path) {
-    if (!g || !path) return fail(GLASS_HIP_EINVAL, "null argument");
-    std::ofstream f(path, std::ios::binary);
-    if (!f) return fail(GLASS_HIP_EIO, "cannot write %s", path);
-    int32_t nep = (int32_t)g->eps.size();
-    f.write((const char *)&nep, 4);
-    f.write((const char *)g->eps.data(), 4 * (size_t)nep);
-    f.write((const char *)&g->N, 4);
-    f.write((const char *)&g->K, 4);
-    f.write((const char *)g->adj.data(), (std::streamsize)(4 * (size_t)g->N * g->K));
-    if (g->has_upper) {  // HNSWInitializer::save (HNSWInitializer.hpp:77-86)
+    try {
+        if (!g || !path) return fail(GLASS_HIP_EINVAL, "null argument");
+        std::ofstream f(path, std::ios::binary);
+        if (!f) return fail(GLASS_HIP_EIO, "cannot write %s", path);
+        int32_t nep = (int32_t)g->eps.size();
+        f.write((const char *)&nep, 4);
+        f.write((const char *)g->eps.data(), 4 * (size_t)nep);
         f.write((const char *)&g->N, 4);
-        f.write((const char *)&g->up_K, 4);
-        f.write((const char *)&g->ep, 4);
-        for (int32_t i = 0; i < g->N; ++i) {
-            int32_t cur = g->levels[i] * g->up_K;
-            f.write((const char *)&cur, 4);
-            if (cur) f.write((const char *)(g->up_lists.data() + g->up_off[i]), 4 * (size_t)cur);
+        f.write((const char *)&g->K, 4);
+        f.write((const char *)g->adj.data(), (std::streamsize)(4 * (size_t)g->N * g->K));
+        if (g->has_upper) {  // HNSWInitializer::save (HNSWInitializer.hpp:77-86)
+            f.write((const char *)&g->N, 4);
+            f.write((const char *)&g->up_K, 4);
+            f.write((const char *)&g->ep, 4);
+            for (int32_t i = 0; i < g->N; ++i) {
+                int32_t cur = g->levels[i] * g->up_K;
+                f.write((const char *)&cur, 4);
+                if (cur) f.write((const char *)(g->up_lists.data() + g->up_off[i]), 4 * (size_t)cur);
+            }
         }
+        if (!f) return fail(GLASS_HIP_EIO, "write to %s failed", path);
+        return GLASS_HIP_OK;
+    } catch (...) {
+        return exception_status();
     }
-    if (!f) return fail(GLASS_HIP_EIO, "write to %s failed", path);
-    return GLASS_HIP_OK;
 }
 
 int glass_hip_graph_create(int32_t n, int32_t k, const int32_t *adj, int32_t nep, const int32_t *eps,
                            glass_hip_graph **out) {
-    if (n <= 0 || k <= 0 || !adj || nep < 0 || (nep && !eps) || !out) return fail(GLASS_HIP_EINVAL, "bad graph arrays");
-    auto g = std::make_unique<glass_hip_graph>();
-    g->N = n;
-    g->K = k;
-    g->adj.assign(adj, adj + (size_t)n * k);
-    g->eps.assign(eps, eps + nep);
-    *out = g.release();
-    return GLASS_HIP_OK;
+    try {
+        if (n <= 0 || k <= 0 || !adj || nep < 0 || (nep && !eps) || !out) return fail(GLASS_HIP_EINVAL, "bad graph arrays");
+        auto g = std::make_unique<glass_hip_graph>();
+        g->N = n;
+        g->K = k;
+        g->adj.assign(adj, adj + (size_t)n * k);
+        g->eps.assign(eps, eps + nep);
+        *out = g.release();
+        return GLASS_HIP_OK;
+    } catch (...) {
+        return exception_status();
+    }
 }
 
 int glass_hip_graph_set_upper(glass_hip_graph *g, int32_t up_k, int32_t ep, const int32_t *levels,
                               const int32_t *lists) {
-    if (!g || up_k <= 0 || !levels) return fail(GLASS_HIP_EINVAL, "bad upper-layer arrays");
-    g->has_upper = true;
-    g->up_K = up_k;
-    g->ep = ep;
-    g->levels.assign(levels, levels + g->N);
-    g->up_off.assign(g->N, 0);
-    int64_t total = 0;
-    for (int32_t i = 0; i < g->N; ++i) {
-        if (levels[i] < 0) return fail(GLASS_HIP_EINVAL, "negative level");
-        g->up_off[i] = total;
-        total += (int64_t)levels[i] * up_k;
+    try {
+        if (!g || up_k <= 0 || !levels) return fail(GLASS_HIP_EINVAL, "bad upper-layer arrays");
+        g->has_upper = true;
+        g->up_K = up_k;
+        g->ep = ep;
+        g->levels.assign(levels, levels + g->N);
+        g->up_off.assign(g->N, 0);
+        int64_t total = 0;
+        for (int32_t i = 0; i < g->N; ++i) {
+            if (levels[i] < 0) return fail(GLASS_HIP_EINVAL, "negative level");
+            g->up_off[i] = total;
+            total += (int64_t)levels[i] * up_k;
+        }
+        if (total && !lists) return fail(GLASS_HIP_EINVAL, "missing upper lists");
+        g->up_lists.assign(lists, lists + total);
+        return GLASS_HIP_OK;
+    } catch (...) {
+        return exception_status();
     }
-    if (total && !lists) return fail(GLASS_HIP_EINVAL, "missing upper lists");
-    g->up_lists.assign(lists, lists + total);
-    return GLASS_HIP_OK;
 }
 
 int glass_hip_graph_info(const glass_hip_graph *g, int32_t *n, int32_t *k, int32_t *nep, int32_t *has_upper) {
-    if (!g) return fail(GLASS_HIP_EINVAL, "null graph");
-    if (n) *n = g->N;
-    if (k) *k = g->K;
-    if (nep) *nep = (int32_t)g->eps.size();
-    if (has_upper) *has_upper = g->has_upper ? 1 : 0;
-    return GLASS_HIP_OK;
+    try {
+        if (!g) return fail(GLASS_HIP_EINVAL, "null graph");
+        if (n) *n = g->N;
+        if (k) *k = g->K;
+        if (nep) *nep = (int32_t)g->eps.size();
+        if (has_upper) *has_upper = g->has_upper ? 1 : 0;
+        return GLASS_HIP_OK;
+    } catch (...) {
+        return exception_status();
+    }
 }
 
 int glass_hip_graph_copy(const glass_hip_graph *g, int32_t *adj, int32_t *eps) {
-    if (!g) return fail(GLASS_HIP_EINVAL, "null graph");
-    if (adj) std::copy(g->adj.begin(), g->adj.end(), adj);
-    if (eps) std::copy(g->eps.begin(), g->eps.end(), eps);
-    return GLASS_HIP_OK;
+    try {
+        if (!g) return fail(GLASS_HIP_EINVAL, "null graph");
+        if (adj) std::copy(g->adj.begin(), g->adj.end(), adj);
+        if (eps) std::copy(g->eps.begin(), g->eps.end(), eps);
+        return GLASS_HIP_OK;
+    } catch (...) {
+        return exception_status();
+    }
 }
 
 void glass_hip_graph_free(glass_hip_graph *g) { delete g; }
 
 int glass_hip_build_graph(const char *index_type, const float *data, int32_t n, int32_t dim, const char *metric,
                           int32_t R, int32_t L, int32_t num_threads, glass_hip_graph **out, double *seconds) {
-    if (!index_type || !data || n <= 0 || dim <= 0 || !metric || !out || R < 2 || R > 64)
-        return fail(GLASS_HIP_EINVAL, "bad build arguments");
-    std::string t(index_type), m(metric);
-    if (m != "L2" && m != "IP") return fail(GLASS_HIP_EINVAL, "unknown metric '%s'", metric);
-    if (t == "NSG" || t == "SSG") {
-        // create_nsg (glass/nsg/nsg.hpp:411-434): the metric is ignored, graphs are L2
+    try {
+        if (!index_type || !data || n <= 0 || dim <= 0 || !metric || !out || R < 2 || R > 64)
+            return fail(GLASS_HIP_EINVAL, "bad build arguments");
+        std::string t(index_type), m(metric);
+        if (m != "L2" && m != "IP") return fail(GLASS_HIP_EINVAL, "unknown metric '%s'", metric);
+        if (t == "NSG" || t == "SSG") {
+            // create_nsg (glass/nsg/nsg.hpp:411-434): the metric is ignored, graphs are L2
+            auto g = std::make_unique<glass_hip_graph>();
+            int32_t ep = 0;
+            if (glass_build::build_nsg(data, n, dim, R, L, t == "SSG", num_threads, g->adj, ep, seconds))
+                return fail(GLASS_HIP_EINVAL, "%s build failed", index_type);
+            g->N = n;
+            g->K = R;
+            g->eps = {ep};
+            *out = g.release();
+            return GLASS_HIP_OK;
+        }
+        if (t != "HNSW") return fail(GLASS_HIP_EINVAL, "unsupported index type '%s'", index_type);
         auto g = std::make_unique<glass_hip_graph>();
+        std::vector<int32_t> levels, lists;
         int32_t ep = 0;
-        if (glass_build::build_nsg(data, n, dim, R, L, t == "SSG", num_threads, g->adj, ep, seconds))
-            return fail(GLASS_HIP_EINVAL, "%s build failed", index_type);
+        if (glass_build::build_hnsw(data, n, dim, m == "IP", R, L, num_threads, g->adj, ep, levels, lists, seconds))
+            return fail(GLASS_HIP_EINVAL, "HNSW build failed");
         g->N = n;
         g->K = R;
         g->eps = {ep};
+        if (int r = glass_hip_graph_set_upper(g.get(), R / 2, ep, levels.data(), lists.data())) return r;
         *out = g.release();
         return GLASS_HIP_OK;
+    } catch (...) {
+        return exception_status();
     }
-    if (t != "HNSW") return fail(GLASS_HIP_EINVAL, "unsupported index type '%s'", index_type);
-    auto g = std::make_unique<glass_hip_graph>();
-    std::vector<int32_t> levels, lists;
-    int32_t ep = 0;
-    if (glass_build::build_hnsw(data, n, dim, m == "IP", R, L, num_threads, g->adj, ep, levels, lists, seconds))
-        return fail(GLASS_HIP_EINVAL, "HNSW build failed");
-    g->N = n;
-    g->K = R;
-    g->eps = {ep};
-    if (int r = glass_hip_graph_set_upper(g.get(), R / 2, ep, levels.data(), lists.data())) return r;
-    *out = g.release();
-    return GLASS_HIP_OK;
 }
 
 int glass_hip_searcher_create(glass_hip_graph *g, const char *metric, const char *quantizer, const char *refine_quant,
                               glass_hip_searcher **out) {
-    if (!g || !metric || !quantizer || !out) return fail(GLASS_HIP_EINVAL, "null argument");
-    if (g->N <= 0) return fail(GLASS_HIP_EINVAL, "empty graph (was it moved into another searcher?)");
-    std::string m(metric), q(quantizer), r(refine_quant ? refine_quant : "");
-    int32_t mk;
-    if (m == "L2") mk = MK_L2;
-    else if (m == "IP") mk = MK_IP;
-    else return fail(GLASS_HIP_EINVAL, "unknown metric '%s'", metric);
-    int32_t qk = parse_quant(q);
-    if (qk == QK_NONE) return fail(GLASS_HIP_EINVAL, "unsupported quantizer '%s'", quantizer);
-    // make_refiner (graph_searcher.hpp:240-254): FP32 and FP16 re-rank rows
-    int32_t rk = QK_NONE;
-    if (!r.empty()) {
-        if (r == "FP32") rk = QK_FP32;
-        else if (r == "FP16") rk = QK_FP16;
-        else return fail(GLASS_HIP_EINVAL, "unsupported refine quantizer '%s'", refine_quant);
+    try {
+        if (!g || !metric || !quantizer || !out) return fail(GLASS_HIP_EINVAL, "null argument");
+        if (g->N <= 0) return fail(GLASS_HIP_EINVAL, "empty graph (was it moved into another searcher?)");
+        std::string m(metric), q(quantizer), r(refine_quant ? refine_quant : "");
+        int32_t mk;
+        if (m == "L2") mk = MK_L2;
+        else if (m == "IP") mk = MK_IP;
+        else return fail(GLASS_HIP_EINVAL, "unknown metric '%s'", metric);
+        int32_t qk = parse_quant(q);
+        if (qk == QK_NONE) return fail(GLASS_HIP_EINVAL, "unsupported quantizer '%s'", quantizer);
+        // make_refiner (graph_searcher.hpp:240-254): FP32 and FP16 re-rank rows
+        int32_t rk = QK_NONE;
+        if (!r.empty()) {
+            if (r == "FP32") rk = QK_FP32;
+            else if (r == "FP16") rk = QK_FP16;
+            else return fail(GLASS_HIP_EINVAL, "unsupported refine quantizer '%s'", refine_quant);
+        }
+        if (g->K > 64) return fail(GLASS_HIP_EINVAL, "graph degree %d > 64 is not supported", g->K);
+        if (g->has_upper && g->up_K > 64) return fail(GLASS_HIP_EINVAL, "upper-layer degree %d > 64", g->up_K);
+        if (!g->has_upper && g->eps.empty()) return fail(GLASS_HIP_EINVAL, "graph has no entry point");
+        if (!g->has_upper && g->eps.size() > 64) return fail(GLASS_HIP_EINVAL, "more than 64 entry points");
+        auto s = std::make_unique<glass_hip_searcher>();
+        s->graph = std::move(*g);
+        *g = glass_hip_graph();  // the caller's graph is left empty (bindings.cc:94-98)
+        s->metric = mk;
+        // SQ2U with IP instantiates the L2 searcher (graph_searcher.hpp:357)
+        s->quant = qk;
+        s->refine = rk;
+        if (int rr = sanitize_graph(s->graph)) return rr;
+        *out = s.release();
+        return GLASS_HIP_OK;
+    } catch (...) {
+        return exception_status();
     }
-    if (g->K > 64) return fail(GLASS_HIP_EINVAL, "graph degree %d > 64 is not supported", g->K);
-    if (g->has_upper && g->up_K > 64) return fail(GLASS_HIP_EINVAL, "upper-layer degree %d > 64", g->up_K);
-    if (!g->has_upper && g->eps.empty()) return fail(GLASS_HIP_EINVAL, "graph has no entry point");
-    if (!g->has_upper && g->eps.size() > 64) return fail(GLASS_HIP_EINVAL, "more than 64 entry points");
-    auto s = std::make_unique<glass_hip_searcher>();
-    s->graph = std::move(*g);
-    *g = glass_hip_graph();  // the caller's graph is left empty (bindings.cc:94-98)
-    s->metric = mk;
-    // SQ2U with IP instantiates the L2 searcher (graph_searcher.hpp:357)
-    s->quant = qk;
-    s->refine = rk;
-    if (int rr = sanitize_graph(s->graph)) return rr;
-    *out = s.release();
-    return GLASS_HIP_OK;
 }
 
 int glass_hip_searcher_set_data(glass_hip_searcher *s, const float *data, int32_t n, int32_t dim) {
-    if (!s || !data || n <= 0 || dim <= 0) return fail(GLASS_HIP_EINVAL, "bad data");
-    if (n != s->graph.N) return fail(GLASS_HIP_EINVAL, "data has %d rows but the graph has %d nodes", n, s->graph.N);
-    std::lock_guard<std::mutex> lk(s->mu);
-    if (int r = ensure_stream(*s)) return r;
-    if (s->quant == QK_PQ8 && dim % 4 != 0)
-        return fail(GLASS_HIP_EINVAL, "PQ8 needs a dimension divisible by 4 (got %d)", dim);
-    s->nb = n;
-    s->dim = dim;
-    quant_layout(s->quant, dim, &s->dim_align, &s->code_size);
-    s->lut_words = s->quant == QK_PQ8 ? (int64_t)s->code_size * 256 : 0;
-    s->nslots = 0;  // the slot stride depends on lut_words and N: recomputed at the next launch
-    int32_t r_row_bytes = 0;
-    if (s->refine != QK_NONE) quant_layout(s->refine, dim, &s->r_dim_align, &r_row_bytes);
-    // graph
-    const glass_hip_graph &g = s->graph;
-    HIP_TRY(s->d_adj.alloc(g.adj.size()));
-    HIP_TRY(hipMemcpy(s->d_adj.p, g.adj.data(), 4 * g.adj.size(), hipMemcpyHostToDevice));
-    HIP_TRY(s->d_eps.alloc(std::max<size_t>(1, g.eps.size())));
-    if (!g.eps.empty()) HIP_TRY(hipMemcpy(s->d_eps.p, g.eps.data(), 4 * g.eps.size(), hipMemcpyHostToDevice));
-    if (g.has_upper) {
-        HIP_TRY(s->d_levels.alloc(g.levels.size()));
-        HIP_TRY(hipMemcpy(s->d_levels.p, g.levels.data(), 4 * g.levels.size(), hipMemcpyHostToDevice));
-        HIP_TRY(s->d_up_off.alloc(g.up_off.size()));
-        HIP_TRY(hipMemcpy(s->d_up_off.p, g.up_off.data(), 8 * g.up_off.size(), hipMemcpyHostToDevice));
-        HIP_TRY(s->d_up_lists.alloc(std::max<size_t>(1, g.up_lists.size())));
-        if (!g.up_lists.empty())
-            HIP_TRY(hipMemcpy(s->d_up_lists.p, g.up_lists.data(), 4 * g.up_lists.size(), hipMemcpyHostToDevice));
+    try {
+        if (!s || !data || n <= 0 || dim <= 0) return fail(GLASS_HIP_EINVAL, "bad data");
+        if (n != s->graph.N) return fail(GLASS_HIP_EINVAL, "data has %d rows but the graph has %d nodes", n, s->graph.N);
+        std::lock_guard<std::mutex> lk(s->mu);
+        if (int r = ensure_stream(*s)) return r;
+        if (s->quant == QK_PQ8 && dim % 4 != 0)
+            return fail(GLASS_HIP_EINVAL, "PQ8 needs a dimension divisible by 4 (got %d)", dim);
+        s->nb = n;
+        s->dim = dim;
+        quant_layout(s->quant, dim, &s->dim_align, &s->code_size);
+        s->lut_words = s->quant == QK_PQ8 ? (int64_t)s->code_size * 256 : 0;
+        s->nslots = 0;  // the slot stride depends on lut_words and N: recomputed at the next launch
+        int32_t r_row_bytes = 0;
+        if (s->refine != QK_NONE) quant_layout(s->refine, dim, &s->r_dim_align, &r_row_bytes);
+        // graph
+        const glass_hip_graph &g = s->graph;
+        HIP_TRY(s->d_adj.alloc(g.adj.size()));
+        HIP_TRY(hipMemcpy(s->d_adj.p, g.adj.data(), 4 * g.adj.size(), hipMemcpyHostToDevice));
+        HIP_TRY(s->d_eps.alloc(std::max<size_t>(1, g.eps.size())));
+        if (!g.eps.empty()) HIP_TRY(hipMemcpy(s->d_eps.p, g.eps.data(), 4 * g.eps.size(), hipMemcpyHostToDevice));
+        if (g.has_upper) {
+            HIP_TRY(s->d_levels.alloc(g.levels.size()));
+            HIP_TRY(hipMemcpy(s->d_levels.p, g.levels.data(), 4 * g.levels.size(), hipMemcpyHostToDevice));
+            HIP_TRY(s->d_up_off.alloc(g.up_off.size()));
+            HIP_TRY(hipMemcpy(s->d_up_off.p, g.up_off.data(), 8 * g.up_off.size(), hipMemcpyHostToDevice));
+            HIP_TRY(s->d_up_lists.alloc(std::max<size_t>(1, g.up_lists.size())));
+            if (!g.up_lists.empty())
+                HIP_TRY(hipMemcpy(s->d_up_lists.p, g.up_lists.data(), 4 * g.up_lists.size(), hipMemcpyHostToDevice));
+        }
+        // base rows -> device, encode there
+        DevBuf<float> raw;
+        HIP_TRY(raw.alloc((size_t)n * dim));
+        HIP_TRY(hipMemcpy(raw.p, data, sizeof(float) * (size_t)n * dim, hipMemcpyHostToDevice));
+        if (int r = train_quant(*s, raw.p, n)) return r;
+        if (s->quant == QK_SQ8 || s->quant == QK_SQ6 || s->quant == QK_SQ4) {
+            HIP_TRY(s->d_mins.alloc(s->dim_align));
+            HIP_TRY(s->d_difs.alloc(s->dim_align));
+            HIP_TRY(hipMemcpy(s->d_mins.p, s->sq8_mins.data(), 4 * (size_t)s->dim_align, hipMemcpyHostToDevice));
+            HIP_TRY(hipMemcpy(s->d_difs.p, s->sq8_difs.data(), 4 * (size_t)s->dim_align, hipMemcpyHostToDevice));
+        }
+        HIP_TRY(s->d_codes.alloc((size_t)n * s->code_size));
+        if (s->quant == QK_FP32) {
+            HIP_TRY(launch_pad_rows(raw.p, n, dim, s->dim_align, (float *)s->d_codes.p, s->stream));
+        } else if (s->quant == QK_PQ8) {
+            // k-means on the host, as ProductQuant::train; nearest-centroid codes on the device
+            pq::train(data, n, dim, s->pq_codebook);
+            HIP_TRY(s->d_codebook.alloc(s->pq_codebook.size()));
+            HIP_TRY(hipMemcpy(s->d_codebook.p, s->pq_codebook.data(), 4 * s->pq_codebook.size(), hipMemcpyHostToDevice));
+            HIP_TRY(launch_pq_encode(raw.p, n, s->code_size, s->d_codebook.p, s->d_codes.p, s->stream));
+        } else {
+            HIP_TRY(launch_encode(raw.p, n, dim, s->code_size, s->quant, s->metric, s->qmin, s->qdif, s->qmax,
+                                  s->d_mins.p, s->d_difs.p, s->d_codes.p, s->stream));
+        }
+        if (s->refine == QK_FP32) {
+            HIP_TRY(s->d_rerank.alloc((size_t)n * r_row_bytes));
+            HIP_TRY(launch_pad_rows(raw.p, n, dim, s->r_dim_align, (float *)s->d_rerank.p, s->stream));
+        } else if (s->refine == QK_FP16) {  // FP16Quantizer::add, as for the search codes
+            HIP_TRY(s->d_rerank.alloc((size_t)n * r_row_bytes));
+            HIP_TRY(launch_encode(raw.p, n, dim, r_row_bytes, QK_FP16, s->metric, 0.0f, 0.0f, 0.0f, nullptr, nullptr,
+                                  s->d_rerank.p, s->stream));
+        }
+        HIP_TRY(hipStreamSynchronize(s->stream));
+        raw.release();
+        // Optimize sample (graph_searcher.hpp:62-69)
+        s->sample_points_num = std::min(1000, n - 1);
+        if (s->sample_points_num > 0) {
+            std::vector<int32_t> pts(s->sample_points_num);
+            std::mt19937 rng;
+            gen_random(rng, pts.data(), s->sample_points_num, n);
+            s->optimize_queries.resize((size_t)s->sample_points_num * dim);
+            for (int32_t i = 0; i < s->sample_points_num; ++i)
+                std::memcpy(s->optimize_queries.data() + (size_t)i * dim, data + (size_t)pts[i] * dim, 4 * (size_t)dim);
+        }
+        for (auto &t : s->tune) t.team = default_team(s->index());
+        s->has_data = true;
+        return GLASS_HIP_OK;
+    } catch (...) {
+        return exception_status();
     }
-    // base rows -> device, encode there
-    DevBuf<float> raw;
-    HIP_TRY(raw.alloc((size_t)n * dim));
-    HIP_TRY(hipMemcpy(raw.p, data, sizeof(float) * (size_t)n * dim, hipMemcpyHostToDevice));
-    if (int r = train_quant(*s, raw.p, n)) return r;
-    if (s->quant == QK_SQ8 || s->quant == QK_SQ6 || s->quant == QK_SQ4) {
-        HIP_TRY(s->d_mins.alloc(s->dim_align));
-        HIP_TRY(s->d_difs.alloc(s->dim_align));
-        HIP_TRY(hipMemcpy(s->d_mins.p, s->sq8_mins.data(), 4 * (size_t)s->dim_align, hipMemcpyHostToDevice));
-        HIP_TRY(hipMemcpy(s->d_difs.p, s->sq8_difs.data(), 4 * (size_t)s->dim_align, hipMemcpyHostToDevice));
-    }
-    HIP_TRY(s->d_codes.alloc((size_t)n * s->code_size));
-    if (s->quant == QK_FP32) {
-        HIP_TRY(launch_pad_rows(raw.p, n, dim, s->dim_align, (float *)s->d_codes.p, s->stream));
-    } else if (s->quant == QK_PQ8) {
-        // k-means on the host, as ProductQuant::train; nearest-centroid codes on the device
-        pq::train(data, n, dim, s->pq_codebook);
-        HIP_TRY(s->d_codebook.alloc(s->pq_codebook.size()));
-        HIP_TRY(hipMemcpy(s->d_codebook.p, s->pq_codebook.data(), 4 * s->pq_codebook.size(), hipMemcpyHostToDevice));
-        HIP_TRY(launch_pq_encode(raw.p, n, s->code_size, s->d_codebook.p, s->d_codes.p, s->stream));
-    } else {
-        HIP_TRY(launch_encode(raw.p, n, dim, s->code_size, s->quant, s->metric, s->qmin, s->qdif, s->qmax,
-                              s->d_mins.p, s->d_difs.p, s->d_codes.p, s->stream));
-    }
-    if (s->refine == QK_FP32) {
-        HIP_TRY(s->d_rerank.alloc((size_t)n * r_row_bytes));
-        HIP_TRY(launch_pad_rows(raw.p, n, dim, s->r_dim_align, (float *)s->d_rerank.p, s->stream));
-    } else if (s->refine == QK_FP16) {  // FP16Quantizer::add, as for the search codes
-        HIP_TRY(s->d_rerank.alloc((size_t)n * r_row_bytes));
-        HIP_TRY(launch_encode(raw.p, n, dim, r_row_bytes, QK_FP16, s->metric, 0.0f, 0.0f, 0.0f, nullptr, nullptr,
-                              s->d_rerank.p, s->stream));
-    }
-    HIP_TRY(hipStreamSynchronize(s->stream));
-    raw.release();
-    // Optimize sample (graph_searcher.hpp:62-69)
-    s->sample_points_num = std::min(1000, n - 1);
-    if (s->sample_points_num > 0) {
-        std::vector<int32_t> pts(s->sample_points_num);
-        std::mt19937 rng;
-        gen_random(rng, pts.data(), s->sample_points_num, n);
-        s->optimize_queries.resize((size_t)s->sample_points_num * dim);
-        for (int32_t i = 0; i < s->sample_points_num; ++i)
-            std::memcpy(s->optimize_queries.data() + (size_t)i * dim, data + (size_t)pts[i] * dim, 4 * (size_t)dim);
-    }
-    for (auto &t : s->tune) t.team = default_team(s->index());
-    s->has_data = true;
-    return GLASS_HIP_OK;
 }
 
 int glass_hip_searcher_set_ef(glass_hip_searcher *s, int32_t ef) {
-    if (!s || ef <= 0) return fail(GLASS_HIP_EINVAL, "ef must be positive");
-    s->ef = ef;
-    return GLASS_HIP_OK;
+    try {
+        if (!s || ef <= 0) return fail(GLASS_HIP_EINVAL, "ef must be positive");
+        s->ef = ef;
+        return GLASS_HIP_OK;
+    } catch (...) {
+        return exception_status();
+    }
 }
 
 int32_t glass_hip_searcher_get_ef(const glass_hip_searcher *s) { return s ? s->ef : 0; }
 
 int glass_hip_searcher_search(glass_hip_searcher *s, const float *q, int32_t k, int32_t *dst, float *scores) {
-    if (!s || !q || !dst || k <= 0) return fail(GLASS_HIP_EINVAL, "bad search arguments");
-    std::lock_guard<std::mutex> lk(s->mu);
-    // one query per launch: a single wave
-    const int32_t saved = s->blocks, saved_wpb = s->force_wpb;
-    s->blocks = 1;
-    s->force_wpb = 1;
-    int r = host_batch(*s, q, 1, nullptr, nullptr, k, MODE_SEARCH, dst, scores, nullptr, false);
-    s->blocks = saved;
-    s->force_wpb = saved_wpb;
-    return r;
+    try {
+        if (!s || !q || !dst || k <= 0) return fail(GLASS_HIP_EINVAL, "bad search arguments");
+        std::lock_guard<std::mutex> lk(s->mu);
+        // one query per launch: a single wave
+        const int32_t saved = s->blocks, saved_wpb = s->force_wpb;
+        s->blocks = 1;
+        s->force_wpb = 1;
+        int r = host_batch(*s, q, 1, nullptr, nullptr, k, MODE_SEARCH, dst, scores, nullptr, false);
+        s->blocks = saved;
+        s->force_wpb = saved_wpb;
+        return r;
+    } catch (...) {
+        return exception_status();
+    }
 }
 
 int glass_hip_searcher_search_batch(glass_hip_searcher *s, const float *qs, int32_t nq, int32_t k, int32_t *dst,
                                     float *scores) {
-    if (!s || (!qs && nq) || (!dst && nq) || nq < 0 || k <= 0) return fail(GLASS_HIP_EINVAL, "bad batch arguments");
-    std::lock_guard<std::mutex> lk(s->mu);
-    return host_batch(*s, qs, nq, nullptr, nullptr, k, MODE_BATCH, dst, scores, nullptr, true);
+    try {
+        if (!s || (!qs && nq) || (!dst && nq) || nq < 0 || k <= 0) return fail(GLASS_HIP_EINVAL, "bad batch arguments");
+        std::lock_guard<std::mutex> lk(s->mu);
+        return host_batch(*s, qs, nq, nullptr, nullptr, k, MODE_BATCH, dst, scores, nullptr, true);
+    } catch (...) {
+        return exception_status();
+    }
 }
 
 int glass_hip_searcher_search_batch_efk(glass_hip_searcher *s, const float *qs, int32_t nq, const int32_t *ef,
                                         const int32_t *k, int32_t *dst, float *scores, int32_t *counters) {
-    if (!s || nq < 0 || (nq && (!qs || !ef || !k || !dst))) return fail(GLASS_HIP_EINVAL, "bad batch arguments");
-    std::lock_guard<std::mutex> lk(s->mu);
-    return host_batch(*s, qs, nq, ef, k, 0, MODE_BATCH, dst, scores, counters, true);
+    try {
+        if (!s || nq < 0 || (nq && (!qs || !ef || !k || !dst))) return fail(GLASS_HIP_EINVAL, "bad batch arguments");
+        std::lock_guard<std::mutex> lk(s->mu);
+        return host_batch(*s, qs, nq, ef, k, 0, MODE_BATCH, dst, scores, counters, true);
+    } catch (...) {
+        return exception_status();
+    }
 }
 
 int glass_hip_searcher_search_batch_dev(glass_hip_searcher *s, const float *qs_dev, int32_t nq, int32_t k,
                                         int32_t mode, int32_t *dst_dev, float *scores_dev, int32_t *counters_dev,
                                         void *stream) {
-    if (!s || nq < 0 || k <= 0 || (nq && (!qs_dev || !dst_dev))) return fail(GLASS_HIP_EINVAL, "bad batch arguments");
-    if (mode != MODE_SEARCH && mode != MODE_BATCH) return fail(GLASS_HIP_EINVAL, "mode must be 0 or 1");
-    std::lock_guard<std::mutex> lk(s->mu);
-    const int32_t max_cap = std::max(inner_k(*s, k), s->ef);
-    if (round_cap(max_cap) > 4096) return fail(GLASS_HIP_EINVAL, "max(k, ef) = %d exceeds 4096", max_cap);
-    return run_batch(*s, qs_dev, nq, nullptr, nullptr, s->ef, k, max_cap, s->ef, mode, nullptr, dst_dev, scores_dev,
-                     counters_dev, (hipStream_t)stream, nullptr, s->graph_capture);
+    try {
+        if (!s || nq < 0 || k <= 0 || (nq && (!qs_dev || !dst_dev))) return fail(GLASS_HIP_EINVAL, "bad batch arguments");
+        if (mode != MODE_SEARCH && mode != MODE_BATCH) return fail(GLASS_HIP_EINVAL, "mode must be 0 or 1");
+        std::lock_guard<std::mutex> lk(s->mu);
+        const int32_t max_cap = std::max(inner_k(*s, k), s->ef);
+        if (round_cap(max_cap) > 4096) return fail(GLASS_HIP_EINVAL, "max(k, ef) = %d exceeds 4096", max_cap);
+        return run_batch(*s, qs_dev, nq, nullptr, nullptr, s->ef, k, max_cap, s->ef, mode, nullptr, dst_dev, scores_dev,
+                         counters_dev, (hipStream_t)stream, nullptr, s->graph_capture);
+    } catch (...) {
+        return exception_status();
+    }
 }
 
 int glass_hip_searcher_set_graph_capture(glass_hip_searcher *s, int32_t on) {
-    if (!s) return fail(GLASS_HIP_EINVAL, "null searcher");
-    std::lock_guard<std::mutex> lk(s->mu);
-    s->graph_capture = on != 0;
-    return GLASS_HIP_OK;
+    try {
+        if (!s) return fail(GLASS_HIP_EINVAL, "null searcher");
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->graph_capture = on != 0;
+        return GLASS_HIP_OK;
+    } catch (...) {
+        return exception_status();
+    }
 }
 
 int glass_hip_searcher_optimize(glass_hip_searcher *s, int32_t num_threads) {
-    (void)num_threads;  // unused by the reference too (graph_searcher.hpp:77-79)
-    if (!s) return fail(GLASS_HIP_EINVAL, "null searcher");
-    if (!s->has_data) return fail(GLASS_HIP_ESTATE, "optimize before set_data");
-    std::lock_guard<std::mutex> lk(s->mu);
-    if (s->sample_points_num <= 0) return GLASS_HIP_OK;
-    // GraphSearcher::Optimize (graph_searcher.hpp:76-129) times SearchBatch on
-    // the sampled base rows at k=10 over a grid of prefetch knobs and keeps
-    // the fastest.  Here the knobs are the launch shape (lanes per candidate,
-    // waves per workgroup, LDS visited-table size), tuned per ef bucket on the
-    // samples tiled to a device-filling batch.
-    const int32_t kTryK = 10;
-    const int32_t base = s->sample_points_num;
-    const int32_t nq = std::max(base, std::min(8192, 8 * base));
-    std::vector<float> qs((size_t)nq * s->dim);
-    for (int32_t i = 0; i < nq; ++i)
-        std::memcpy(qs.data() + (size_t)i * s->dim, s->optimize_queries.data() + (size_t)(i % base) * s->dim,
-                    4 * (size_t)s->dim);
-    std::vector<int32_t> teams;
-    for (int32_t t : {4, 8, 16}) {
-        if (team_valid(s->index(), t)) teams.push_back(t);
-    }
-    const int32_t wpbs[] = {1, 2, 4};
-    const int32_t vhs[] = {1024, 2048, 4096};
-    std::vector<int32_t> ids((size_t)nq * kTryK);
-    const int32_t saved_ef = s->ef;
-    const int32_t saved_vo = s->vis_override, saved_ft = s->force_team, saved_fw = s->force_wpb,
-                  saved_fs = s->force_spec;
-    const bool int_codes = s->quant == QK_SQ8U || s->quant == QK_SQ4U || s->quant == QK_SQ2U;
-    auto timeit = [&]() -> double {
-        auto t0 = std::chrono::high_resolution_clock::now();
-        int r = host_batch(*s, qs.data(), nq, nullptr, nullptr, kTryK, MODE_BATCH, ids.data(), nullptr, nullptr,
-                           false);
-        auto t1 = std::chrono::high_resolution_clock::now();
-        return r ? 1e30 : std::chrono::duration<double>(t1 - t0).count();
-    };
-    for (int b = 0; b < glass_hip_searcher::kBuckets - 1; ++b) {
-        s->ef = 16 << b;  // upper edge of the bucket
-        double best = 1e30;
-        glass_hip_searcher::Tune bt = s->tune[b];
-        for (int32_t t : teams) {
-            for (int32_t w : wpbs) {
-                for (int32_t v : vhs) {
-                    for (int32_t sp : {0, 1}) {
-                        if (sp && !(int_codes && spec_fits(*s, t))) continue;
-                        s->force_team = t;
-                        s->force_wpb = w;
-                        s->vis_override = v;
-                        s->force_spec = sp;
-                        const size_t lds = lds_bytes_per_wave(
-                                               s->index(), round_cap(std::max(inner_k(*s, kTryK), s->ef)), v) *
-                                           (size_t)w;
-                        if (lds > 160 * 1024) continue;
-                        timeit();
-                        double e = std::min(timeit(), timeit());
-                        if (e < best) {
-                            best = e;
-                            bt.team = t;
-                            bt.waves_per_block = w;
-                            bt.vis_hash = v;
-                            bt.spec = sp;
+    try {
+        (void)num_threads;  // unused by the reference too (graph_searcher.hpp:77-79)
+        if (!s) return fail(GLASS_HIP_EINVAL, "null searcher");
+        if (!s->has_data) return fail(GLASS_HIP_ESTATE, "optimize before set_data");
+        std::lock_guard<std::mutex> lk(s->mu);
+        if (s->sample_points_num <= 0) return GLASS_HIP_OK;
+        // GraphSearcher::Optimize (graph_searcher.hpp:76-129) times SearchBatch on
+        // the sampled base rows at k=10 over a grid of prefetch knobs and keeps
+        // the fastest.  Here the knobs are the launch shape (lanes per candidate,
+        // waves per workgroup, LDS visited-table size), tuned per ef bucket on the
+        // samples tiled to a device-filling batch.
+        const int32_t kTryK = 10;
+        const int32_t base = s->sample_points_num;
+        const int32_t nq = std::max(base, std::min(8192, 8 * base));
+        std::vector<float> qs((size_t)nq * s->dim);
+        for (int32_t i = 0; i < nq; ++i)
+            std::memcpy(qs.data() + (size_t)i * s->dim, s->optimize_queries.data() + (size_t)(i % base) * s->dim,
+                        4 * (size_t)s->dim);
+        std::vector<int32_t> teams;
+        for (int32_t t : {4, 8, 16}) {
+            if (team_valid(s->index(), t)) teams.push_back(t);
+        }
+        const int32_t wpbs[] = {1, 2, 4};
+        const int32_t vhs[] = {1024, 2048, 4096};
+        std::vector<int32_t> ids((size_t)nq * kTryK);
+        const int32_t saved_ef = s->ef;
+        const int32_t saved_vo = s->vis_override, saved_ft = s->force_team, saved_fw = s->force_wpb,
+                      saved_fs = s->force_spec;
+        const bool int_codes = s->quant == QK_SQ8U || s->quant == QK_SQ4U || s->quant == QK_SQ2U;
+        auto timeit = [&]() -> double {
+            auto t0 = std::chrono::high_resolution_clock::now();
+            int r = host_batch(*s, qs.data(), nq, nullptr, nullptr, kTryK, MODE_BATCH, ids.data(), nullptr, nullptr,
+                               false);
+            auto t1 = std::chrono::high_resolution_clock::now();
+            return r ? 1e30 : std::chrono::duration<double>(t1 - t0).count();
+        };
+        for (int b = 0; b < glass_hip_searcher::kBuckets - 1; ++b) {
+            s->ef = 16 << b;  // upper edge of the bucket
+            double best = 1e30;
+            glass_hip_searcher::Tune bt = s->tune[b];
+            for (int32_t t : teams) {
+                for (int32_t w : wpbs) {
+                    for (int32_t v : vhs) {
+                        for (int32_t sp : {0, 1}) {
+                            if (sp && !(int_codes && spec_fits(*s, t))) continue;
+                            s->force_team = t;
+                            s->force_wpb = w;
+                            s->vis_override = v;
+                            s->force_spec = sp;
+                            const size_t lds = lds_bytes_per_wave(
+                                                   s->index(), round_cap(std::max(inner_k(*s, kTryK), s->ef)), v) *
+                                               (size_t)w;
+                            if (lds > 160 * 1024) continue;
+                            timeit();
+                            double e = std::min(timeit(), timeit());
+                            if (e < best) {
+                                best = e;
+                                bt.team = t;
+                                bt.waves_per_block = w;
+                                bt.vis_hash = v;
+                                bt.spec = sp;
+                            }
                         }
                     }
                 }
             }
+            s->tune[b] = bt;
         }
-        s->tune[b] = bt;
+        s->tune[glass_hip_searcher::kBuckets - 1] = s->tune[glass_hip_searcher::kBuckets - 2];
+        s->ef = saved_ef;
+        s->vis_override = saved_vo;
+        s->force_team = saved_ft;
+        s->force_wpb = saved_fw;
+        s->force_spec = saved_fs;
+        return GLASS_HIP_OK;
+    } catch (...) {
+        return exception_status();
     }
-    s->tune[glass_hip_searcher::kBuckets - 1] = s->tune[glass_hip_searcher::kBuckets - 2];
-    s->ef = saved_ef;
-    s->vis_override = saved_vo;
-    s->force_team = saved_ft;
-    s->force_wpb = saved_fw;
-    s->force_spec = saved_fs;
-    return GLASS_HIP_OK;
 }
 
 double glass_hip_searcher_last_avg_dist_cmps(const glass_hip_searcher *s) { return s ? s->last_avg_dist_cmps : 0.0; }
@@ -1226,161 +1306,209 @@ double glass_hip_searcher_last_avg_dist_cmps(const glass_hip_searcher *s) { retu
 void glass_hip_searcher_free(glass_hip_searcher *s) { delete s; }
 
 int glass_hip_searcher_set_refine_factor(glass_hip_searcher *s, float factor) {
-    if (!s || !(factor >= 0.0f) || factor > 64.0f) return fail(GLASS_HIP_EINVAL, "refine factor must be in [0, 64]");
-    std::lock_guard<std::mutex> lk(s->mu);
-    s->refine_override = factor;
-    return GLASS_HIP_OK;
+    try {
+        if (!s || !(factor >= 0.0f) || factor > 64.0f) return fail(GLASS_HIP_EINVAL, "refine factor must be in [0, 64]");
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->refine_override = factor;
+        return GLASS_HIP_OK;
+    } catch (...) {
+        return exception_status();
+    }
 }
 
 int glass_hip_searcher_set_launch(glass_hip_searcher *s, int32_t team, int32_t waves_per_block, int32_t blocks) {
-    if (!s) return fail(GLASS_HIP_EINVAL, "null searcher");
-    if (waves_per_block < 0 || waves_per_block > 4 || blocks < 0) return fail(GLASS_HIP_EINVAL, "bad launch config");
-    if (team && s->has_data && !team_valid(s->index(), team))
-        return fail(GLASS_HIP_EINVAL, "team %d invalid for this layout", team);
-    std::lock_guard<std::mutex> lk(s->mu);
-    s->force_team = team;
-    s->force_wpb = waves_per_block;
-    s->blocks = blocks;
-    return GLASS_HIP_OK;
+    try {
+        if (!s) return fail(GLASS_HIP_EINVAL, "null searcher");
+        if (waves_per_block < 0 || waves_per_block > 4 || blocks < 0) return fail(GLASS_HIP_EINVAL, "bad launch config");
+        if (team && s->has_data && !team_valid(s->index(), team))
+            return fail(GLASS_HIP_EINVAL, "team %d invalid for this layout", team);
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->force_team = team;
+        s->force_wpb = waves_per_block;
+        s->blocks = blocks;
+        return GLASS_HIP_OK;
+    } catch (...) {
+        return exception_status();
+    }
 }
 
 int glass_hip_searcher_set_gather(glass_hip_searcher *s, int32_t mode) {
-    if (!s || mode < -1 || mode > 1) return fail(GLASS_HIP_EINVAL, "gather mode must be -1, 0 or 1");
-    std::lock_guard<std::mutex> lk(s->mu);
-    s->force_spec = mode;
-    return GLASS_HIP_OK;
+    try {
+        if (!s || mode < -1 || mode > 1) return fail(GLASS_HIP_EINVAL, "gather mode must be -1, 0 or 1");
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->force_spec = mode;
+        return GLASS_HIP_OK;
+    } catch (...) {
+        return exception_status();
+    }
 }
 
 int glass_hip_searcher_set_visited_capacity(glass_hip_searcher *s, int32_t entries) {
-    if (!s) return fail(GLASS_HIP_EINVAL, "null searcher");
-    if (entries != 0 && (entries < 256 || entries > 16384 || (entries & (entries - 1))))
-        return fail(GLASS_HIP_EINVAL, "visited table size must be 0 or a power of two in [256, 16384]");
-    std::lock_guard<std::mutex> lk(s->mu);
-    s->vis_override = entries;
-    return GLASS_HIP_OK;
+    try {
+        if (!s) return fail(GLASS_HIP_EINVAL, "null searcher");
+        if (entries != 0 && (entries < 256 || entries > 16384 || (entries & (entries - 1))))
+            return fail(GLASS_HIP_EINVAL, "visited table size must be 0 or a power of two in [256, 16384]");
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->vis_override = entries;
+        return GLASS_HIP_OK;
+    } catch (...) {
+        return exception_status();
+    }
 }
 
 int glass_hip_searcher_get_launch(const glass_hip_searcher *s, int32_t *team, int32_t *waves_per_block,
                                   int32_t *blocks, int32_t *slots) {
-    if (!s) return fail(GLASS_HIP_EINVAL, "null searcher");
-    const glass_hip_searcher::Tune &tn = s->tune[glass_hip_searcher::bucket(s->ef)];
-    if (team) *team = s->force_team ? s->force_team : tn.team;
-    if (waves_per_block) *waves_per_block = s->force_wpb ? s->force_wpb : tn.waves_per_block;
-    if (blocks) *blocks = s->last_launch.blocks;
-    // the table a batch at the current ef would use (as run_batch picks it)
-    if (slots) *slots = s->vis_override ? s->vis_override : (tn.vis_hash ? tn.vis_hash : glass_hip_searcher::auto_vis(s->ef));
-    return GLASS_HIP_OK;
+    try {
+        if (!s) return fail(GLASS_HIP_EINVAL, "null searcher");
+        const glass_hip_searcher::Tune &tn = s->tune[glass_hip_searcher::bucket(s->ef)];
+        if (team) *team = s->force_team ? s->force_team : tn.team;
+        if (waves_per_block) *waves_per_block = s->force_wpb ? s->force_wpb : tn.waves_per_block;
+        if (blocks) *blocks = s->last_launch.blocks;
+        // the table a batch at the current ef would use (as run_batch picks it)
+        if (slots) *slots = s->vis_override ? s->vis_override : (tn.vis_hash ? tn.vis_hash : glass_hip_searcher::auto_vis(s->ef));
+        return GLASS_HIP_OK;
+    } catch (...) {
+        return exception_status();
+    }
 }
 
 int glass_hip_searcher_kernel_info(const glass_hip_searcher *s, int32_t *vgprs, int32_t *lds_per_block,
                                    int32_t *blocks_per_cu) {
-    if (!s) return fail(GLASS_HIP_EINVAL, "null searcher");
-    if (!s->has_data || !s->last_launch.team) return fail(GLASS_HIP_ESTATE, "no launch yet");
-    const LaunchConfig &lc = s->last_launch;
-    if (vgprs) *vgprs = kernel_vgprs(s->index(), lc);
-    if (lds_per_block)
-        *lds_per_block = (int32_t)(lds_bytes_per_wave(s->index(), lc.cap_max, lc.vis_hash) * lc.waves_per_block);
-    if (blocks_per_cu) *blocks_per_cu = max_blocks_per_cu(s->index(), lc);
-    return GLASS_HIP_OK;
+    try {
+        if (!s) return fail(GLASS_HIP_EINVAL, "null searcher");
+        if (!s->has_data || !s->last_launch.team) return fail(GLASS_HIP_ESTATE, "no launch yet");
+        const LaunchConfig &lc = s->last_launch;
+        if (vgprs) *vgprs = kernel_vgprs(s->index(), lc);
+        if (lds_per_block)
+            *lds_per_block = (int32_t)(lds_bytes_per_wave(s->index(), lc.cap_max, lc.vis_hash) * lc.waves_per_block);
+        if (blocks_per_cu) *blocks_per_cu = max_blocks_per_cu(s->index(), lc);
+        return GLASS_HIP_OK;
+    } catch (...) {
+        return exception_status();
+    }
 }
 
 int glass_hip_device_info(char *name, int32_t name_len, int32_t *cu_count, int64_t *lds_per_cu, int64_t *hbm_bytes,
                           int32_t *clock_khz) {
-    int dev = 0;
-    HIP_TRY(hipGetDevice(&dev));
-    hipDeviceProp_t p;
-    HIP_TRY(hipGetDeviceProperties(&p, dev));
-    if (name && name_len > 0) {
-        std::snprintf(name, (size_t)name_len, "%s (%s)", p.name, p.gcnArchName);
+    try {
+        int dev = 0;
+        HIP_TRY(hipGetDevice(&dev));
+        hipDeviceProp_t p;
+        HIP_TRY(hipGetDeviceProperties(&p, dev));
+        if (name && name_len > 0) {
+            std::snprintf(name, (size_t)name_len, "%s (%s)", p.name, p.gcnArchName);
+        }
+        if (cu_count) *cu_count = p.multiProcessorCount;
+        if (lds_per_cu) *lds_per_cu = (int64_t)p.maxSharedMemoryPerMultiProcessor;
+        if (hbm_bytes) *hbm_bytes = (int64_t)p.totalGlobalMem;
+        if (clock_khz) *clock_khz = p.clockRate;
+        return GLASS_HIP_OK;
+    } catch (...) {
+        return exception_status();
     }
-    if (cu_count) *cu_count = p.multiProcessorCount;
-    if (lds_per_cu) *lds_per_cu = (int64_t)p.maxSharedMemoryPerMultiProcessor;
-    if (hbm_bytes) *hbm_bytes = (int64_t)p.totalGlobalMem;
-    if (clock_khz) *clock_khz = p.clockRate;
-    return GLASS_HIP_OK;
 }
 
 int glass_hip_searcher_info(const glass_hip_searcher *s, int32_t *code_size, int32_t *rerank_row_bytes,
                             int32_t *degree, int32_t *n, float *refine_factor) {
-    if (!s) return fail(GLASS_HIP_EINVAL, "null searcher");
-    if (code_size) *code_size = s->code_size;
-    if (rerank_row_bytes)
-        *rerank_row_bytes = s->refine == QK_FP32 ? 4 * s->r_dim_align : (s->refine == QK_FP16 ? 2 * s->r_dim_align : 0);
-    if (degree) *degree = s->graph.K;
-    if (n) *n = s->graph.N;
-    if (refine_factor) *refine_factor = s->refine != QK_NONE ? s->factor() : 1.0f;
-    return GLASS_HIP_OK;
+    try {
+        if (!s) return fail(GLASS_HIP_EINVAL, "null searcher");
+        if (code_size) *code_size = s->code_size;
+        if (rerank_row_bytes)
+            *rerank_row_bytes = s->refine == QK_FP32 ? 4 * s->r_dim_align : (s->refine == QK_FP16 ? 2 * s->r_dim_align : 0);
+        if (degree) *degree = s->graph.K;
+        if (n) *n = s->graph.N;
+        if (refine_factor) *refine_factor = s->refine != QK_NONE ? s->factor() : 1.0f;
+        return GLASS_HIP_OK;
+    } catch (...) {
+        return exception_status();
+    }
 }
 
 int glass_hip_searcher_get_codes(const glass_hip_searcher *s, uint8_t *codes) {
-    if (!s || !codes) return fail(GLASS_HIP_EINVAL, "null argument");
-    if (!s->has_data) return fail(GLASS_HIP_ESTATE, "no data");
-    HIP_TRY(hipMemcpy(codes, s->d_codes.p, (size_t)s->nb * s->code_size, hipMemcpyDeviceToHost));
-    return GLASS_HIP_OK;
+    try {
+        if (!s || !codes) return fail(GLASS_HIP_EINVAL, "null argument");
+        if (!s->has_data) return fail(GLASS_HIP_ESTATE, "no data");
+        HIP_TRY(hipMemcpy(codes, s->d_codes.p, (size_t)s->nb * s->code_size, hipMemcpyDeviceToHost));
+        return GLASS_HIP_OK;
+    } catch (...) {
+        return exception_status();
+    }
 }
 
 int glass_hip_searcher_get_calibration(const glass_hip_searcher *s, float *params, int32_t *n) {
-    if (!s || !n) return fail(GLASS_HIP_EINVAL, "null argument");
-    if (s->quant == QK_PQ8) {
-        *n = (int32_t)s->pq_codebook.size();
-        if (params) std::copy(s->pq_codebook.begin(), s->pq_codebook.end(), params);
-    } else if (s->quant == QK_SQ8 || s->quant == QK_SQ6 || s->quant == QK_SQ4) {
-        *n = 2 * s->dim_align;
-        if (params) {
-            std::copy(s->sq8_mins.begin(), s->sq8_mins.end(), params);
-            std::copy(s->sq8_difs.begin(), s->sq8_difs.end(), params + s->dim_align);
+    try {
+        if (!s || !n) return fail(GLASS_HIP_EINVAL, "null argument");
+        if (s->quant == QK_PQ8) {
+            *n = (int32_t)s->pq_codebook.size();
+            if (params) std::copy(s->pq_codebook.begin(), s->pq_codebook.end(), params);
+        } else if (s->quant == QK_SQ8 || s->quant == QK_SQ6 || s->quant == QK_SQ4) {
+            *n = 2 * s->dim_align;
+            if (params) {
+                std::copy(s->sq8_mins.begin(), s->sq8_mins.end(), params);
+                std::copy(s->sq8_difs.begin(), s->sq8_difs.end(), params + s->dim_align);
+            }
+        } else if (s->quant == QK_SQ8U && s->metric == MK_IP) {
+            *n = 1;
+            if (params) params[0] = s->qmax;
+        } else if (s->quant == QK_SQ8U || s->quant == QK_SQ4U || s->quant == QK_SQ2U) {
+            *n = 2;
+            if (params) {
+                params[0] = s->qmin;
+                params[1] = s->qdif;
+            }
+        } else {
+            *n = 0;
         }
-    } else if (s->quant == QK_SQ8U && s->metric == MK_IP) {
-        *n = 1;
-        if (params) params[0] = s->qmax;
-    } else if (s->quant == QK_SQ8U || s->quant == QK_SQ4U || s->quant == QK_SQ2U) {
-        *n = 2;
-        if (params) {
-            params[0] = s->qmin;
-            params[1] = s->qdif;
-        }
-    } else {
-        *n = 0;
+        return GLASS_HIP_OK;
+    } catch (...) {
+        return exception_status();
     }
-    return GLASS_HIP_OK;
 }
 
 int glass_hip_quant_train(const char *quantizer, const char *metric, const float *data, int32_t n, int32_t dim,
                           float *params, int32_t *nparams) {
-    if (!quantizer || !nparams || dim <= 0) return fail(GLASS_HIP_EINVAL, "bad quantizer arguments");
-    const int32_t q = parse_quant(quantizer);
-    if (q == QK_NONE) return fail(GLASS_HIP_EINVAL, "unknown quantizer %s", quantizer);
-    if (q == QK_PQ8 && dim % 4 != 0) return fail(GLASS_HIP_EINVAL, "PQ8 needs a dimension divisible by 4");
-    glass_hip_searcher t;
-    t.quant = q;
-    t.metric = (metric && std::string(metric) == "IP") ? MK_IP : MK_L2;
-    t.dim = dim;
-    quant_layout(q, dim, &t.dim_align, &t.code_size);
-    if (!params) {  // size query: the layout of glass_hip_searcher_get_calibration
-        if (q == QK_PQ8) *nparams = (dim / 4) * 256 * 4;
-        else if (q == QK_SQ8 || q == QK_SQ6 || q == QK_SQ4) *nparams = 2 * t.dim_align;
-        else if (q == QK_SQ8U && t.metric == MK_IP) *nparams = 1;
-        else if (q == QK_SQ8U || q == QK_SQ4U || q == QK_SQ2U) *nparams = 2;
-        else *nparams = 0;
-        return GLASS_HIP_OK;
+    try {
+        if (!quantizer || !nparams || dim <= 0) return fail(GLASS_HIP_EINVAL, "bad quantizer arguments");
+        const int32_t q = parse_quant(quantizer);
+        if (q == QK_NONE) return fail(GLASS_HIP_EINVAL, "unknown quantizer %s", quantizer);
+        if (q == QK_PQ8 && dim % 4 != 0) return fail(GLASS_HIP_EINVAL, "PQ8 needs a dimension divisible by 4");
+        glass_hip_searcher t;
+        t.quant = q;
+        t.metric = (metric && std::string(metric) == "IP") ? MK_IP : MK_L2;
+        t.dim = dim;
+        quant_layout(q, dim, &t.dim_align, &t.code_size);
+        if (!params) {  // size query: the layout of glass_hip_searcher_get_calibration
+            if (q == QK_PQ8) *nparams = (dim / 4) * 256 * 4;
+            else if (q == QK_SQ8 || q == QK_SQ6 || q == QK_SQ4) *nparams = 2 * t.dim_align;
+            else if (q == QK_SQ8U && t.metric == MK_IP) *nparams = 1;
+            else if (q == QK_SQ8U || q == QK_SQ4U || q == QK_SQ2U) *nparams = 2;
+            else *nparams = 0;
+            return GLASS_HIP_OK;
+        }
+        if (!data || n <= 0) return fail(GLASS_HIP_EINVAL, "no training data");
+        if (q == QK_PQ8) {
+            pq::train(data, n, dim, t.pq_codebook);
+        } else if (q != QK_FP32 && q != QK_FP16 && q != QK_BF16 && q != QK_E5M2) {
+            if (int r = ensure_stream(t)) return r;
+            DevBuf<float> raw;
+            HIP_TRY(raw.alloc((size_t)n * dim));
+            HIP_TRY(hipMemcpy(raw.p, data, sizeof(float) * (size_t)n * dim, hipMemcpyHostToDevice));
+            if (int r = train_quant(t, raw.p, n)) return r;
+        }
+        return glass_hip_searcher_get_calibration(&t, params, nparams);
+    } catch (...) {
+        return exception_status();
     }
-    if (!data || n <= 0) return fail(GLASS_HIP_EINVAL, "no training data");
-    if (q == QK_PQ8) {
-        pq::train(data, n, dim, t.pq_codebook);
-    } else if (q != QK_FP32 && q != QK_FP16 && q != QK_BF16 && q != QK_E5M2) {
-        if (int r = ensure_stream(t)) return r;
-        DevBuf<float> raw;
-        HIP_TRY(raw.alloc((size_t)n * dim));
-        HIP_TRY(hipMemcpy(raw.p, data, sizeof(float) * (size_t)n * dim, hipMemcpyHostToDevice));
-        if (int r = train_quant(t, raw.p, n)) return r;
-    }
-    return glass_hip_searcher_get_calibration(&t, params, nparams);
 }
 
 int glass_hip_searcher_sync(glass_hip_searcher *s) {
-    if (!s) return fail(GLASS_HIP_EINVAL, "null searcher");
-    if (s->stream) HIP_TRY(hipStreamSynchronize(s->stream));
-    return GLASS_HIP_OK;
+    try {
+        if (!s) return fail(GLASS_HIP_EINVAL, "null searcher");
+        if (s->stream) HIP_TRY(hipStreamSynchronize(s->stream));
+        return GLASS_HIP_OK;
+    } catch (...) {
+        return exception_status();
+    }
 }
 
 }  // extern "C"
@@ -1678,92 +1806,100 @@ extern "C" {
 
 int glass_hip_server_create(glass_hip_searcher *s, int32_t max_batch, int32_t max_k, int32_t max_inflight,
                             glass_hip_server **out) {
-    if (!s || !out || max_batch <= 0 || max_k <= 0 || max_inflight <= 0 || max_inflight > 8)
-        return fail(GLASS_HIP_EINVAL, "bad server arguments");
-    if (!s->has_data) return fail(GLASS_HIP_ESTATE, "server before set_data");
-    auto sv = std::make_unique<glass_hip_server>();
-    sv->s = s;
-    sv->max_batch = max_batch;
-    sv->max_k = max_k;
-    sv->max_inflight = max_inflight;
-    sv->dim = s->dim;
-    sv->in_meta = 16 * (size_t)max_batch;
-    sv->in_bytes = sv->in_meta + sizeof(float) * (size_t)max_batch * s->dim;
-    sv->out_bytes = 8 * (size_t)max_batch * max_k;
-    {
-        std::lock_guard<std::mutex> lk(s->mu);
-        if (int r = ensure_stream(*s)) return r;
-        // waves one batch can have resident: the most any launch shape allows
-        LaunchConfig lc{};
-        int32_t most = 0;
-        for (int32_t team : {4, 8, 16}) {
-            if (!team_valid(s->index(), team)) continue;
-            lc.team = team;
-            for (int32_t w = 1; w <= 4; ++w) {
-                // both kernel paths (gather on / off, small / large pools)
-                for (int32_t sp = 0; sp <= 1; ++sp) {
-                    for (int32_t cap : {64, 129}) {
-                        lc.waves_per_block = w;
-                        lc.cap_max = cap;
-                        lc.vis_hash = 256;
-                        lc.spec = sp;
-                        most = std::max(most,
-                                        std::max(1, max_blocks_per_cu(s->index(), lc)) * std::max(1, s->cu_count) * w);
+    try {
+        if (!s || !out || max_batch <= 0 || max_k <= 0 || max_inflight <= 0 || max_inflight > 8)
+            return fail(GLASS_HIP_EINVAL, "bad server arguments");
+        if (!s->has_data) return fail(GLASS_HIP_ESTATE, "server before set_data");
+        auto sv = std::make_unique<glass_hip_server>();
+        sv->s = s;
+        sv->max_batch = max_batch;
+        sv->max_k = max_k;
+        sv->max_inflight = max_inflight;
+        sv->dim = s->dim;
+        sv->in_meta = 16 * (size_t)max_batch;
+        sv->in_bytes = sv->in_meta + sizeof(float) * (size_t)max_batch * s->dim;
+        sv->out_bytes = 8 * (size_t)max_batch * max_k;
+        {
+            std::lock_guard<std::mutex> lk(s->mu);
+            if (int r = ensure_stream(*s)) return r;
+            // waves one batch can have resident: the most any launch shape allows
+            LaunchConfig lc{};
+            int32_t most = 0;
+            for (int32_t team : {4, 8, 16}) {
+                if (!team_valid(s->index(), team)) continue;
+                lc.team = team;
+                for (int32_t w = 1; w <= 4; ++w) {
+                    // both kernel paths (gather on / off, small / large pools)
+                    for (int32_t sp = 0; sp <= 1; ++sp) {
+                        for (int32_t cap : {64, 129}) {
+                            lc.waves_per_block = w;
+                            lc.cap_max = cap;
+                            lc.vis_hash = 256;
+                            lc.spec = sp;
+                            most = std::max(most,
+                                            std::max(1, max_blocks_per_cu(s->index(), lc)) * std::max(1, s->cu_count) * w);
+                        }
                     }
                 }
             }
+            if (most == 0) most = 32 * std::max(1, s->cu_count);  // 8 waves per SIMD, 4 SIMDs per CU
+            sv->slots_per_batch = most;
+            if (int r = ensure_slots(*s, 1)) return r;  // sets the bitset stride
+            // a query clears its wave's bitset when it first needs it
+            HIP_TRY(sv->visited.alloc((size_t)max_inflight * (size_t)most * (size_t)s->vis_stride));
         }
-        if (most == 0) most = 32 * std::max(1, s->cu_count);  // 8 waves per SIMD, 4 SIMDs per CU
-        sv->slots_per_batch = most;
-        if (int r = ensure_slots(*s, 1)) return r;  // sets the bitset stride
-        // a query clears its wave's bitset when it first needs it
-        HIP_TRY(sv->visited.alloc((size_t)max_inflight * (size_t)most * (size_t)s->vis_stride));
+        for (int32_t i = 0; i < max_inflight; ++i) {
+            sv->slots.push_back(std::make_unique<ServerSlot>());
+            ServerSlot &sl = *sv->slots.back();
+            HIP_TRY(hipHostMalloc((void **)&sl.h_in, sv->in_bytes, hipHostMallocDefault));
+            HIP_TRY(hipHostMalloc((void **)&sl.h_out, sv->out_bytes, hipHostMallocMapped | hipHostMallocCoherent));
+            HIP_TRY(hipHostGetDevicePointer((void **)&sl.d_out, sl.h_out, 0));
+            HIP_TRY(sl.d_in.alloc(sv->in_bytes));
+            HIP_TRY(sl.counter.alloc(1));
+            sl.visited = sv->visited.p + (size_t)i * (size_t)sv->slots_per_batch * (size_t)s->vis_stride;
+            HIP_TRY(hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking));
+            HIP_TRY(hipEventCreateWithFlags(&sl.done, hipEventDisableTiming));
+        }
+        glass_hip_server *p = sv.get();
+        sv->worker = std::thread(server_loop, p);
+        *out = sv.release();
+        return GLASS_HIP_OK;
+    } catch (...) {
+        return exception_status();
     }
-    for (int32_t i = 0; i < max_inflight; ++i) {
-        sv->slots.push_back(std::make_unique<ServerSlot>());
-        ServerSlot &sl = *sv->slots.back();
-        HIP_TRY(hipHostMalloc((void **)&sl.h_in, sv->in_bytes, hipHostMallocDefault));
-        HIP_TRY(hipHostMalloc((void **)&sl.h_out, sv->out_bytes, hipHostMallocMapped | hipHostMallocCoherent));
-        HIP_TRY(hipHostGetDevicePointer((void **)&sl.d_out, sl.h_out, 0));
-        HIP_TRY(sl.d_in.alloc(sv->in_bytes));
-        HIP_TRY(sl.counter.alloc(1));
-        sl.visited = sv->visited.p + (size_t)i * (size_t)sv->slots_per_batch * (size_t)s->vis_stride;
-        HIP_TRY(hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking));
-        HIP_TRY(hipEventCreateWithFlags(&sl.done, hipEventDisableTiming));
-    }
-    glass_hip_server *p = sv.get();
-    sv->worker = std::thread(server_loop, p);
-    *out = sv.release();
-    return GLASS_HIP_OK;
 }
 
 int glass_hip_server_submit(glass_hip_server *sv, const float *qs, int32_t nq, int32_t ef, int32_t k,
                             int64_t *ticket) {
-    if (!sv || !qs || !ticket) return fail(GLASS_HIP_EINVAL, "null argument");
-    auto r = std::make_unique<ServerReq>();
-    r->nq = nq;
-    r->ef = ef;
-    r->k = k;
-    if (int rc = server_enqueue(*sv, r.get())) return rc;
-    r->own_q.assign(qs, qs + (size_t)nq * sv->dim);
-    r->own_ids.assign((size_t)nq * k, -1);
-    r->own_dists.assign((size_t)nq * k, 0.0f);
-    r->q = r->own_q.data();
-    r->ids = r->own_ids.data();
-    r->dists = r->own_dists.data();
-    r->arrival_ns = now_ns();
-    {
-        std::lock_guard<std::mutex> lk(sv->mu);
-        if (sv->err_code) return fail(sv->err_code, "%s", sv->err_msg.c_str());
-        if (sv->stop) return fail(GLASS_HIP_ESTATE, "server is stopping");
-        r->ticket = sv->next_ticket++;
-        *ticket = r->ticket;
-        sv->pending.push_back(r.get());
-        sv->pending_n.store((int64_t)sv->pending.size());
-        sv->owned.emplace(r->ticket, std::move(r));
+    try {
+        if (!sv || !qs || !ticket) return fail(GLASS_HIP_EINVAL, "null argument");
+        auto r = std::make_unique<ServerReq>();
+        r->nq = nq;
+        r->ef = ef;
+        r->k = k;
+        if (int rc = server_enqueue(*sv, r.get())) return rc;
+        r->own_q.assign(qs, qs + (size_t)nq * sv->dim);
+        r->own_ids.assign((size_t)nq * k, -1);
+        r->own_dists.assign((size_t)nq * k, 0.0f);
+        r->q = r->own_q.data();
+        r->ids = r->own_ids.data();
+        r->dists = r->own_dists.data();
+        r->arrival_ns = now_ns();
+        {
+            std::lock_guard<std::mutex> lk(sv->mu);
+            if (sv->err_code) return fail(sv->err_code, "%s", sv->err_msg.c_str());
+            if (sv->stop) return fail(GLASS_HIP_ESTATE, "server is stopping");
+            r->ticket = sv->next_ticket++;
+            *ticket = r->ticket;
+            sv->pending.push_back(r.get());
+            sv->pending_n.store((int64_t)sv->pending.size());
+            sv->owned.emplace(r->ticket, std::move(r));
+        }
+        sv->cv_work.notify_one();
+        return GLASS_HIP_OK;
+    } catch (...) {
+        return exception_status();
     }
-    sv->cv_work.notify_one();
-    return GLASS_HIP_OK;
 }
 
 static int server_take(glass_hip_server *sv, std::unique_lock<std::mutex> &lk, int64_t ticket, int32_t *dst,
@@ -1788,95 +1924,115 @@ static int server_take(glass_hip_server *sv, std::unique_lock<std::mutex> &lk, i
 }
 
 int glass_hip_server_poll(glass_hip_server *sv, int64_t ticket, int32_t *dst, float *scores, int32_t *done) {
-    if (!sv || !done) return fail(GLASS_HIP_EINVAL, "null argument");
-    std::unique_lock<std::mutex> lk(sv->mu);
-    return server_take(sv, lk, ticket, dst, scores, false, done);
+    try {
+        if (!sv || !done) return fail(GLASS_HIP_EINVAL, "null argument");
+        std::unique_lock<std::mutex> lk(sv->mu);
+        return server_take(sv, lk, ticket, dst, scores, false, done);
+    } catch (...) {
+        return exception_status();
+    }
 }
 
 int glass_hip_server_wait(glass_hip_server *sv, int64_t ticket, int32_t *dst, float *scores) {
-    if (!sv) return fail(GLASS_HIP_EINVAL, "null argument");
-    std::unique_lock<std::mutex> lk(sv->mu);
-    return server_take(sv, lk, ticket, dst, scores, true, nullptr);
+    try {
+        if (!sv) return fail(GLASS_HIP_EINVAL, "null argument");
+        std::unique_lock<std::mutex> lk(sv->mu);
+        return server_take(sv, lk, ticket, dst, scores, true, nullptr);
+    } catch (...) {
+        return exception_status();
+    }
 }
 
 int glass_hip_server_cancel(glass_hip_server *sv, int64_t ticket) {
-    if (!sv) return fail(GLASS_HIP_EINVAL, "null argument");
-    std::lock_guard<std::mutex> lk(sv->mu);
-    auto it = sv->owned.find(ticket);
-    if (it == sv->owned.end()) return fail(GLASS_HIP_EINVAL, "unknown ticket %lld", (long long)ticket);
-    ServerReq *r = it->second.get();
-    if (r->state != RQ_QUEUED) return fail(GLASS_HIP_ESTATE, "request %lld already dispatched", (long long)ticket);
-    sv->pending.erase(std::find(sv->pending.begin(), sv->pending.end(), r));
-    sv->pending_n.store((int64_t)sv->pending.size());
-    sv->owned.erase(it);
-    return GLASS_HIP_OK;
+    try {
+        if (!sv) return fail(GLASS_HIP_EINVAL, "null argument");
+        std::lock_guard<std::mutex> lk(sv->mu);
+        auto it = sv->owned.find(ticket);
+        if (it == sv->owned.end()) return fail(GLASS_HIP_EINVAL, "unknown ticket %lld", (long long)ticket);
+        ServerReq *r = it->second.get();
+        if (r->state != RQ_QUEUED) return fail(GLASS_HIP_ESTATE, "request %lld already dispatched", (long long)ticket);
+        sv->pending.erase(std::find(sv->pending.begin(), sv->pending.end(), r));
+        sv->pending_n.store((int64_t)sv->pending.size());
+        sv->owned.erase(it);
+        return GLASS_HIP_OK;
+    } catch (...) {
+        return exception_status();
+    }
 }
 
 int glass_hip_server_replay(glass_hip_server *sv, const float *qs, int64_t nreq, const int32_t *nq, const int32_t *ef,
                             const int32_t *k, const int64_t *arrival_ns, int64_t *latency_ns, int32_t *dst,
                             float *scores) {
-    if (!sv || nreq < 0 || (nreq && (!qs || !nq || !ef || !k || !arrival_ns || !latency_ns || !dst || !scores)))
-        return fail(GLASS_HIP_EINVAL, "bad replay arguments");
-    std::vector<ServerReq> reqs((size_t)nreq);
-    int64_t done_count = 0;  // guarded by sv->mu
-    int64_t qoff = 0, roff = 0;
-    for (int64_t i = 0; i < nreq; ++i) {
-        ServerReq &r = reqs[(size_t)i];
-        r.nq = nq[i];
-        r.ef = ef[i];
-        r.k = k[i];
-        if (int rc = server_enqueue(*sv, &r)) return rc;
-        if (i && arrival_ns[i] < arrival_ns[i - 1]) return fail(GLASS_HIP_EINVAL, "arrivals must be sorted");
-        r.q = qs + qoff * sv->dim;
-        r.group_done = &done_count;
-        r.ids = dst + roff;
-        r.dists = scores + roff;
-        qoff += r.nq;
-        roff += (int64_t)r.nq * r.k;
-    }
-    const int64_t t0 = now_ns();
-    int64_t next = 0;
-    while (next < nreq) {
-        const int64_t t = now_ns() - t0;
-        if (arrival_ns[next] > t) {
-            if (arrival_ns[next] - t > 200000) std::this_thread::sleep_for(std::chrono::microseconds(50));
-            continue;
+    try {
+        if (!sv || nreq < 0 || (nreq && (!qs || !nq || !ef || !k || !arrival_ns || !latency_ns || !dst || !scores)))
+            return fail(GLASS_HIP_EINVAL, "bad replay arguments");
+        std::vector<ServerReq> reqs((size_t)nreq);
+        int64_t done_count = 0;  // guarded by sv->mu
+        int64_t qoff = 0, roff = 0;
+        for (int64_t i = 0; i < nreq; ++i) {
+            ServerReq &r = reqs[(size_t)i];
+            r.nq = nq[i];
+            r.ef = ef[i];
+            r.k = k[i];
+            if (int rc = server_enqueue(*sv, &r)) return rc;
+            if (i && arrival_ns[i] < arrival_ns[i - 1]) return fail(GLASS_HIP_EINVAL, "arrivals must be sorted");
+            r.q = qs + qoff * sv->dim;
+            r.group_done = &done_count;
+            r.ids = dst + roff;
+            r.dists = scores + roff;
+            qoff += r.nq;
+            roff += (int64_t)r.nq * r.k;
         }
-        {
-            std::lock_guard<std::mutex> lk(sv->mu);
-            if (sv->err_code) break;
-            while (next < nreq && arrival_ns[next] <= t) {
-                ServerReq &r = reqs[(size_t)next];
-                r.arrival_ns = t0 + arrival_ns[next];
-                sv->pending.push_back(&r);
-                ++next;
+        const int64_t t0 = now_ns();
+        int64_t next = 0;
+        while (next < nreq) {
+            const int64_t t = now_ns() - t0;
+            if (arrival_ns[next] > t) {
+                if (arrival_ns[next] - t > 200000) std::this_thread::sleep_for(std::chrono::microseconds(50));
+                continue;
+            }
+            {
+                std::lock_guard<std::mutex> lk(sv->mu);
+                if (sv->err_code) break;
+                while (next < nreq && arrival_ns[next] <= t) {
+                    ServerReq &r = reqs[(size_t)next];
+                    r.arrival_ns = t0 + arrival_ns[next];
+                    sv->pending.push_back(&r);
+                    ++next;
+                }
+                sv->pending_n.store((int64_t)sv->pending.size());
+            }
+            sv->cv_work.notify_one();
+        }
+        std::unique_lock<std::mutex> lk(sv->mu);
+        sv->cv_done.wait(lk, [&] { return sv->err_code || done_count == nreq; });
+        if (sv->err_code) {
+            // requests of this call may still sit in the queue: drop them
+            for (auto it = sv->pending.begin(); it != sv->pending.end();) {
+                ServerReq *p = *it;
+                it = (p >= reqs.data() && p < reqs.data() + reqs.size()) ? sv->pending.erase(it) : it + 1;
             }
             sv->pending_n.store((int64_t)sv->pending.size());
+            return fail(sv->err_code, "%s", sv->err_msg.c_str());
         }
-        sv->cv_work.notify_one();
+        for (int64_t i = 0; i < nreq; ++i) latency_ns[i] = reqs[(size_t)i].done_ns - reqs[(size_t)i].arrival_ns;
+        return GLASS_HIP_OK;
+    } catch (...) {
+        return exception_status();
     }
-    std::unique_lock<std::mutex> lk(sv->mu);
-    sv->cv_done.wait(lk, [&] { return sv->err_code || done_count == nreq; });
-    if (sv->err_code) {
-        // requests of this call may still sit in the queue: drop them
-        for (auto it = sv->pending.begin(); it != sv->pending.end();) {
-            ServerReq *p = *it;
-            it = (p >= reqs.data() && p < reqs.data() + reqs.size()) ? sv->pending.erase(it) : it + 1;
-        }
-        sv->pending_n.store((int64_t)sv->pending.size());
-        return fail(sv->err_code, "%s", sv->err_msg.c_str());
-    }
-    for (int64_t i = 0; i < nreq; ++i) latency_ns[i] = reqs[(size_t)i].done_ns - reqs[(size_t)i].arrival_ns;
-    return GLASS_HIP_OK;
 }
 
 int glass_hip_server_stats(glass_hip_server *sv, int64_t *batches, int64_t *queries, int64_t *requests) {
-    if (!sv) return fail(GLASS_HIP_EINVAL, "null argument");
-    std::lock_guard<std::mutex> lk(sv->mu);
-    if (batches) *batches = sv->n_batches;
-    if (queries) *queries = sv->n_queries;
-    if (requests) *requests = sv->n_requests;
-    return GLASS_HIP_OK;
+    try {
+        if (!sv) return fail(GLASS_HIP_EINVAL, "null argument");
+        std::lock_guard<std::mutex> lk(sv->mu);
+        if (batches) *batches = sv->n_batches;
+        if (queries) *queries = sv->n_queries;
+        if (requests) *requests = sv->n_requests;
+        return GLASS_HIP_OK;
+    } catch (...) {
+        return exception_status();
+    }
 }
 
 void glass_hip_server_free(glass_hip_server *sv) {

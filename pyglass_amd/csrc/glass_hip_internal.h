@@ -100,6 +100,9 @@ bool team_valid(const DeviceIndex &ix, int32_t team);
 
 // Sets the calling thread's last-error text (glass_hip_last_error) and returns code.
 int set_error(int code, const char *msg);
+// Status for the exception being handled: no C++ exception crosses the C ABI.
+// Every entry point that returns a status catches (...) and returns this.
+int exception_status() noexcept;
 
 // GenRandom (glass/utils.hpp:13-27): size sorted, distinct, rotated draws in [0, N).
 void gen_random(std::mt19937 &rng, int32_t *addr, int32_t size, int32_t N);

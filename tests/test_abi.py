@@ -138,3 +138,12 @@ def test_clustering_argument_errors_without_a_gpu():
         big.fit(np.zeros((200000, 1), np.float32))  # the sample is capped at 100,000 rows
     with pytest.raises(pg.GlassError, match="dim"):
         pg.quant_sq8(0)
+
+
+def test_host_allocation_failure_returns_a_status():
+    # (2^31 - 1) x 2^20 centroid floats (8 PiB) cannot be allocated on the host.
+    # The entry point catches std::bad_alloc and returns GLASS_HIP_ENOMEM; an
+    # exception escaping through extern "C" would end the process instead.
+    c = pg.Clustering(2**31 - 1, 1, "kmeans++")
+    with pytest.raises(pg.GlassError, match=r"error -3: host allocation failed"):
+        c.fit(np.zeros((2, 1 << 20), np.float32))
