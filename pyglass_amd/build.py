@@ -32,6 +32,12 @@ HIPCC_FLAGS = [
     "-I" + os.path.join(ROOT, "include"),
 ]
 
+# Per-source extra flags.  The search kernel is a chain of dependent gathers:
+# the memory-clause scheduling strategy groups its loads into clauses, which
+# measured about 1% faster at 1M x 128, ef 64 (0.636 against 0.644 ms) with
+# unchanged results.
+SOURCE_FLAGS = {"search_kernels.hip": ["-mllvm", "-amdgpu-sched-strategy=max-memory-clause"]}
+
 
 def hipcc() -> str:
     for p in (os.environ.get("HIPCC"), "/opt/rocm/bin/hipcc"):
@@ -46,7 +52,7 @@ def build(verbose: bool = False) -> str:
     os.makedirs(obj_dir, exist_ok=True)
     for src in SOURCES:
         obj = os.path.join(obj_dir, os.path.splitext(src)[0] + ".o")
-        cmd = [hipcc(), *HIPCC_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj]
+        cmd = [hipcc(), *HIPCC_FLAGS, *SOURCE_FLAGS.get(src, []), "-c", os.path.join(CSRC, src), "-o", obj]
         if verbose:
             print(" ".join(cmd), flush=True)
         r = subprocess.run(cmd, capture_output=True, text=True)
