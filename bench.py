@@ -254,7 +254,7 @@ def _pmc_values(db_path, last=3):
     return kern, copy
 
 
-def pmc_run(counters, point: str, launch: str, timeout_s: int = 240):
+def pmc_run(counters, point: str, launch: str, timeout_s: int = 240, cfg: str | None = None):
     """One rocprofv3 --pmc child run of `bench.py --point` with a 512 MiB
     device copy; returns (search-kernel values, copy values) per counter, or
     (None, error)."""
@@ -269,7 +269,8 @@ def pmc_run(counters, point: str, launch: str, timeout_s: int = 240):
     env = dict(os.environ, TMPDIR="/tmp")
     try:
         cmd = [prof, "--pmc", *counters, "-d", tmp, "-o", "run", "--", sys.executable, os.path.abspath(__file__),
-               "--point", point, "--launch", launch, "--steps", "2", "--warmup", "1", "--no-cpu", "--calib-copy"]
+               "--point", point, "--launch", launch, "--steps", "2", "--warmup", "1", "--no-cpu", "--calib-copy",
+               *(["--config", cfg] if cfg else [])]
         log(f"pmc: {' '.join(counters)} pass")
         try:
             subprocess.run(cmd, cwd="/tmp", env=env, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL,
@@ -293,7 +294,7 @@ def info_launch_waves(s):
     return ki["blocks_per_cu"] * lz["waves_per_block"]
 
 
-def pmc_traffic(point: str, launch: str):
+def pmc_traffic(point: str, launch: str, cfg: str | None = None):
     """HBM bytes per search launch at an operating point: one rocprofv3 --pmc
     run per counter (FETCH_SIZE, WRITE_SIZE), each counter scaled by what it
     reads for a 512 MiB device copy in the same run (bytes = KiB * 1024 *
@@ -301,7 +302,7 @@ def pmc_traffic(point: str, launch: str):
     detail = {"point": point, "launch": launch}
     total = 0.0
     for ctr in ("FETCH_SIZE", "WRITE_SIZE"):
-        kern, copy = pmc_run([ctr], point, launch)
+        kern, copy = pmc_run([ctr], point, launch, timeout_s=480 if cfg else 240, cfg=cfg)
         if kern is None or ctr not in copy:
             return None, dict(detail, error=f"{ctr}: {copy if kern is None else 'copy dispatch not found'}")
         scale = (512 << 20) / (copy[ctr] * 1024.0)
@@ -486,6 +487,507 @@ def clustering_mode(gpath, X, Q, args, pg):
     print(json.dumps(out))
 
 
+# ---------------------------------------------------------------------------
+# The other workloads at full size (--config c3 / c4 / c5).  Each prints one
+# JSON line of its own; the default run (no --config) is the headline above.
+
+WORKLOADS = {
+    "c3": {"desc": "1M x 960 Gamma(2, 0.05) rows L2-normalised x10, NSG R=64, L2, SQ8U, per-query ef "
+                   "log-uniform 16-512 and k uniform 1-100, 1,000 queries in one persistent launch, optimize() on",
+           "n": 1_000_000, "d": 960, "nq": 1000, "metric": "L2", "quant": "SQ8U", "refine": "",
+           "kind": "NSG", "R": 64, "L": 96, "graph": "c3_gamma1m_960_nsg_R64",
+           "base": lambda n: __import__("pyglass_amd.synth", fromlist=["x"]).gamma_rows(n, 960, 1234),
+           "query": lambda m: __import__("pyglass_amd.synth", fromlist=["x"]).gamma_rows(m, 960, 5678)},
+    "c4": {"desc": "1M x 768 unit-normalised N(0,1) rows, NSG R=48, inner product, SQ8 + FP32 re-rank, requests "
+                   "of U[1,256] queries with ef in {32,64,128,256}, k=10, Poisson arrivals swept to saturation",
+           "n": 1_000_000, "d": 768, "nq": 10000, "metric": "IP", "quant": "SQ8", "refine": "FP32",
+           "kind": "NSG", "R": 48, "L": 80, "graph": "c4_unit1m_768_nsg_R48",
+           "base": lambda n: __import__("pyglass_amd.synth", fromlist=["x"]).unit_rows(n, 768, 1234),
+           "query": lambda m: __import__("pyglass_amd.synth", fromlist=["x"]).unit_rows(m, 768, 5678)},
+    "c5": {"desc": "10M x 128 mixture of 4,096 Gaussians (centers N(0,4I), sigma 1), HNSW R=32 L=100, L2, SQ2U + "
+                   "FP32 re-rank, ef 128, k=10, 100,000 queries in one batch",
+           "n": 10_000_000, "d": 128, "nq": 100_000, "metric": "L2", "quant": "SQ2U", "refine": "FP32",
+           "kind": "HNSW", "R": 32, "L": 100, "graph": "c5_mix10m_hnsw_R32_L100",
+           "base": lambda n: __import__("pyglass_amd.synth", fromlist=["x"]).normal_mixture(n, 128, 1234),
+           "query": lambda m: __import__("pyglass_amd.synth", fromlist=["x"]).normal_mixture(m, 128, 5678)},
+}
+
+
+def exact_knn_chunked(X, Q, k, metric, device, qchunk=1024, bchunk=1 << 21):
+    """Exact kNN on the GPU for any base size: fp32 GEMM per (query chunk,
+    base chunk), top k+16 candidates per base chunk, then the candidates
+    re-sorted by their float64 distance (ties by id).  L2 or inner product
+    (largest dot first)."""
+    import torch
+
+    torch.backends.cuda.matmul.allow_tf32 = False
+    xb = torch.from_numpy(X).to(device)
+    xn = (xb * xb).sum(1) if metric == "L2" else None
+    n, m = X.shape[0], k + 16
+    out = np.empty((Q.shape[0], k), np.int64)
+    for s in range(0, Q.shape[0], qchunk):
+        q = torch.from_numpy(Q[s:s + qchunk]).to(device)
+        cand = []
+        for b in range(0, n, bchunk):
+            e = min(n, b + bchunk)
+            g = q @ xb[b:e].T
+            d = xn[b:e][None, :] - 2.0 * g if metric == "L2" else -g
+            cand.append(torch.topk(d, min(m, e - b), dim=1, largest=False).indices + b)
+            del g, d
+        ci = torch.cat(cand, 1)
+        xc = xb[ci].double()
+        qd = q.double()[:, None, :]
+        dd = ((xc - qd) ** 2).sum(-1) if metric == "L2" else -(xc * qd).sum(-1)
+        ci, perm = torch.sort(ci, dim=1)
+        dd = torch.gather(dd, 1, perm)
+        order = torch.sort(dd, dim=1, stable=True).indices
+        out[s:s + qchunk] = torch.gather(ci, 1, order)[:, :k].cpu().numpy()
+        del xc, qd, dd, ci
+    del xb, xn
+    torch.cuda.empty_cache()
+    return out
+
+
+def recall_var_k(ids, gt, ks):
+    """Mean over queries of |result_i intersect gt_i[:k_i]| / k_i (per-query k)."""
+    return float(np.mean([np.intersect1d(a[:kk], b[:kk]).size / kk for a, b, kk in zip(ids, gt, ks)]))
+
+
+def workload_graph(w, X, small):
+    """The workload's graph under data/: the file shipped with the tree (NSG at
+    1M rows takes far longer than a GPU run may) or built here on the host."""
+    import pyglass_amd as pg
+
+    name = w["graph"] + ("_small" if small else "")
+    path = os.path.join(ROOT, "data", name + ".glass")
+    meta = path + ".json"
+    if os.path.exists(path):
+        m = json.load(open(meta)) if os.path.exists(meta) else {}
+        log(f"graph: {path} ({m})")
+        return path, m
+    os.makedirs(os.path.dirname(path), exist_ok=True)
+    log(f"graph: building {w['kind']} R={w['R']} L={w['L']} over {X.shape[0]} rows on {os.cpu_count()} cores ...")
+    secs = pg.build_graph(w["kind"], X, path, w["metric"], "FP32", w["R"], w["L"])
+    m = {"seconds": secs, "host": "this run", "cores": os.cpu_count(), "R": w["R"], "L": w["L"]}
+    json.dump(m, open(meta, "w"))
+    log(f"graph: built in {secs:.1f}s")
+    return path, m
+
+
+def alg_bytes(cnt, info, d, ks, rq):
+    """Algorithmic bytes per launch: B_q = C_q*code + E_q*4K + U_q*4 +
+    r_q*rerank_row + 4d + 8k_q, summed over the queries."""
+    ks = np.broadcast_to(np.asarray(ks, np.int64), (cnt.shape[0],))
+    rq = np.broadcast_to(np.asarray(rq, np.int64), (cnt.shape[0],))
+    bq = (cnt[:, 0].astype(np.int64) * info["code_size"] + cnt[:, 1].astype(np.int64) * 4 * info["degree"]
+          + cnt[:, 2].astype(np.int64) * 4 + rq * info["rerank_row_bytes"] + 4 * d + 8 * ks)
+    return float(bq.sum())
+
+
+def host_cpu():
+    """CPU model and the threads the CPU leg uses."""
+    model = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"model": model, "threads": os.cpu_count()}
+
+
+def ref_points(gpath, X, Q, gt, k, w, efs, factor=None, budget_s=20.0, runs=3):
+    """The compiled reference Searcher on all host cores (optimize() first),
+    per ef the best of up to `runs` batch_search calls over Q, within a time
+    budget; returns [(ef, recall, qps, ids)]."""
+    from oracle import refwrap as R_
+
+    if not R_.available(portable=True):
+        return None
+    with stdout_to_stderr():
+        R_.set_num_threads(os.cpu_count() or 1, portable=True)
+        t = time.time()
+        r = R_.RefSearcher(gpath, X, w["metric"], w["quant"], w["refine"], portable=True, refine_factor=factor)
+        r.optimize()
+        log(f"cpu: reference searcher ready in {time.time() - t:.1f}s")
+        out, spent = [], 0.0
+        for ef in efs:
+            r.set_ef(ef)
+            best, ids = None, None
+            for run in range(runs):
+                t0 = time.perf_counter()
+                I, _ = r.batch_search(Q, k)
+                dt = time.perf_counter() - t0
+                spent += dt
+                best = dt if best is None else min(best, dt)
+                ids = I if ids is None else ids
+                if spent > budget_s:
+                    break
+            rec = recall_at(ids, gt, k) if gt is not None else None
+            out.append((ef, rec, Q.shape[0] / best, ids))
+            log(f"cpu: factor={factor} ef={ef} recall={rec} qps={Q.shape[0] / best:.0f}")
+            if spent > budget_s:
+                break
+        del r
+    return out
+
+
+def config_mode(cfg, args, pg, torch, dev):
+    w = WORKLOADS[cfg]
+    n = w["n"] // 10 if args.small else w["n"]
+    nq = w["nq"]
+    t = time.time()
+    X = w["base"](n)
+    Q = w["query"](nq)
+    log(f"{cfg}: data {n} x {w['d']} + {nq} queries in {time.time() - t:.1f}s")
+    gpath, gmeta = workload_graph(w, X, args.small)
+    t = time.time()
+    s = pg.Searcher(pg.Graph(gpath), X, w["metric"], w["quant"], w["refine"])
+    setup_s = time.time() - t
+    t = time.time()
+    if not (args.point and args.launch):
+        s.optimize()
+    if args.launch:
+        lt, lw, lv = (int(v) for v in args.launch.split(":"))
+        s.set_launch(lt, lw, 0)
+        s.set_visited_capacity(lv)
+    opt_s = time.time() - t
+    info = s.info()
+    log(f"{cfg}: set_data {setup_s:.1f}s, optimize {opt_s:.1f}s, info {info}, device {pg.device_info()}")
+    stream = torch.cuda.Stream(device=dev)
+    if cfg == "c3":
+        return c3_run(w, s, X, Q, gpath, gmeta, info, args, torch, dev, stream)
+    if cfg == "c4":
+        return c4_run(w, s, X, Q, gpath, gmeta, info, args, torch, dev, stream)
+    return c5_run(w, s, X, Q, gpath, gmeta, info, args, torch, dev, stream)
+
+
+def point_or_sweep(s, w, X, Q, gt, k, info, args, torch, dev, stream, factors, efs):
+    """Device-resident batched sweep (W warm-up + K timed launches per point)."""
+    nq = Q.shape[0]
+    q_dev = torch.from_numpy(Q).to(dev)
+    ids_dev = torch.empty((nq, k), dtype=torch.int32, device=dev)
+    d_dev = torch.empty((nq, k), dtype=torch.float32, device=dev)
+    cnt_dev = torch.empty((nq, 3), dtype=torch.int32, device=dev)
+    pts = []
+    for f in factors:
+        for ef in efs:
+            if f:
+                s.set_refine_factor(f)
+            s.set_ef(ef)
+            wall, kern = time_batched(s, q_dev, nq, k, ids_dev, d_dev, cnt_dev, args.steps, args.warmup, torch, stream)
+            ids = ids_dev.cpu().numpy()
+            cnt = cnt_dev.cpu().numpy()
+            rq = int(k * (f or info["refine_factor"])) if w["refine"] else 0
+            p = {"factor": f or info["refine_factor"], "ef": ef, "recall": recall_at(ids, gt, k) if gt is not None
+                 else None, "ms_per_step": wall / args.steps * 1e3, "qps": nq / (wall / args.steps),
+                 "kernel_ms": float(np.mean(kern)), "dist_cmps": float(cnt[:, 0].mean()),
+                 "expansions": float(cnt[:, 1].mean()), "launch": s.get_launch(),
+                 "algorithmic_bytes_per_launch": alg_bytes(cnt, info, X.shape[1], k, rq), "ids": ids}
+            p["achieved_gbs"] = p["algorithmic_bytes_per_launch"] / (p["kernel_ms"] / 1e3) / 1e9
+            pts.append(p)
+            log(f"gpu: factor={p['factor']} ef={ef} recall={p['recall']} qps={p['qps']:.0f} "
+                f"kernel={p['kernel_ms']:.3f}ms dist_cmps={p['dist_cmps']:.0f} {p['achieved_gbs']:.0f} GB/s "
+                f"launch={p['launch']} {s.kernel_info()}")
+    return pts
+
+
+def strip(p):
+    return {kk: v for kk, v in p.items() if kk not in ("ids", "cnt")}
+
+
+def pmc_for(cfg, point, launch):
+    """FETCH_SIZE / WRITE_SIZE traffic of the operating point (child runs)."""
+    return pmc_traffic(point, launch, cfg)
+
+
+def c5_run(w, s, X, Q, gpath, gmeta, info, args, torch, dev, stream):
+    k = 10
+    if args.point:
+        pf, pe = args.point.split(":")
+        p = point_or_sweep(s, w, X, Q, None, k, info, args, torch, dev, stream, [float(pf)], [int(pe)])[0]
+        res = strip(p)
+        if args.calib_copy:
+            res["calib_copy_bytes"] = calib_copy(torch, dev)
+        print(json.dumps(res))
+        return
+    t = time.time()
+    gt = exact_knn_chunked(X, Q, k, w["metric"], dev)
+    log(f"gt: {Q.shape[0]} queries in {time.time() - t:.1f}s")
+    efs = [int(e) for e in args.efs.split(",")] if args.efs else [64, 96, 128, 192, 256]
+    pts = point_or_sweep(s, w, X, Q, gt, k, info, args, torch, dev, stream, [3.0, 4.0], efs)
+    op = next(p for p in pts if p["factor"] == 4.0 and p["ef"] == 128) if any(
+        p["factor"] == 4.0 and p["ef"] == 128 for p in pts) else pts[-1]
+    curves = {f: [(p["recall"], p["qps"]) for p in pts if p["factor"] == f] for f in (3.0, 4.0)}
+    at95 = {str(f): qps_at_recall(c) for f, c in curves.items()}
+    traffic, tdet = (None, None)
+    if not args.no_pmc:
+        lz = op["launch"]
+        traffic, tdet = pmc_for("c5", f"{op['factor']}:{op['ef']}", f"{lz['team']}:{lz['waves_per_block']}:"
+                                                                      f"{lz['vis_entries']}")
+    cpu = None
+    if not args.no_cpu:
+        ns = min(Q.shape[0], 20000)
+        cpu = {}
+        for f in (3.0, 4.0):
+            r = ref_points(gpath, X, Q[:ns], gt[:ns], k, w, [128], factor=f, budget_s=20.0)
+            if r:
+                ef, rec, qps, ids = r[0]
+                gp = next(p for p in pts if p["factor"] == f and p["ef"] == 128)
+                cpu[str(f)] = {"ef": 128, "recall": rec, "qps": qps,
+                               "ids_equal_gpu": float(np.mean(np.all(ids == gp["ids"][:ns], axis=1)))}
+    out = {
+        "metric": "QPS and recall@10 at ef=128 (batched GPU search, 10M x 128 SQ2U+FP32 rerank)",
+        "value": op["qps"], "unit": "queries/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": op["ms_per_step"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "u2 codes, int32 distances; fp32 rerank", "data": "synthetic (seeded Gaussian mixture)",
+        "config": {"workload": w["desc"] + (" (small: 1M rows)" if args.small else ""),
+                   "operating_point": {"factor": op["factor"], "ef": op["ef"], "recall": op["recall"]},
+                   "qps_at_recall_0.95": at95, "sweep": [strip(p) for p in pts], "graph": gmeta,
+                   "resident_bytes": resident_bytes(info, w, X.shape[0])},
+        "roofline": {"bound": "hbm", "achieved": op["achieved_gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": op["achieved_gbs"] / HBM_PEAK_GBS, "traffic": traffic, "traffic_detail": tdet,
+                     "algorithmic_bytes_per_launch": op["algorithmic_bytes_per_launch"],
+                     "kernel_ms": op["kernel_ms"]},
+        "cpu_baseline": None if not cpu else {
+            "value": cpu.get("4.0", {}).get("qps"), "unit": "queries/s", "cores": os.cpu_count(),
+            "cpu": host_cpu(), "kind": "reference",
+            "sample": f"first {min(Q.shape[0], 20000)} queries at ef 128, factors 3.0 and 4.0, best of <=3 "
+                      f"batch_search runs after optimize()", "points": cpu},
+    }
+    print(json.dumps(out))
+
+
+def resident_bytes(info, w, n):
+    graph = n * 4 * info["degree"]
+    return {"graph": graph, "codes": n * info["code_size"], "rerank_rows": n * info["rerank_row_bytes"]}
+
+
+def calib_copy(torch, dev):
+    nbytes = 512 << 20
+    a = torch.empty(nbytes // 4, dtype=torch.float32, device=dev).fill_(1.0)
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    return nbytes
+
+
+def c3_run(w, s, X, Q, gpath, gmeta, info, args, torch, dev, stream):
+    """Per-query ef (log-uniform 16-512) and k (1-100) in one persistent launch."""
+    from pyglass_amd import synth
+
+    nq = Q.shape[0]
+    efv, kv = synth.per_query_ef_k(nq)
+    off = np.concatenate([[0], np.cumsum(kv)]).astype(np.int64)
+    q_dev = torch.from_numpy(Q).to(dev)
+    ef_dev = torch.from_numpy(efv).to(dev)
+    k_dev = torch.from_numpy(kv).to(dev)
+    off_dev = torch.from_numpy(off[:-1].copy()).to(dev)
+    ids_dev = torch.empty(int(off[-1]), dtype=torch.int32, device=dev)
+    d_dev = torch.empty(int(off[-1]), dtype=torch.float32, device=dev)
+    cnt_dev = torch.empty((nq, 3), dtype=torch.int32, device=dev)
+    sp = stream.cuda_stream
+
+    def launch():
+        s.search_batch_efk_dev(q_dev.data_ptr(), nq, ef_dev.data_ptr(), k_dev.data_ptr(), off_dev.data_ptr(),
+                               ids_dev.data_ptr(), int(efv.max()), int(kv.max()), d_dev.data_ptr(),
+                               cnt_dev.data_ptr(), stream=sp)
+
+    for _ in range(args.warmup):
+        launch()
+    torch.cuda.synchronize()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        ev[i][0].record(stream)
+        launch()
+        ev[i][1].record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    kern = float(np.mean([a.elapsed_time(b) for a, b in ev]))
+    cnt = cnt_dev.cpu().numpy()
+    ab = alg_bytes(cnt, info, X.shape[1], kv, 0)
+    mix = {"ms_per_step": wall / args.steps * 1e3, "qps": nq / (wall / args.steps), "kernel_ms": kern,
+           "dist_cmps": float(cnt[:, 0].mean()), "algorithmic_bytes_per_launch": ab,
+           "achieved_gbs": ab / (kern / 1e3) / 1e9, "launch": s.get_launch(), "kernel": s.kernel_info()}
+    log(f"c3 mix: {mix}")
+    if args.point:
+        if args.calib_copy:
+            mix["calib_copy_bytes"] = calib_copy(torch, dev)
+        print(json.dumps(mix))
+        return
+    t = time.time()
+    gt = exact_knn_chunked(X, Q, 100, w["metric"], dev)
+    log(f"gt: {nq} queries, k=100 in {time.time() - t:.1f}s")
+    ids = ids_dev.cpu().numpy()
+    res = [ids[off[i]:off[i + 1]] for i in range(nq)]
+    mix["recall_at_own_k"] = recall_var_k(res, gt, kv)
+    log(f"c3 mix recall@k_i {mix['recall_at_own_k']:.4f}")
+    # uniform k=10 sweep for QPS at recall@10 = 0.95
+    efs = [int(e) for e in args.efs.split(",")] if args.efs else [16, 24, 32, 48, 64, 96, 128, 192, 256]
+    pts = point_or_sweep(s, w, X, Q, gt[:, :10], 10, info, args, torch, dev, stream, [None], efs)
+    v10 = qps_at_recall([(p["recall"], p["qps"]) for p in pts])
+    traffic, tdet = (None, None)
+    if not args.no_pmc:
+        lz = mix["launch"]
+        traffic, tdet = pmc_for("c3", "mix", f"{lz['team']}:{lz['waves_per_block']}:{lz['vis_entries']}")
+    cpu = None
+    if not args.no_cpu:
+        r = ref_points(gpath, X, Q, gt[:, :10], 10, w, efs, budget_s=30.0)
+        if r:
+            eq = [float(np.mean(np.all(ids_ == p["ids"], axis=1))) for (ef, _, _, ids_), p in zip(r, pts)]
+            cpu = {"value": qps_at_recall([(rec, q) for _, rec, q, _ in r]), "unit": "queries/s",
+                   "cores": os.cpu_count(), "cpu": host_cpu(), "kind": "reference",
+                   "sample": f"{nq} queries, uniform k=10 ef sweep, best of <=3 batch_search runs after optimize()",
+                   "points": [(ef, rec, q) for ef, rec, q, _ in r], "ids_equal_gpu": min(eq) if eq else None}
+    out = {
+        "metric": "QPS of the per-query ef/k batch and QPS at recall@10=0.95 (1M x 960 NSG SQ8U)",
+        "value": mix["qps"], "unit": "queries/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": mix["ms_per_step"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "u8 codes, int32 distances", "data": "synthetic (seeded Gamma rows)",
+        "config": {"workload": w["desc"] + (" (small: 100k rows)" if args.small else ""), "mixed_batch": mix,
+                   "qps_at_recall10_0.95": v10, "sweep_k10": [strip(p) for p in pts], "graph": gmeta},
+        "roofline": {"bound": "hbm", "achieved": mix["achieved_gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": mix["achieved_gbs"] / HBM_PEAK_GBS, "traffic": traffic, "traffic_detail": tdet,
+                     "algorithmic_bytes_per_launch": ab, "kernel_ms": kern},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+def c4_run(w, s, X, Q, gpath, gmeta, info, args, torch, dev, stream):
+    """Served form: request batches of U[1,256] queries, ef in {32,64,128,256},
+    k=10, Poisson arrivals at fractions of saturation; beside it the same
+    query mix as one device-resident batch, and the reference CPU searcher."""
+    from pyglass_amd import synth
+
+    k = 10
+    EF_SET = [32, 64, 128, 256]
+    rng = np.random.default_rng(synth.ARRIVAL_SEED)
+    nreq = args.requests
+    rnq = rng.integers(1, 257, nreq).astype(np.int32)
+    ref = rng.choice(EF_SET, nreq).astype(np.int32)
+    starts = np.concatenate([[0], np.cumsum(rnq)]).astype(np.int64)
+    total = int(starts[-1])
+    rows = (np.arange(total) % Q.shape[0]).astype(np.int64)
+    Qr = Q[rows]
+    qef = np.repeat(ref, rnq)
+    # the same mix as one device-resident batch (per-query ef, k = 10)
+    q_dev = torch.from_numpy(Qr).to(dev)
+    ef_dev = torch.from_numpy(qef).to(dev)
+    kk = np.full(total, k, np.int32)
+    k_dev = torch.from_numpy(kk).to(dev)
+    off_dev = torch.from_numpy(np.arange(total, dtype=np.int64) * k).to(dev)
+    ids_dev = torch.empty(total * k, dtype=torch.int32, device=dev)
+    d_dev = torch.empty(total * k, dtype=torch.float32, device=dev)
+    cnt_dev = torch.empty((total, 3), dtype=torch.int32, device=dev)
+    sp = stream.cuda_stream
+
+    def launch():
+        s.search_batch_efk_dev(q_dev.data_ptr(), total, ef_dev.data_ptr(), k_dev.data_ptr(), off_dev.data_ptr(),
+                               ids_dev.data_ptr(), int(qef.max()), k, d_dev.data_ptr(), cnt_dev.data_ptr(),
+                               stream=sp)
+
+    for _ in range(args.warmup):
+        launch()
+    torch.cuda.synchronize()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        ev[i][0].record(stream)
+        launch()
+        ev[i][1].record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    kern = float(np.mean([a.elapsed_time(b) for a, b in ev]))
+    cnt = cnt_dev.cpu().numpy()
+    ab = alg_bytes(cnt, info, X.shape[1], k, int(k * info["refine_factor"]))
+    batched = {"queries": total, "ms_per_step": wall / args.steps * 1e3, "qps": total / (wall / args.steps),
+               "kernel_ms": kern, "algorithmic_bytes_per_launch": ab, "achieved_gbs": ab / (kern / 1e3) / 1e9,
+               "dist_cmps": float(cnt[:, 0].mean()), "launch": s.get_launch(), "kernel": s.kernel_info()}
+    log(f"c4 batched mix: {batched}")
+    if args.point:
+        if args.calib_copy:
+            batched["calib_copy_bytes"] = calib_copy(torch, dev)
+        print(json.dumps(batched))
+        return
+    bids = ids_dev.cpu().numpy().reshape(total, k)
+    del q_dev
+    t = time.time()
+    gt = exact_knn_chunked(X, Q, k, w["metric"], dev)
+    log(f"gt: {Q.shape[0]} queries in {time.time() - t:.1f}s")
+    gtr = gt[rows]
+    rec_ef = {int(e): recall_at(bids[qef == e], gtr[qef == e], k) for e in EF_SET}
+    batched["recall_by_ef"] = rec_ef
+    log(f"c4 batched recall by ef {rec_ef}")
+    srv = pg_server(s, args)
+    srv.replay(Qr[: starts[min(nreq, 50)]], rnq[:50], ref[:50], k, np.zeros(min(nreq, 50), np.int64))  # warm-up
+    lat, sids, _ = srv.replay(Qr, rnq, ref, k, np.zeros(nreq, np.int64))
+    sat = total / (lat.max() / 1e9)
+    same = bool(np.array_equal(sids.reshape(total, k), bids))
+    log(f"c4 served: saturation {sat:.0f} QPS ({srv.stats()}), ids equal batched: {same}")
+    served = {"saturation_qps": sat, "ids_equal_batched": same, "requests": nreq, "queries": total,
+              "max_batch": args.max_batch, "max_inflight": args.max_inflight, "loads": []}
+    arng = np.random.default_rng(synth.ARRIVAL_SEED + 1)
+    for f in (0.25, 0.5, 0.75, 0.9):
+        req_rate = f * sat / float(rnq.mean())
+        arr = np.cumsum(arng.exponential(1e9 / req_rate, nreq)).astype(np.int64)
+        b0 = srv.stats()
+        lat, _, _ = srv.replay(Qr, rnq, ref, k, arr)
+        b1 = srv.stats()
+        lt = lat[nreq // 10:] / 1e6
+        span = (arr[-1] + lat[-1]) / 1e9
+        row = {"offered_fraction": f, "offered_qps": f * sat, "achieved_qps": total / span,
+               "p50_ms": float(np.percentile(lt, 50)), "p99_ms": float(np.percentile(lt, 99)),
+               "mean_batch": (b1["queries"] - b0["queries"]) / max(1, b1["batches"] - b0["batches"])}
+        served["loads"].append(row)
+        log(f"c4 served: offered {row['offered_qps']:.0f} QPS -> p50 {row['p50_ms']:.3f} ms, "
+            f"p99 {row['p99_ms']:.3f} ms, mean batch {row['mean_batch']:.0f}")
+    srv.close()
+    half = next(r for r in served["loads"] if r["offered_fraction"] == 0.5)
+    served["p50_ms_at_half_saturation"] = half["p50_ms"]
+    served["p99_ms_at_half_saturation"] = half["p99_ms"]
+    served["served_over_batched"] = sat / batched["qps"]
+    traffic, tdet = (None, None)
+    if not args.no_pmc:
+        lz = batched["launch"]
+        traffic, tdet = pmc_for("c4", "mix", f"{lz['team']}:{lz['waves_per_block']}:{lz['vis_entries']}")
+    cpu = None
+    if not args.no_cpu:
+        ns = min(Q.shape[0], 4000)
+        r = ref_points(gpath, X, Q[:ns], gt[:ns], k, w, EF_SET, budget_s=40.0, runs=2)
+        if r and len(r) == len(EF_SET):
+            frac = {e: float(np.mean(qef == e)) for e in EF_SET}
+            mix_qps = 1.0 / sum(frac[ef] / q for ef, _, q, _ in r)
+            cpu = {"value": mix_qps, "unit": "queries/s", "cores": os.cpu_count(), "cpu": host_cpu(),
+                   "kind": "reference",
+                   "sample": f"first {ns} queries at each ef of the mix, best of <=2 batch_search runs after "
+                             f"optimize(); the mix's rate from the per-ef rates and the mix's ef shares",
+                   "points": [(ef, rec, q) for ef, rec, q, _ in r]}
+    out = {
+        "metric": "served saturation QPS with p50/p99 at half load (1M x 768 IP NSG SQ8+FP32, request batches)",
+        "value": sat, "unit": "queries/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": batched["ms_per_step"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "u8 codes, fp32 distances; fp32 rerank", "data": "synthetic (seeded unit rows)",
+        "config": {"workload": w["desc"] + (" (small: 100k rows)" if args.small else ""), "served": served,
+                   "batched_same_mix": batched, "graph": gmeta},
+        "roofline": {"bound": "hbm", "achieved": batched["achieved_gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": batched["achieved_gbs"] / HBM_PEAK_GBS, "traffic": traffic, "traffic_detail": tdet,
+                     "algorithmic_bytes_per_launch": ab, "kernel_ms": kern},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+def pg_server(s, args):
+    import pyglass_amd as pg
+
+    return pg.Server(s, max_batch=args.max_batch, max_k=10, max_inflight=args.max_inflight)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -510,6 +1012,11 @@ def main():
                     help="comma-separated re-rank factors to sweep (default: %s)" % ",".join(str(f) for f in FACTORS))
     ap.add_argument("--clustering", action="store_true",
                     help="time Clustering fit / predict and the searcher set-up against the reference")
+    ap.add_argument("--config", type=str, default=None, choices=sorted(WORKLOADS),
+                    help="measure another workload at full size instead of the headline (c3, c4, c5)")
+    ap.add_argument("--requests", type=int, default=2000, help="--config c4: requests per served run")
+    ap.add_argument("--max-batch", type=int, default=16384, help="--config c4: server max_batch")
+    ap.add_argument("--max-inflight", type=int, default=3, help="--config c4: server staging slots")
     ap.add_argument("--quant", type=str, default=None,
                     help="Q[:REFINE] - measure quantizer Q (with optional re-rank quantizer) instead of the headline")
     args = ap.parse_args()
@@ -528,6 +1035,9 @@ def main():
     from pyglass_amd import synth
 
     dev = torch.device("cuda:0")
+    if args.config:
+        config_mode(args.config, args, pg, torch, dev)
+        return
     n = 100_000 if args.small else 1_000_000
     nq = 10_000
     k = 10

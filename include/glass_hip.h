@@ -111,6 +111,15 @@ int glass_hip_searcher_search_batch_efk(glass_hip_searcher *s, const float *qs, 
 int glass_hip_searcher_search_batch_dev(glass_hip_searcher *s, const float *qs_dev, int32_t nq, int32_t k,
                                         int32_t mode, int32_t *dst_dev, float *scores_dev, int32_t *counters_dev,
                                         void *stream);
+/* Device-resident batch with per-query ef_dev[nq] and k_dev[nq] (SearchBatch
+ * semantics per query); results of query i start at off_dev[i] in dst_dev /
+ * scores_dev.  max_ef and max_k bound every entry of ef_dev and k_dev (the
+ * launch is sized for them; a query beyond them gets ids -1).  Enqueued on
+ * `stream` (NULL = the searcher's stream) without host sync. */
+int glass_hip_searcher_search_batch_efk_dev(glass_hip_searcher *s, const float *qs_dev, int32_t nq,
+                                            const int32_t *ef_dev, const int32_t *k_dev, const int64_t *off_dev,
+                                            int32_t max_ef, int32_t max_k, int32_t *dst_dev, float *scores_dev,
+                                            int32_t *counters_dev, void *stream);
 /* search_batch_dev replays a captured hipGraph (counter-reset kernel +
  * search kernel) when every kernel argument and the stream repeat the
  * previous call's; the first call with new arguments runs eagerly and is

@@ -245,6 +245,16 @@ class Searcher:
             self._h, C.c_void_p(q_ptr), int(nq), int(k), int(mode), C.c_void_p(ids_ptr),
             C.c_void_p(dists_ptr or None), C.c_void_p(counters_ptr or None), C.c_void_p(stream or None)))
 
+    def search_batch_efk_dev(self, q_ptr: int, nq: int, ef_ptr: int, k_ptr: int, off_ptr: int, ids_ptr: int,
+                             max_ef: int, max_k: int, dists_ptr: int = 0, counters_ptr: int = 0, stream: int = 0):
+        """Device-resident batch with per-query ef / k (int32 device arrays)
+        and result offsets (int64 device array); max_ef / max_k bound every
+        entry (the launch's LDS is sized for them)."""
+        _lib.check(self._L.glass_hip_searcher_search_batch_efk_dev(
+            self._h, C.c_void_p(q_ptr), int(nq), C.c_void_p(ef_ptr), C.c_void_p(k_ptr), C.c_void_p(off_ptr),
+            int(max_ef), int(max_k), C.c_void_p(ids_ptr), C.c_void_p(dists_ptr or None),
+            C.c_void_p(counters_ptr or None), C.c_void_p(stream or None)))
+
     def optimize(self, num_threads: int = 0):
         """Autotune the launch configuration (GraphSearcher::Optimize)."""
         _lib.check(self._L.glass_hip_searcher_optimize(self._h, int(num_threads)))

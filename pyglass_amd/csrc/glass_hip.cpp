@@ -271,8 +271,14 @@ struct glass_hip_searcher {
     hipGraphExec_t gexec = nullptr;
     GraphKey gkey{};
 
+    // Launches without their own batch state share d_counter and d_visited:
+    // a launch on another stream than the previous one waits for it.
+    hipEvent_t shared_done = nullptr;
+    hipStream_t shared_stream = nullptr;
+
     ~glass_hip_searcher() {
         if (gexec) (void)hipGraphExecDestroy(gexec);
+        if (shared_done) (void)hipEventDestroy(shared_done);
         if (stream) (void)hipStreamDestroy(stream);
     }
 
@@ -641,6 +647,7 @@ int ensure_stream(glass_hip_searcher &s) {
     if (!s.stream) {
         HIP_TRY(hipGetDevice(&s.device));
         HIP_TRY(hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking));
+        HIP_TRY(hipEventCreateWithFlags(&s.shared_done, hipEventDisableTiming));
     }
     if (!s.cu_count) {
         int dev = 0;
@@ -685,6 +692,19 @@ bool spec_fits(const glass_hip_searcher &s, int32_t team) {
 // CU holds).
 int32_t round_cap(int32_t cap) { return (std::max(cap, 1) + 15) / 16 * 16; }
 
+// LDS of one CU (CDNA4: 160 KiB); a workgroup's per-wave areas must fit it.
+constexpr size_t kLdsPerCu = 160 * 1024;
+
+// The pool lives in LDS: a capacity is served when one wave's area fits a CU
+// with the smallest visited table (about 19,000 entries at 128 dimensions).
+int check_cap(const glass_hip_searcher &s, int32_t max_cap) {
+    const size_t need = lds_bytes_per_wave(s.index(), round_cap(max_cap), 256);
+    if (need > kLdsPerCu)
+        return fail(GLASS_HIP_EINVAL, "max(k, ef) = %d needs %zu bytes of LDS per query (a CU has %zu)", max_cap, need,
+                    kLdsPerCu);
+    return GLASS_HIP_OK;
+}
+
 // Per-launch state for batches that run concurrently on several streams:
 // their own work counter and visited bitsets (max_slots x vis_stride words),
 // and optionally (id, dist) pair outputs.
@@ -712,8 +732,16 @@ int run_batch(glass_hip_searcher &s, const float *qs_dev, int32_t nq, const int3
     lc.cap_max = round_cap(max_cap);
     lc.vis_hash = s.vis_override ? s.vis_override : (tn.vis_hash ? tn.vis_hash : glass_hip_searcher::auto_vis(max_ef));
     lc.spec = s.force_spec >= 0 ? s.force_spec : tn.spec;
-    const size_t lds = lds_bytes_per_wave(s.index(), lc.cap_max, lc.vis_hash) * (size_t)lc.waves_per_block;
-    if (lds > 160 * 1024) return fail(GLASS_HIP_EINVAL, "pool capacity %d needs %zu bytes of LDS", lc.cap_max, lds);
+    // Large pools: fewer waves per workgroup, then a smaller visited table
+    // (exact at any size: it moves to the global bitset when it fills), until
+    // a workgroup's LDS fits the CU.
+    auto lds_of = [&](const LaunchConfig &c) {
+        return lds_bytes_per_wave(s.index(), c.cap_max, c.vis_hash) * (size_t)c.waves_per_block;
+    };
+    while (lds_of(lc) > kLdsPerCu && lc.waves_per_block > 1) lc.waves_per_block -= 1;
+    while (lds_of(lc) > kLdsPerCu && lc.vis_hash > 256) lc.vis_hash /= 2;
+    const size_t lds = lds_of(lc);
+    if (lds > kLdsPerCu) return fail(GLASS_HIP_EINVAL, "pool capacity %d needs %zu bytes of LDS", lc.cap_max, lds);
     int32_t blocks = grid_blocks(s, lc, nq);
     if (bs) blocks = std::max(1, std::min(blocks, bs->max_slots / lc.waves_per_block));
     lc.blocks = blocks;
@@ -749,9 +777,18 @@ int run_batch(glass_hip_searcher &s, const float *qs_dev, int32_t nq, const int3
         return e == hipSuccess ? launch_search(ix, qb, lc, ss, stream) : e;
     };
     s.last_launch = lc;
+    if (!bs) {
+        // the shared counter and bitsets: order after the previous launch
+        if (s.shared_stream && s.shared_stream != stream) HIP_TRY(hipStreamWaitEvent(stream, s.shared_done, 0));
+        s.shared_stream = stream;
+    }
+    auto mark = [&]() -> int {
+        if (!bs) HIP_TRY(hipEventRecord(s.shared_done, stream));
+        return GLASS_HIP_OK;
+    };
     if (!graph) {
         HIP_TRY(enqueue());
-        return GLASS_HIP_OK;
+        return mark();
     }
     // Graph replay: the key is every kernel argument plus the stream, so a
     // replay launches exactly what an eager call with these arguments would.
@@ -763,11 +800,12 @@ int run_batch(glass_hip_searcher &s, const float *qs_dev, int32_t nq, const int3
     key.stream = stream;
     if (s.gexec && std::memcmp(&key, &s.gkey, sizeof(key)) == 0) {
         HIP_TRY(hipGraphLaunch(s.gexec, stream));
-        return GLASS_HIP_OK;
+        return mark();
     }
     // first call with these arguments: run eagerly (this also does any
     // one-time set-up), then capture the same two launches for next time
     HIP_TRY(enqueue());
+    if (int r = mark()) return r;
     if (s.gexec) {
         (void)hipGraphExecDestroy(s.gexec);
         s.gexec = nullptr;
@@ -823,7 +861,7 @@ int host_batch(glass_hip_searcher &s, const float *qs, int32_t nq, const int32_t
         total = (int64_t)nq * k_uniform;
         max_cap = std::max(inner_k(s, k_uniform), s.ef);
     }
-    if (round_cap(max_cap) > 4096) return fail(GLASS_HIP_EINVAL, "max(k, ef) = %d exceeds 4096", max_cap);
+    if (int r = check_cap(s, max_cap)) return r;
     HIP_TRY(s.d_q.alloc((size_t)nq * s.dim));
     HIP_TRY(s.d_ids.alloc((size_t)total));
     HIP_TRY(s.d_dists.alloc((size_t)total));
@@ -989,7 +1027,7 @@ void glass_hip_graph_free(glass_hip_graph *g) { delete g; }
 int glass_hip_build_graph(const char *index_type, const float *data, int32_t n, int32_t dim, const char *metric,
                           int32_t R, int32_t L, int32_t num_threads, glass_hip_graph **out, double *seconds) {
     try {
-        if (!index_type || !data || n <= 0 || dim <= 0 || !metric || !out || R < 2 || R > 64)
+        if (!index_type || !data || n <= 0 || dim <= 0 || !metric || !out || R < 2 || R > 128)
             return fail(GLASS_HIP_EINVAL, "bad build arguments");
         std::string t(index_type), m(metric);
         if (m != "L2" && m != "IP") return fail(GLASS_HIP_EINVAL, "unknown metric '%s'", metric);
@@ -1041,10 +1079,9 @@ int glass_hip_searcher_create(glass_hip_graph *g, const char *metric, const char
             else if (r == "FP16") rk = QK_FP16;
             else return fail(GLASS_HIP_EINVAL, "unsupported refine quantizer '%s'", refine_quant);
         }
-        if (g->K > 64) return fail(GLASS_HIP_EINVAL, "graph degree %d > 64 is not supported", g->K);
+        if (g->K > 128) return fail(GLASS_HIP_EINVAL, "graph degree %d > 128 is not supported", g->K);
         if (g->has_upper && g->up_K > 64) return fail(GLASS_HIP_EINVAL, "upper-layer degree %d > 64", g->up_K);
         if (!g->has_upper && g->eps.empty()) return fail(GLASS_HIP_EINVAL, "graph has no entry point");
-        if (!g->has_upper && g->eps.size() > 64) return fail(GLASS_HIP_EINVAL, "more than 64 entry points");
         auto s = std::make_unique<glass_hip_searcher>();
         s->graph = std::move(*g);
         *g = glass_hip_graph();  // the caller's graph is left empty (bindings.cc:94-98)
@@ -1201,9 +1238,27 @@ int glass_hip_searcher_search_batch_dev(glass_hip_searcher *s, const float *qs_d
         if (mode != MODE_SEARCH && mode != MODE_BATCH) return fail(GLASS_HIP_EINVAL, "mode must be 0 or 1");
         std::lock_guard<std::mutex> lk(s->mu);
         const int32_t max_cap = std::max(inner_k(*s, k), s->ef);
-        if (round_cap(max_cap) > 4096) return fail(GLASS_HIP_EINVAL, "max(k, ef) = %d exceeds 4096", max_cap);
+        if (int r = check_cap(*s, max_cap)) return r;
         return run_batch(*s, qs_dev, nq, nullptr, nullptr, s->ef, k, max_cap, s->ef, mode, nullptr, dst_dev, scores_dev,
                          counters_dev, (hipStream_t)stream, nullptr, s->graph_capture);
+    } catch (...) {
+        return exception_status();
+    }
+}
+
+int glass_hip_searcher_search_batch_efk_dev(glass_hip_searcher *s, const float *qs_dev, int32_t nq,
+                                            const int32_t *ef_dev, const int32_t *k_dev, const int64_t *off_dev,
+                                            int32_t max_ef, int32_t max_k, int32_t *dst_dev, float *scores_dev,
+                                            int32_t *counters_dev, void *stream) {
+    try {
+        if (!s || nq < 0 || max_ef <= 0 || max_k <= 0 ||
+            (nq && (!qs_dev || !ef_dev || !k_dev || !off_dev || !dst_dev)))
+            return fail(GLASS_HIP_EINVAL, "bad batch arguments");
+        std::lock_guard<std::mutex> lk(s->mu);
+        const int32_t max_cap = std::max(inner_k(*s, max_k), max_ef);
+        if (int r = check_cap(*s, max_cap)) return r;
+        return run_batch(*s, qs_dev, nq, ef_dev, k_dev, 0, 0, max_cap, max_ef, MODE_BATCH, off_dev, dst_dev,
+                         scores_dev, counters_dev, (hipStream_t)stream, nullptr, s->graph_capture);
     } catch (...) {
         return exception_status();
     }
@@ -1273,7 +1328,7 @@ int glass_hip_searcher_optimize(glass_hip_searcher *s, int32_t num_threads) {
                             const size_t lds = lds_bytes_per_wave(
                                                    s->index(), round_cap(std::max(inner_k(*s, kTryK), s->ef)), v) *
                                                (size_t)w;
-                            if (lds > 160 * 1024) continue;
+                            if (lds > kLdsPerCu) continue;
                             timeit();
                             double e = std::min(timeit(), timeit());
                             if (e < best) {
@@ -1553,6 +1608,8 @@ struct ServerReq {
     std::vector<float> own_dists;
     int64_t arrival_ns = 0, done_ns = 0;
     int32_t state = RQ_QUEUED;
+    int32_t err = GLASS_HIP_OK;      // RQ_FAILED by its own batch: the status and message
+    std::string err_msg;
     int64_t *group_done = nullptr;   // replay(): completed-request count of its call (under the server lock)
 };
 
@@ -1615,9 +1672,26 @@ void server_fail_all(glass_hip_server &sv, int code, const std::string &msg) {
     for (ServerReq *r : sv.pending) r->state = RQ_FAILED;
     sv.pending.clear();
     sv.pending_n.store(0);
+    // completed requests are no longer listed (server_collect clears them):
+    // their owners may already have released them
     for (auto &sl : sv.slots)
         for (ServerReq *r : sl->reqs)
-            if (r->state != RQ_DONE) r->state = RQ_FAILED;
+            if (r && r->state != RQ_DONE) r->state = RQ_FAILED;
+    sv.cv_done.notify_all();
+}
+
+// A batch that could not be launched fails its own requests only; the
+// dispatcher and every other request carry on.
+void server_fail_batch(glass_hip_server &sv, ServerSlot &sl, int code, const std::string &msg) {
+    (void)hipStreamSynchronize(sl.stream);
+    std::lock_guard<std::mutex> lk(sv.mu);
+    for (ServerReq *r : sl.reqs) {
+        r->state = RQ_FAILED;
+        r->err = code;
+        r->err_msg = msg;
+        if (r->group_done) *r->group_done += 1;
+    }
+    sl.reqs.clear();
     sv.cv_done.notify_all();
 }
 
@@ -1704,6 +1778,7 @@ bool server_collect(glass_hip_server &sv, ServerSlot &sl, bool all) {
         r->state = RQ_DONE;
         if (r->group_done) *r->group_done += 1;
         sl.req_done[i] = 1;
+        sl.reqs[i] = nullptr;  // its owner may release it from now on
         sl.pending -= 1;
         sv.n_requests += 1;
         sv.n_queries += r->nq;
@@ -1773,12 +1848,11 @@ void server_loop(glass_hip_server *psv) {
             }
             if (!sl.reqs.empty()) {
                 if (int rc = server_launch(sv, sl)) {
-                    server_fail_all(sv, rc, g_last_error);
-                    drain();
-                    return;
+                    server_fail_batch(sv, sl, rc, g_last_error);
+                } else {
+                    running.push_back(free_slots.back());
+                    free_slots.pop_back();
                 }
-                running.push_back(free_slots.back());
-                free_slots.pop_back();
                 progressed = true;
             }
         }
@@ -1795,8 +1869,7 @@ int server_enqueue(glass_hip_server &sv, ServerReq *r) {
         return fail(GLASS_HIP_EINVAL, "request of %d queries (max_batch %d)", r->nq, sv.max_batch);
     if (r->k <= 0 || r->k > sv.max_k || r->ef <= 0)
         return fail(GLASS_HIP_EINVAL, "bad ef %d / k %d (max_k %d)", r->ef, r->k, sv.max_k);
-    if (round_cap(std::max(inner_k(*sv.s, r->k), r->ef)) > 4096)
-        return fail(GLASS_HIP_EINVAL, "max(k, ef) = %d exceeds 4096", std::max(inner_k(*sv.s, r->k), r->ef));
+    if (int rc = check_cap(*sv.s, std::max(inner_k(*sv.s, r->k), r->ef))) return rc;
     return GLASS_HIP_OK;
 }
 
@@ -1909,8 +1982,10 @@ static int server_take(glass_hip_server *sv, std::unique_lock<std::mutex> &lk, i
     ServerReq *r = it->second.get();
     if (wait) sv->cv_done.wait(lk, [&] { return r->state == RQ_DONE || r->state == RQ_FAILED; });
     if (r->state == RQ_FAILED) {
+        const int code = r->err ? r->err : (sv->err_code ? sv->err_code : GLASS_HIP_EDEVICE);
+        const std::string msg = r->err ? r->err_msg : sv->err_msg;
         sv->owned.erase(it);
-        return fail(sv->err_code ? sv->err_code : GLASS_HIP_EDEVICE, "%s", sv->err_msg.c_str());
+        return fail(code, "%s", msg.c_str());
     }
     if (r->state != RQ_DONE) {
         if (done) *done = 0;
@@ -2015,6 +2090,8 @@ int glass_hip_server_replay(glass_hip_server *sv, const float *qs, int64_t nreq,
             sv->pending_n.store((int64_t)sv->pending.size());
             return fail(sv->err_code, "%s", sv->err_msg.c_str());
         }
+        for (const ServerReq &r : reqs)
+            if (r.state == RQ_FAILED) return fail(r.err ? r.err : GLASS_HIP_EDEVICE, "%s", r.err_msg.c_str());
         for (int64_t i = 0; i < nreq; ++i) latency_ns[i] = reqs[(size_t)i].done_ns - reqs[(size_t)i].arrival_ns;
         return GLASS_HIP_OK;
     } catch (...) {

@@ -1237,6 +1237,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void s
         ps.size = 0;
         ps.cur = 0;
         int32_t cnt_d = 0, cnt_e = 0, cnt_u = 0;
+        // A pool larger than the launch sized LDS for (a caller's bound on ef
+        // and k that did not hold) is never filled: the query is answered
+        // with an empty pool (ids -1) instead.
+        const bool fits = ps.cap <= lc.cap_max;
 
         // stage the fp32 query (zero padded, as MemCpyTag copies into a zeroed buffer)
         const float *x = qb.queries + (int64_t)qi * qb.dim;
@@ -1291,7 +1295,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void s
         };
 
         // initialize_search (graph.hpp:95-104)
-        if (ix.has_init) {
+        if (!fits) {
+        } else if (ix.has_init) {
             // HNSWInitializer::initialize (HNSWInitializer.hpp:41-62)
             uint32_t u = (uint32_t)ix.ep;
             const int top = uni(ix.levels[ix.ep]);
@@ -1347,20 +1352,27 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void s
             ps.cur = 0;
             wave_sync();
         } else {
-            // insert every entry point in order, then mark it visited
-            const int nep = ix.nep < 64 ? ix.nep : 64;
-            if (l < nep) w.ci[l] = (uint32_t)ix.eps[l];
-            wave_sync();
-            dists(nep, qb.mode | kInitCtx);
-            cnt_d += nep;
-            if (lc.cap_max <= 64 * kMaxChunks) {
-                int32_t unused = -1;
-                pool_merge_reg_lds<kMaxChunks>(ix, w, ps, nep, unused);
-            } else {
-                pool_merge_lds(w, ps, nep);
+            // insert every entry point in order, then mark it visited, 64 at
+            // a time (insert never consults the visited set, so marking a
+            // chunk after inserting it changes nothing).  A repeated entry
+            // point is inserted again, as the reference's loop does.
+            for (int e0 = 0; e0 < ix.nep; e0 += 64) {
+                const int nep = ix.nep - e0 < 64 ? ix.nep - e0 : 64;
+                const uint32_t ep = l < nep ? (uint32_t)ix.eps[e0 + l] : 0u;
+                if (l < nep) w.ci[l] = ep;
+                wave_sync();
+                dists(nep, qb.mode | kInitCtx);
+                cnt_d += nep;
+                if (lc.cap_max <= 64 * kMaxChunks) {
+                    int32_t unused = -1;
+                    pool_merge_reg_lds<kMaxChunks>(ix, w, ps, nep, unused);
+                } else {
+                    pool_settle(w, ps);
+                    pool_merge_lds(w, ps, nep);
+                }
+                vs.test_set(ep, l < nep);
+                vs.account(nep);
             }
-            vs.test_set(l < nep ? (uint32_t)ix.eps[l] : 0u, l < nep);
-            vs.account(nep);
         }
 
         // best-first expansion (SearchImpl2, graph_searcher.hpp:196-224).
@@ -1380,7 +1392,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void s
         const uint64_t lead_k = ballot(iq.part == 0 && nj < ix.K);  // first lane of each group with an entry
         uint32_t pred = kIdMask;
         int32_t pred_row = -1;
-        while (ps.cur < ps.size && ps.cur < ps.ef) {
+        while (fits && ps.cur < ps.size && ps.cur < ps.ef) {
             const uint32_t u = pool_pop(w, ps, pred);
             cnt_e += 1;
             const int32_t v = (u == pred) ? pred_row : (nj < ix.K ? ix.adj[(int64_t)u * ix.K + nj] : -1);
@@ -1421,35 +1433,48 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void s
                 }
                 continue;
             }
-            const uint64_t inv = ballot(l < ix.K && v == -1);
-            const int n = inv ? __builtin_ctzll(inv) : ix.K;
-            const bool isnew = vs.test_set((uint32_t)v, l < n);
-            const uint64_t nm = ballot(isnew);
-            if (nm == 0) {
-                // nothing new: the pool is unchanged and the next pop takes the
-                // first unchecked entry
-                pred = kIdMask;
+            // The row in chunks of 64 entries (rows of up to 128): each chunk's
+            // new neighbours are merged before the next chunk is read, which
+            // leaves the pool as the reference's one-at-a-time insertion in
+            // adjacency order does.  The first chunk may come from the
+            // prefetch; a merge's prefetch covers the first chunk of the next
+            // node's row.
+            pred = kIdMask;
+            int32_t vc = v;
+            for (int c0 = 0;; c0 += 64) {
+                const int rem = ix.K - c0;  // row entries from c0 on
+                const uint64_t inv = ballot(l < rem && vc == -1);
+                const int n = inv ? __builtin_ctzll(inv) : (rem < 64 ? rem : 64);
+                const bool isnew = vs.test_set((uint32_t)vc, l < n);
+                const uint64_t nm = ballot(isnew);
+                if (nm) {
+                    const int m = __builtin_popcountll(nm);
+                    vs.account(m);
+                    if (isnew) w.ci[mbcnt(nm)] = (uint32_t)vc;
+                    wave_sync();
+                    dists(m, qb.mode);
+                    cnt_d += m;
+                    if (kFast || at_use(lc.cap_max) <= 128) {
+                        pred = pool_merge_reg_lds<2>(ix, w, ps, m, pred_row);
+                    } else if (lc.cap_max <= 64 * kMaxChunks) {
+                        pred = pool_merge_reg_lds<kMaxChunks>(ix, w, ps, m, pred_row);
+                    } else {
+                        pool_settle(w, ps);
+                        pool_merge_lds(w, ps, m);
+                        pred = kIdMask;
+                    }
+                }
+                if (inv || rem <= 64) break;
+                vc = c0 + 64 + l < ix.K ? ix.adj[(int64_t)u * ix.K + c0 + 64 + l] : -1;
+            }
+            if (pred == kIdMask) {
+                // the pool did not change since the last merge gave no next
+                // node: the next pop takes the first unchecked entry
                 pool_settle(w, ps);
                 if (ps.cur < ps.size && ps.cur < ps.ef) {
                     pred = uni(w.pid[ps.cur]) & kIdMask;
                     pred_row = nj < ix.K ? ix.adj[(int64_t)pred * ix.K + nj] : -1;
                 }
-                continue;
-            }
-            const int m = __builtin_popcountll(nm);
-            vs.account(m);
-            if (isnew) w.ci[mbcnt(nm)] = (uint32_t)v;
-            wave_sync();
-            dists(m, qb.mode);
-            cnt_d += m;
-            if (kFast || at_use(lc.cap_max) <= 128) {
-                pred = pool_merge_reg_lds<2>(ix, w, ps, m, pred_row);
-            } else if (lc.cap_max <= 64 * kMaxChunks) {
-                pred = pool_merge_reg_lds<kMaxChunks>(ix, w, ps, m, pred_row);
-            } else {
-                pool_settle(w, ps);
-                pool_merge_lds(w, ps, m);
-                pred = kIdMask;
             }
         }
 
@@ -1498,9 +1523,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void s
                 rerank<V_FP32_IP>(ix, w, m);
             }
             wave_sync();
+            // Ranks by (dist, id), with equal pairs (a repeated entry point
+            // is in the pool twice, as in the reference's) ranked by pool
+            // position, so the ranks are a permutation and each output
+            // position is written once.  Equal pairs are indistinguishable,
+            // so this is MaxHeap's output order.
             if (m <= 128) {
-                // (key, id) pairs in registers, two per lane; ids are distinct,
-                // so the 64-bit order is total and is the (dist, id) order
+                // (key, id) pairs in registers, two per lane
                 const int j1 = l + 64;
                 const uint32_t ka = w.pk[l < m ? l : 0], ia = w.pid[l < m ? l : 0] & kIdMask;
                 const uint32_t kb = w.pk[j1 < m ? j1 : 0], ib = w.pid[j1 < m ? j1 : 0] & kIdMask;
@@ -1510,14 +1539,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void s
                 for (int i = 0; i < m0; ++i) {
                     const uint64_t ci = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)ka, i) << 32) |
                                         (uint32_t)__builtin_amdgcn_readlane((int)ia, i);
-                    ra += ci < ca;
-                    rb += ci < cb;
+                    ra += (ci < ca) | ((ci == ca) & (i < l));
+                    rb += ci <= cb;  // every i here is before position j1
                 }
                 for (int i = 64; i < m; ++i) {
                     const uint64_t ci = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)kb, i - 64) << 32) |
                                         (uint32_t)__builtin_amdgcn_readlane((int)ib, i - 64);
-                    ra += ci < ca;
-                    rb += ci < cb;
+                    ra += ci < ca;  // every i here is after position l
+                    rb += (ci < cb) | ((ci == cb) & (i < j1));
                 }
                 if (l < m && ra < k) put_result(qb, ob + ra, (int32_t)ia, float_of_key(ka));
                 if (j1 < m && rb < k) put_result(qb, ob + rb, (int32_t)ib, float_of_key(kb));
@@ -1527,7 +1556,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void s
                     int rank = 0;
                     for (int i = 0; i < m; ++i) {
                         const uint32_t ki = w.pk[i], ii = w.pid[i] & kIdMask;
-                        rank += (ki < kj) | ((ki == kj) & (ii < ij));
+                        rank += (ki < kj) | ((ki == kj) & (ii < ij)) | ((ki == kj) & (ii == ij) & (i < j));
                     }
                     if (rank < k) put_result(qb, ob + rank, (int32_t)ij, float_of_key(kj));
                 }

@@ -115,4 +115,4 @@ def test_builder_rejects_bad_arguments():
     with pytest.raises(pg.GlassError):
         pg.Index("HNSW", "COSINE").build(X)
     with pytest.raises(pg.GlassError):
-        pg.Index("NSG", "L2", "FP32", 128, 64).build(X)  # degree above the 64 the searcher supports
+        pg.Index("NSG", "L2", "FP32", 160, 64).build(X)  # degree above the 128 the searcher supports

@@ -192,6 +192,113 @@ def test_device_pointer_entry_point_matches_host_batch(c1):
     assert cnt.cpu().numpy()[:, 0].mean() == s.get_last_search_dist_cmps()
 
 
+def test_device_batches_on_two_streams_back_to_back(c1):
+    """Two device-pointer batches on different streams with no host sync
+    between them share the searcher's work counter and visited bitsets; the
+    second launch is ordered after the first, so both match the host path."""
+    import torch
+
+    X, Q, path = c1
+    s = pg.Searcher(pg.Graph(path), X, "L2", "SQ4U", "FP32")
+    s.set_ef(64)
+    I1, D1 = s.batch_search(Q[:400], 10)
+    s.set_ef(128)
+    I2, D2 = s.batch_search(Q[400:1000], 10)
+    dev = torch.device("cuda:0")
+    q = torch.from_numpy(Q[:1000].copy()).to(dev)
+    out = [(torch.full((n, 10), -7, dtype=torch.int32, device=dev), torch.zeros((n, 10), dtype=torch.float32,
+                                                                                 device=dev)) for n in (400, 600)]
+    st = [torch.cuda.Stream(device=dev), torch.cuda.Stream(device=dev)]
+    torch.cuda.synchronize()
+    print("two streams back to back", flush=True)
+    s.set_graph_capture(False)
+    for rep in range(3):
+        s.set_ef(64)
+        s.search_batch_dev(q.data_ptr(), 400, 10, out[0][0].data_ptr(), out[0][1].data_ptr(), 0,
+                           stream=st[0].cuda_stream)
+        s.set_ef(128)
+        s.search_batch_dev(q[400:].data_ptr(), 600, 10, out[1][0].data_ptr(), out[1][1].data_ptr(), 0,
+                           stream=st[1].cuda_stream)
+        torch.cuda.synchronize()
+        np.testing.assert_array_equal(out[0][0].cpu().numpy(), I1)
+        np.testing.assert_array_equal(out[0][1].cpu().numpy(), D1)
+        np.testing.assert_array_equal(out[1][0].cpu().numpy(), I2)
+        np.testing.assert_array_equal(out[1][1].cpu().numpy(), D2)
+
+
+@pytest.mark.parametrize("ef", [1500, 3500])
+def test_large_pools_fit_lds_by_fewer_waves(c1, ef):
+    """Pools whose LDS at the tuned shape exceeds a CU run with fewer waves
+    per workgroup (and, if needed, a smaller visited table)."""
+    X, Q, path = c1
+    s = pg.Searcher(pg.Graph(path), X, "L2", "SQ8U", "FP32")
+    o = O.Searcher(O.Graph(path), X, "L2", "SQ8U", "FP32")
+    s.set_ef(ef)
+    o.set_ef(ef)
+    print(f"large pool ef={ef}", flush=True)
+    I, D = s.batch_search(Q[:64], 10)
+    Io, Do = o.batch_search(Q[:64], 10)
+    np.testing.assert_array_equal(I, Io)
+    np.testing.assert_array_equal(D, Do)
+    ki = s.kernel_info()
+    assert ki["lds_per_block"] <= 160 * 1024
+
+
+@pytest.mark.parametrize("kind,R", [("HNSW", 128), ("NSG", 100)])
+@pytest.mark.parametrize("quant,refine", [("SQ8U", ""), ("SQ4U", "FP32"), ("FP32", "")])
+def test_degree_above_64_matches_oracle(tmp_path, kind, R, quant, refine):
+    """Rows of more than 64 neighbours (HNSW R=128: 128 at level 0, 64 in the
+    upper layers; NSG R=100) are read in chunks of 64 entries."""
+    X = mixture(4000, 64, 3)
+    Q = mixture(128, 64, 4)
+    path = str(tmp_path / f"{kind}{R}.glass")
+    pg.build_graph(kind, X, path, "L2", "FP32", R, 200 if kind == "HNSW" else R + 32)
+    G = pg.Graph(path)
+    assert G.info()["K"] == R
+    s = pg.Searcher(G, X, "L2", quant, refine)
+    o = O.Searcher(O.Graph(path), X, "L2", quant, refine)
+    print(f"degree {R} {kind} {quant}+{refine or 'none'}", flush=True)
+    for ef in [16, 100]:
+        s.set_ef(ef)
+        o.set_ef(ef)
+        I, D = s.batch_search(Q, 10)
+        Io, Do = o.batch_search(Q, 10)
+        np.testing.assert_array_equal(I, Io)
+        np.testing.assert_array_equal(D, Do)
+        assert s.get_last_search_dist_cmps() == o.get_last_search_dist_cmps()
+        for i in range(8):
+            si, sd = s.search(Q[i], 10)
+            oi, od = o.search(Q[i], 10)
+            np.testing.assert_array_equal(si, oi)
+            np.testing.assert_array_equal(sd, od)
+
+
+def test_pool_beyond_4096_entries(c1):
+    """Pools are bounded by a CU's LDS, not by a fixed count: ef = 6000 and
+    a re-rank depth of 4,500 entries against the oracle; a pool too large
+    for LDS is an error, not a truncation."""
+    X, Q, path = c1
+    s = pg.Searcher(pg.Graph(path), X, "L2", "SQ4U", "FP32")
+    o = O.Searcher(O.Graph(path), X, "L2", "SQ4U", "FP32")
+    print("pool ef=6000", flush=True)
+    s.set_ef(6000)
+    o.set_ef(6000)
+    I, D = s.batch_search(Q[:16], 10)
+    Io, Do = o.batch_search(Q[:16], 10)
+    np.testing.assert_array_equal(I, Io)
+    np.testing.assert_array_equal(D, Do)
+    s.set_ef(64)
+    o.set_ef(64)
+    print("pool reorder_k=4500", flush=True)
+    I, D = s.batch_search(Q[:8], 3000)
+    Io, Do = o.batch_search(Q[:8], 3000)
+    np.testing.assert_array_equal(I, Io)
+    np.testing.assert_array_equal(D, Do)
+    s.set_ef(40000)
+    with pytest.raises(pg.GlassError):
+        s.batch_search(Q[:2], 10)
+
+
 def test_empty_batch_and_bad_arguments(c1):
     X, Q, path = c1
     s = pg.Searcher(pg.Graph(path), X, "L2", "SQ8U")

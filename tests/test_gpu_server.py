@@ -92,7 +92,7 @@ def test_poll_cancel_and_errors(setup):
     with pytest.raises(pg.GlassError):
         srv.submit(Q[:2], k=17, ef=32)  # k above max_k
     with pytest.raises(pg.GlassError):
-        srv.submit(Q[:2], k=5, ef=5000)  # pool above the LDS limit
+        srv.submit(Q[:2], k=5, ef=40000)  # a pool larger than a CU's LDS
     srv.close()
 
 
@@ -114,3 +114,68 @@ def test_replay_poisson_matches_batch(setup):
     st = srv.stats()
     assert st["queries"] == n and st["batches"] > 1
     srv.close()
+
+
+def test_large_pool_request_does_not_stop_the_server(setup):
+    """A request whose pool needs fewer waves per workgroup to fit LDS runs
+    like any other, and the requests after it are served."""
+    X, Q, path, s = setup
+    srv = pg.Server(s, max_batch=1024, max_k=16, max_inflight=2)
+    print("server ef=3500 then ef=64", flush=True)
+    t1 = srv.submit(Q[:4], k=10, ef=3500)
+    t2 = srv.submit(Q[4:6], k=10, ef=64)
+    i1, d1 = srv.wait(t1)
+    i2, d2 = srv.wait(t2)
+    for (ids, dis), rows, ef in [((i1, d1), slice(0, 4), 3500), ((i2, d2), slice(4, 6), 64)]:
+        bi, bd = s.batch_search_efk(Q[rows], ef, 10)
+        np.testing.assert_array_equal(ids, np.stack(bi))
+        np.testing.assert_array_equal(dis, np.stack(bd))
+    i3, _ = srv.wait(srv.submit(Q[9], k=10, ef=32))
+    assert i3.shape == (1, 10)
+    srv.close()
+
+
+@pytest.mark.parametrize("quant", ["SQ8", "SQ8U"])
+def test_served_768d_ip_requests(tmp_path, quant):
+    """The serving workload's shape at reduced N: 768-d unit rows, NSG R=48,
+    inner product, per-dim SQ8 (float) or SQ8U (integer) codes + FP32
+    re-rank; requests of U[1, 256] queries with per-request ef in {32, 64,
+    128, 256} and k = 10 arriving as a Poisson process.  Each request gets
+    the same ids and distances alone, in the mix, through
+    batch_search_efk and from the oracle."""
+    X = synth.unit_rows(3000, 768, synth.BASE_SEED)
+    Q = synth.unit_rows(1500, 768, synth.QUERY_SEED)
+    path = str(tmp_path / "g768.glass")
+    pg.build_graph("NSG", X, path, "IP", "FP32", 48, 80)
+    s = pg.Searcher(pg.Graph(path), X, "IP", quant, "FP32")
+    rng = np.random.default_rng(synth.ARRIVAL_SEED)
+    nreq = 24
+    nq = rng.integers(1, 257, nreq).astype(np.int32)
+    ef = rng.choice([32, 64, 128, 256], nreq).astype(np.int32)
+    starts = np.concatenate([[0], np.cumsum(nq)])
+    rows = np.arange(starts[-1]) % len(Q)
+    arr = np.cumsum(rng.exponential(1e9 / 20000, nreq)).astype(np.int64)
+    srv = pg.Server(s, max_batch=1024, max_k=10, max_inflight=3)
+    alone = {}
+    for j in (0, 7, 15):
+        r = rows[starts[j]:starts[j + 1]]
+        alone[j] = srv.wait(srv.submit(Q[r], k=10, ef=int(ef[j])))
+    print(f"served 768-d IP {quant}+FP32: {nreq} requests, {starts[-1]} queries", flush=True)
+    lat, ids, dis = srv.replay(Q[rows], nq, ef, 10, arr)
+    srv.close()
+    o = O.Searcher(O.Graph(path), X, "IP", quant, "FP32")
+    for j in range(nreq):
+        r = rows[starts[j]:starts[j + 1]]
+        mi = ids[starts[j] * 10:starts[j + 1] * 10].reshape(-1, 10)
+        md = dis[starts[j] * 10:starts[j + 1] * 10].reshape(-1, 10)
+        bi, bd = s.batch_search_efk(Q[r], int(ef[j]), 10)
+        np.testing.assert_array_equal(mi, np.stack(bi), err_msg=f"request {j}")
+        np.testing.assert_array_equal(md, np.stack(bd), err_msg=f"request {j}")
+        if j in alone:
+            np.testing.assert_array_equal(mi, alone[j][0])
+            np.testing.assert_array_equal(md, alone[j][1])
+        o.set_ef(int(ef[j]))
+        oi, od = o.batch_search(Q[r], 10)
+        np.testing.assert_array_equal(mi, oi, err_msg=f"request {j} vs oracle")
+        np.testing.assert_array_equal(md, od, err_msg=f"request {j} vs oracle")
+    assert np.all(lat > 0)
