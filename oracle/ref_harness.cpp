@@ -202,6 +202,15 @@ void *ref_searcher_new_factor(const char *graph_path, const float *data, int n, 
     return h;
 }
 
+// Graph::load(filename, format) then Graph::save (glass/graph.hpp:106-153):
+// converts a foreign graph file (diskann, nsg, hnswlib) to the .glass layout.
+int ref_graph_convert(const char *src, const char *format, const char *dst) {
+    glass::Graph<int> g;
+    g.load(src, format);
+    g.save(dst);
+    return g.N;
+}
+
 void ref_searcher_free(void *h) { delete (RefSearcher *)h; }
 
 void ref_set_ef(void *h, int ef) { ((RefSearcher *)h)->s->SetEf(ef); }

@@ -71,6 +71,14 @@ def build_graph(index_type: str, data: np.ndarray, path: str, metric: str = "L2"
     return t
 
 
+def graph_convert(src: str, fmt: str, dst: str, portable: bool = False) -> int:
+    """glass::Graph::load(src, fmt) + save(dst); returns N."""
+    lib = load(portable)
+    lib.ref_graph_convert.argtypes = [C.c_char_p, C.c_char_p, C.c_char_p]
+    lib.ref_graph_convert.restype = C.c_int
+    return lib.ref_graph_convert(src.encode(), fmt.encode(), dst.encode())
+
+
 class RefSearcher:
     """Mirror of glass.Searcher (python/bindings.cc:90-154) over the harness."""
 

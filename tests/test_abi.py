@@ -147,3 +147,16 @@ def test_host_allocation_failure_returns_a_status():
     c = pg.Clustering(2**31 - 1, 1, "kmeans++")
     with pytest.raises(pg.GlassError, match=r"error -3: host allocation failed"):
         c.fit(np.zeros((2, 1 << 20), np.float32))
+
+def test_diskann_format_matches_reference(golden_dir):
+    """Graph(path, "diskann") reads a ragged diskann file as the reference's
+    Graph::load_diskann does (glass/graph.hpp:155-182): its .glass save of the
+    same file (tests/golden/make_golden_formats.py) has the same adjacency,
+    -1 padding and entry point."""
+    g = pg.Graph(os.path.join(golden_dir, "diskann_1k.bin"), "diskann")
+    r = pg.Graph(os.path.join(golden_dir, "diskann_1k_ref.glass"))
+    assert g.info() == r.info() == {"N": 1000, "K": 24, "nep": 1, "has_upper": False}
+    (ga, ge), (ra, re_) = g.arrays(), r.arrays()
+    np.testing.assert_array_equal(ga, ra)
+    np.testing.assert_array_equal(ge, re_)
+    assert ge[0] == 417 and (ga == -1).any() and (ga[:, 0] >= 0).any()
