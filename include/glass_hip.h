@@ -111,6 +111,20 @@ int glass_hip_searcher_search_batch_efk(glass_hip_searcher *s, const float *qs, 
 int glass_hip_searcher_search_batch_dev(glass_hip_searcher *s, const float *qs_dev, int32_t nq, int32_t k,
                                         int32_t mode, int32_t *dst_dev, float *scores_dev, int32_t *counters_dev,
                                         void *stream);
+/* search_batch_dev that allocates nothing.  No counterpart in the reference:
+ * it is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The work
+ * counter and the per-wave overflow bitsets (and PQ8 lookup tables) live in a
+ * fixed 128 MiB device array that every GPU the library runs on holds until
+ * the process ends; the persistent grid is capped to the waves whose slot
+ * areas fit in it (N/8 bytes each, 4 waves at least: N up to about 268M
+ * rows), and larger requests fail with GLASS_HIP_EINVAL.  Requires set_data
+ * first; launches eagerly (no graph replay).  Every call on a GPU shares that
+ * GPU's array, so calls that overlap on one GPU race: order them (one stream,
+ * or events). */
+int glass_hip_searcher_search_batch_dev_noalloc(glass_hip_searcher *s, const float *qs_dev, int32_t nq, int32_t k,
+                                                int32_t mode, int32_t *dst_dev, float *scores_dev,
+                                                int32_t *counters_dev, void *stream);
 /* Device-resident batch with per-query ef_dev[nq] and k_dev[nq] (SearchBatch
  * semantics per query); results of query i start at off_dev[i] in dst_dev /
  * scores_dev.  max_ef and max_k bound every entry of ef_dev and k_dev (the

@@ -245,6 +245,15 @@ class Searcher:
             self._h, C.c_void_p(q_ptr), int(nq), int(k), int(mode), C.c_void_p(ids_ptr),
             C.c_void_p(dists_ptr or None), C.c_void_p(counters_ptr or None), C.c_void_p(stream or None)))
 
+    def search_batch_dev_noalloc(self, q_ptr: int, nq: int, k: int, ids_ptr: int, dists_ptr: int = 0,
+                                 counters_ptr: int = 0, stream: int = 0, mode: int = 1):
+        """search_batch_dev without any device allocation: the library's fixed
+        per-device scratch array holds the work counter and overflow bitsets.
+        Calls that overlap on one GPU race (they share that array)."""
+        _lib.check(self._L.glass_hip_searcher_search_batch_dev_noalloc(
+            self._h, C.c_void_p(q_ptr), int(nq), int(k), int(mode), C.c_void_p(ids_ptr),
+            C.c_void_p(dists_ptr or None), C.c_void_p(counters_ptr or None), C.c_void_p(stream or None)))
+
     def search_batch_efk_dev(self, q_ptr: int, nq: int, ef_ptr: int, k_ptr: int, off_ptr: int, ids_ptr: int,
                              max_ef: int, max_k: int, dists_ptr: int = 0, counters_ptr: int = 0, stream: int = 0):
         """Device-resident batch with per-query ef / k (int32 device arrays)

@@ -41,6 +41,7 @@ SIGNATURES = {
     "glass_hip_searcher_free": (None, [_vp]),
     "glass_hip_searcher_search_batch_efk": (C.c_int, [_vp, _vp, _i32, _vp, _vp, _vp, _vp, _vp]),
     "glass_hip_searcher_search_batch_dev": (C.c_int, [_vp, _vp, _i32, _i32, _i32, _vp, _vp, _vp, _vp]),
+    "glass_hip_searcher_search_batch_dev_noalloc": (C.c_int, [_vp, _vp, _i32, _i32, _i32, _vp, _vp, _vp, _vp]),
     "glass_hip_searcher_search_batch_efk_dev": (C.c_int, [_vp, _vp, _i32, _vp, _vp, _vp, _i32, _i32, _vp, _vp, _vp,
                                                           _vp]),
     "glass_hip_searcher_set_graph_capture": (C.c_int, [_vp, _i32]),

@@ -670,9 +670,15 @@ int32_t grid_blocks(const glass_hip_searcher &s, const LaunchConfig &lc, int32_t
     return std::max(1, std::min(full, need));
 }
 
+// Words of slot scratch per resident wave: the PQ8 lookup table, then an
+// N-bit visited bitset (a multiple of 4 words).
+int64_t slot_stride(const glass_hip_searcher &s) {
+    return s.lut_words + ((int64_t)(s.graph.N + 31) / 32 + 3) / 4 * 4;
+}
+
 int ensure_slots(glass_hip_searcher &s, int32_t slots) {
     if (slots <= s.nslots && s.d_visited.p) return GLASS_HIP_OK;
-    s.vis_stride = s.lut_words + ((int64_t)(s.graph.N + 31) / 32 + 3) / 4 * 4;
+    s.vis_stride = slot_stride(s);
     HIP_TRY(s.d_visited.alloc((size_t)slots * (size_t)s.vis_stride));
     s.nslots = slots;
     return GLASS_HIP_OK;
@@ -1109,7 +1115,8 @@ int glass_hip_searcher_set_data(glass_hip_searcher *s, const float *data, int32_
         s->dim = dim;
         quant_layout(s->quant, dim, &s->dim_align, &s->code_size);
         s->lut_words = s->quant == QK_PQ8 ? (int64_t)s->code_size * 256 : 0;
-        s->nslots = 0;  // the slot stride depends on lut_words and N: recomputed at the next launch
+        s->nslots = 0;  // the slot stride depends on lut_words and N: buffers are re-sized at the next launch
+        s->vis_stride = slot_stride(*s);
         int32_t r_row_bytes = 0;
         if (s->refine != QK_NONE) quant_layout(s->refine, dim, &s->r_dim_align, &r_row_bytes);
         // graph
@@ -1241,6 +1248,38 @@ int glass_hip_searcher_search_batch_dev(glass_hip_searcher *s, const float *qs_d
         if (int r = check_cap(*s, max_cap)) return r;
         return run_batch(*s, qs_dev, nq, nullptr, nullptr, s->ef, k, max_cap, s->ef, mode, nullptr, dst_dev, scores_dev,
                          counters_dev, (hipStream_t)stream, nullptr, s->graph_capture);
+    } catch (...) {
+        return exception_status();
+    }
+}
+
+int glass_hip_searcher_search_batch_dev_noalloc(glass_hip_searcher *s, const float *qs_dev, int32_t nq, int32_t k,
+                                                int32_t mode, int32_t *dst_dev, float *scores_dev,
+                                                int32_t *counters_dev, void *stream) {
+    try {
+        if (!s || nq < 0 || k <= 0 || (nq && (!qs_dev || !dst_dev))) return fail(GLASS_HIP_EINVAL, "bad batch arguments");
+        if (mode != MODE_SEARCH && mode != MODE_BATCH) return fail(GLASS_HIP_EINVAL, "mode must be 0 or 1");
+        if (!s->has_data || !s->stream) return fail(GLASS_HIP_ESTATE, "search before set_data");
+        std::lock_guard<std::mutex> lk(s->mu);
+        const int32_t max_cap = std::max(inner_k(*s, k), s->ef);
+        if (int r = check_cap(*s, max_cap)) return r;
+        // the device's scratch array: [work counter | 256 bytes][one slot area per resident wave]
+        void *base = nullptr;
+        size_t bytes = 0;
+        if (int r = noalloc_scratch(s->device, &base, &bytes)) return r;
+        const int64_t slot_bytes = 4 * slot_stride(*s);
+        const int64_t slots = ((int64_t)bytes - 256) / slot_bytes;
+        if (slots < 4)
+            return fail(GLASS_HIP_EINVAL,
+                        "%d rows need %lld bytes of scratch per wave: the %zu-byte no-alloc scratch holds fewer "
+                        "than 4 waves",
+                        s->graph.N, (long long)slot_bytes, bytes);
+        BatchState bs;
+        bs.counter = (int32_t *)base;
+        bs.visited = (uint32_t *)((uint8_t *)base + 256);
+        bs.max_slots = (int32_t)std::min<int64_t>(slots, 1 << 20);
+        return run_batch(*s, qs_dev, nq, nullptr, nullptr, s->ef, k, max_cap, s->ef, mode, nullptr, dst_dev,
+                         scores_dev, counters_dev, (hipStream_t)stream, &bs, false);
     } catch (...) {
         return exception_status();
     }

@@ -85,6 +85,11 @@ struct SlotState {
 hipError_t launch_search(const DeviceIndex &ix, const QueryBatch &qb, const LaunchConfig &lc, const SlotState &ss,
                          hipStream_t stream);
 
+// The device array the no-alloc entry point runs in (no hipMalloc): its
+// address on device `device` (looked up once per device with that device
+// current) and its size.  0 or a GLASS_HIP_* status.
+int noalloc_scratch(int device, void **ptr, size_t *bytes);
+
 // Per-wave dynamic LDS bytes for a configuration.
 size_t lds_bytes_per_wave(const DeviceIndex &ix, int32_t cap_max, int32_t vis_hash);
 

@@ -20,6 +20,9 @@
 //     order) keys, and `cur_` is the first unchecked index.
 // A persistent grid pulls queries from a device work counter, so one launch
 // serves any number of queries with per-query ef / k.
+#include <mutex>
+
+#include "glass_hip.h"
 #include "glass_hip_internal.h"
 
 namespace glass_hip {
@@ -1654,6 +1657,35 @@ hipError_t launch_int(const DeviceIndex &ix, const QueryBatch &qb, const LaunchC
 }
 
 }  // namespace
+
+// Scratch of glass_hip_searcher_search_batch_dev_noalloc: the persistent
+// grid's work counter and one slot area (PQ8 lookup table, overflow visited
+// bitset) per resident wave, for callers that cannot allocate device memory.
+// Each device holds its own copy from module load until the process ends.
+// Host code uses the address noalloc_scratch returns, never this name.
+constexpr size_t kNoAllocScratchBytes = (size_t)128 << 20;
+static __device__ __align__(256) uint8_t scratch[kNoAllocScratchBytes];
+
+int noalloc_scratch(int device, void **ptr, size_t *bytes) {
+    constexpr int kMaxDevices = 64;
+    static std::mutex mu;
+    static void *addr[kMaxDevices] = {};
+    if (device < 0 || device >= kMaxDevices) return set_error(GLASS_HIP_EINVAL, "device index out of range");
+    std::lock_guard<std::mutex> lk(mu);
+    if (!addr[device]) {
+        int cur = 0;
+        if (hipGetDevice(&cur) != hipSuccess) return set_error(GLASS_HIP_EDEVICE, "hipGetDevice failed");
+        if (cur != device && hipSetDevice(device) != hipSuccess) return set_error(GLASS_HIP_EDEVICE, "hipSetDevice failed");
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (cur != device) (void)hipSetDevice(cur);
+        if (e != hipSuccess || !p) return set_error(GLASS_HIP_EDEVICE, "hipGetSymbolAddress(scratch) failed");
+        addr[device] = p;
+    }
+    *ptr = addr[device];
+    *bytes = kNoAllocScratchBytes;
+    return 0;
+}
 
 size_t lds_bytes_per_wave(const DeviceIndex &ix, int32_t cap_max, int32_t vis_hash) {
     return wave_lds_bytes(ix, cap_max, vis_hash);
