@@ -73,8 +73,8 @@ int glass_hip_build_graph(const char *index_type, const float *data, int32_t n, 
 /* create_searcher(graph, metric, quantizer, refine_quant): metric "L2"/"IP";
  * quantizer "SQ8U", "SQ4U", "SQ2U", "E5M2" (alias "FP8_E5M2"), "SQ8", "SQ6",
  * "SQ4", "PQ8" (dim divisible by 4), "FP32", "FP16", "BF16"
- * (graph_searcher.hpp:263-381); refine_quant "", "FP32" or
- * "FP16" (make_refiner, graph_searcher.hpp:240-254).  The graph's contents move into the searcher
+ * (graph_searcher.hpp:263-381); refine_quant "", "FP32", "FP16", "SQ8U" or
+ * "SQ8" (make_refiner, graph_searcher.hpp:240-254).  The graph's contents move into the searcher
  * and `g` is left empty (python/bindings.cc:94-98). */
 int glass_hip_searcher_create(glass_hip_graph *g, const char *metric, const char *quantizer, const char *refine_quant,
                               glass_hip_searcher **out);
@@ -155,6 +155,18 @@ int glass_hip_searcher_set_launch(glass_hip_searcher *s, int32_t team, int32_t w
  * flight (used when one lane group per neighbour fits a wave), 0 = gather
  * only the unvisited neighbours, -1 = tuned by optimize (default). */
 int glass_hip_searcher_set_gather(glass_hip_searcher *s, int32_t mode);
+/* Waves per query: 1 = one wave per query; 2 or 4 = the waves of one
+ * workgroup share a query (the leading wave keeps the pool, the visited set
+ * and the merges; every wave computes a share of each expansion's candidate
+ * distances), which cuts a query's latency where code rows are long or the
+ * batch is too small to fill the device (graphs of at most 64 neighbours per
+ * row).  0 (default) = the choice optimize() tunes per ef bucket on a batch
+ * of a quarter of the resident waves, used when a batch would leave most
+ * wave slots idle. */
+int glass_hip_searcher_set_query_waves(glass_hip_searcher *s, int32_t waves);
+/* Waves per query of the last launch, and optimize()'s choice for small
+ * batches at the current ef. */
+int glass_hip_searcher_get_query_waves(const glass_hip_searcher *s, int32_t *last, int32_t *tuned);
 /* LDS visited-table entries per query (power of two in [256, 16384]; 0 =
  * tuned / sized from ef).  A query that visits more than half as many nodes
  * continues on the global bitset of its slot. */

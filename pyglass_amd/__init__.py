@@ -286,6 +286,17 @@ class Searcher:
         """search_batch_dev replays a captured hipGraph when its arguments repeat."""
         _lib.check(self._L.glass_hip_searcher_set_graph_capture(self._h, int(bool(on))))
 
+    def set_query_waves(self, waves: int = 0):
+        """Waves per query: 1, or 2 / 4 waves of a workgroup sharing each
+        query's distance work; 0 lets optimize() choose for small batches."""
+        _lib.check(self._L.glass_hip_searcher_set_query_waves(self._h, int(waves)))
+
+    def get_query_waves(self):
+        """(waves per query of the last launch, tuned choice for small batches)."""
+        a, b = C.c_int32(), C.c_int32()
+        _lib.check(self._L.glass_hip_searcher_get_query_waves(self._h, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
     def set_gather(self, mode: int = -1):
         """Integer codes: 1 gathers every neighbour's code row before the
         visited test, 0 only the unvisited ones, -1 lets optimize() choose."""

@@ -48,6 +48,8 @@ SIGNATURES = {
     "glass_hip_searcher_set_refine_factor": (C.c_int, [_vp, C.c_float]),
     "glass_hip_searcher_set_launch": (C.c_int, [_vp, _i32, _i32, _i32]),
     "glass_hip_searcher_set_visited_capacity": (C.c_int, [_vp, _i32]),
+    "glass_hip_searcher_set_query_waves": (C.c_int, [_vp, _i32]),
+    "glass_hip_searcher_get_query_waves": (C.c_int, [_vp, _pi32, _pi32]),
     "glass_hip_searcher_set_gather": (C.c_int, [_vp, _i32]),
     "glass_hip_searcher_get_launch": (C.c_int, [_vp, _pi32, _pi32, _pi32, _pi32]),
     "glass_hip_searcher_kernel_info": (C.c_int, [_vp, _pi32, _pi32, _pi32]),

@@ -209,6 +209,10 @@ struct glass_hip_searcher {
     int32_t nb = 0;
     float qmin = 0, qdif = 0, qmax = 0;
     std::vector<float> sq8_mins, sq8_difs;
+    // the re-rank quantizer's row size and calibration (SQ8U / SQ8 re-rank)
+    int32_t r_code_size = 0;
+    float r_qmin = 0, r_qdif = 0, r_qmax = 0;
+    std::vector<float> r_mins, r_difs;
     std::vector<float> pq_codebook;  // PQ8: nsq x 256 x 4
     int32_t ef = 32;
     double last_avg_dist_cmps = 0.0;
@@ -218,6 +222,7 @@ struct glass_hip_searcher {
     struct Tune {
         int32_t team = 8, waves_per_block = 4, vis_hash = 0;  // vis_hash 0: size from ef
         int32_t spec = 1;  // integer codes: gather all neighbours' rows before the visited test
+        int32_t qwaves_small = 1;  // waves per query for batches that leave wave slots idle
     };
     static constexpr int kBuckets = 7;  // ef <= 16, 32, 64, 128, 256, 512, larger
     Tune tune[kBuckets];
@@ -225,6 +230,7 @@ struct glass_hip_searcher {
     int32_t vis_override = 0;     // visited-table entries forced by the caller (0 = tuned/auto)
     int32_t force_team = 0, force_wpb = 0;
     int32_t force_spec = -1;      // -1 tuned, 0 off, 1 on
+    int32_t force_qwaves = 0;     // waves per query forced by the caller (0 = tuned)
     int32_t cu_count = 0;
     int32_t device = 0;           // HIP device the stream and buffers live on
     LaunchConfig last_launch{};
@@ -248,6 +254,7 @@ struct glass_hip_searcher {
     DevBuf<uint8_t> d_codes;
     DevBuf<uint8_t> d_rerank;  // re-rank rows, fp32 or fp16 (refine)
     DevBuf<float> d_mins, d_difs, d_codebook;
+    DevBuf<float> d_rmins, d_rdifs;  // SQ8 re-rank calibration
     int64_t lut_words = 0;  // PQ8: per-slot LUT words ahead of the visited bitset
     // per-slot state
     DevBuf<uint32_t> d_visited;
@@ -313,6 +320,12 @@ struct glass_hip_searcher {
         ix.pq_codebook = d_codebook.p;
         ix.rerank = refine != QK_NONE ? (const void *)d_rerank.p : nullptr;
         ix.r_dim_align = refine != QK_NONE ? r_dim_align : 0;
+        ix.r_code_size = refine != QK_NONE ? r_code_size : 0;
+        ix.r_qmin = r_qmin;
+        ix.r_qdif = r_qdif;
+        ix.r_qmax = r_qmax;
+        ix.r_mins = d_rmins.p;
+        ix.r_difs = d_rdifs.p;
         ix.rerank_kind = refine;
         ix.refine_factor = refine != QK_NONE ? factor() : 1.0f;
         return ix;
@@ -643,6 +656,31 @@ int train_quant(glass_hip_searcher &s, const float *data_dev, int64_t n) {
     return GLASS_HIP_OK;
 }
 
+// The re-rank quantizer's calibration: Refiner::SetData trains its own
+// quantizer on the same rows (refiner.hpp:26-37), so train_quant runs for
+// that quantizer and its results move to the r_* fields.
+int train_refine(glass_hip_searcher &s, const float *data_dev, int64_t n) {
+    const int32_t q = s.quant, da = s.dim_align;
+    const float qmin = s.qmin, qdif = s.qdif, qmax = s.qmax;
+    std::vector<float> mins = std::move(s.sq8_mins), difs = std::move(s.sq8_difs);
+    s.quant = s.refine;
+    s.dim_align = s.r_dim_align;
+    const int r = train_quant(s, data_dev, n);
+    s.r_qmin = s.qmin;
+    s.r_qdif = s.qdif;
+    s.r_qmax = s.qmax;
+    s.r_mins = std::move(s.sq8_mins);
+    s.r_difs = std::move(s.sq8_difs);
+    s.quant = q;
+    s.dim_align = da;
+    s.qmin = qmin;
+    s.qdif = qdif;
+    s.qmax = qmax;
+    s.sq8_mins = std::move(mins);
+    s.sq8_difs = std::move(difs);
+    return r;
+}
+
 int ensure_stream(glass_hip_searcher &s) {
     if (!s.stream) {
         HIP_TRY(hipGetDevice(&s.device));
@@ -666,20 +704,32 @@ int32_t grid_blocks(const glass_hip_searcher &s, const LaunchConfig &lc, int32_t
     if (per_cu <= 0) per_cu = 1;
     int32_t full = per_cu * std::max(1, s.cu_count);
     if (s.blocks > 0) full = std::min(full, s.blocks);
-    int32_t need = (nq + lc.waves_per_block - 1) / lc.waves_per_block;
+    int32_t need = lc.qwaves > 1 ? nq : (nq + lc.waves_per_block - 1) / lc.waves_per_block;
     return std::max(1, std::min(full, need));
 }
 
-// Words of slot scratch per resident wave: the PQ8 lookup table, then an
-// N-bit visited bitset (a multiple of 4 words).
+// Slot areas (visited bitset, PQ8 table) a workgroup uses: one per wave, or
+// one when its waves share a query.
+int32_t slots_per_block(const LaunchConfig &lc) { return lc.qwaves > 1 ? 1 : lc.waves_per_block; }
+
+// Dirty-list words per slot: the bitset words a query may record as set
+// first; a query that sets more clears its whole bitset at its end.
+constexpr int32_t kDirtyWords = 4096;
+
+// Words of slot scratch per resident wave: the PQ8 lookup table, the dirty
+// list, then an N-bit visited bitset (each a multiple of 4 words).
 int64_t slot_stride(const glass_hip_searcher &s) {
-    return s.lut_words + ((int64_t)(s.graph.N + 31) / 32 + 3) / 4 * 4;
+    return s.lut_words + kDirtyWords + ((int64_t)(s.graph.N + 31) / 32 + 3) / 4 * 4;
 }
 
-int ensure_slots(glass_hip_searcher &s, int32_t slots) {
+// Slot areas start zeroed (the visited bitsets must be all zero between
+// queries); a fill kernel on the launch's stream, not hipMemsetAsync, so the
+// launch that follows is ordered after it.
+int ensure_slots(glass_hip_searcher &s, int32_t slots, hipStream_t stream) {
     if (slots <= s.nslots && s.d_visited.p) return GLASS_HIP_OK;
     s.vis_stride = slot_stride(s);
     HIP_TRY(s.d_visited.alloc((size_t)slots * (size_t)s.vis_stride));
+    HIP_TRY(launch_fill_i32((int32_t *)s.d_visited.p, (int64_t)slots * s.vis_stride, 0, stream));
     s.nslots = slots;
     return GLASS_HIP_OK;
 }
@@ -738,21 +788,40 @@ int run_batch(glass_hip_searcher &s, const float *qs_dev, int32_t nq, const int3
     lc.cap_max = round_cap(max_cap);
     lc.vis_hash = s.vis_override ? s.vis_override : (tn.vis_hash ? tn.vis_hash : glass_hip_searcher::auto_vis(max_ef));
     lc.spec = s.force_spec >= 0 ? s.force_spec : tn.spec;
+    lc.qwaves = 1;
+    {
+        // Several waves per query when the batch leaves most wave slots idle
+        // (tuned per ef bucket), or when the caller forces it; rows of at
+        // most 64 neighbours.
+        const int32_t want = s.force_qwaves ? s.force_qwaves : tn.qwaves_small;
+        if (want > 1 && s.graph.K <= 64) {
+            int32_t qw = want;
+            if (!s.force_qwaves) {
+                LaunchConfig one = lc;
+                const int64_t slots_now = (int64_t)std::max(1, max_blocks_per_cu(s.index(), one)) *
+                                          one.waves_per_block * std::max(1, s.cu_count);
+                while (qw > 1 && (int64_t)nq * qw > slots_now) qw /= 2;
+            }
+            if (qw > 1) {
+                lc.qwaves = qw;
+                lc.waves_per_block = qw;
+                lc.spec = 0;
+            }
+        }
+    }
     // Large pools: fewer waves per workgroup, then a smaller visited table
     // (exact at any size: it moves to the global bitset when it fills), until
     // a workgroup's LDS fits the CU.
-    auto lds_of = [&](const LaunchConfig &c) {
-        return lds_bytes_per_wave(s.index(), c.cap_max, c.vis_hash) * (size_t)c.waves_per_block;
-    };
-    while (lds_of(lc) > kLdsPerCu && lc.waves_per_block > 1) lc.waves_per_block -= 1;
+    auto lds_of = [&](const LaunchConfig &c) { return lds_bytes_per_block(s.index(), c); };
+    while (lds_of(lc) > kLdsPerCu && lc.qwaves == 1 && lc.waves_per_block > 1) lc.waves_per_block -= 1;
     while (lds_of(lc) > kLdsPerCu && lc.vis_hash > 256) lc.vis_hash /= 2;
     const size_t lds = lds_of(lc);
     if (lds > kLdsPerCu) return fail(GLASS_HIP_EINVAL, "pool capacity %d needs %zu bytes of LDS", lc.cap_max, lds);
     int32_t blocks = grid_blocks(s, lc, nq);
-    if (bs) blocks = std::max(1, std::min(blocks, bs->max_slots / lc.waves_per_block));
+    if (bs) blocks = std::max(1, std::min(blocks, bs->max_slots / slots_per_block(lc)));
     lc.blocks = blocks;
     if (!bs) {
-        if (int r = ensure_slots(s, blocks * lc.waves_per_block)) return r;
+        if (int r = ensure_slots(s, blocks * slots_per_block(lc), stream)) return r;
         if (!s.d_counter.p) HIP_TRY(s.d_counter.alloc(1));
     }
     int32_t *counter = bs ? bs->counter : s.d_counter.p;
@@ -774,6 +843,7 @@ int run_batch(glass_hip_searcher &s, const float *qs_dev, int32_t nq, const int3
     ss.visited = bs ? bs->visited : s.d_visited.p;
     ss.vis_stride = s.vis_stride;
     ss.lut_words = s.lut_words;
+    ss.dirty_words = kDirtyWords;
     ss.nslots = bs ? bs->max_slots : s.nslots;
     ss.work_counter = counter;
     const DeviceIndex ix = s.index();
@@ -1078,11 +1148,13 @@ int glass_hip_searcher_create(glass_hip_graph *g, const char *metric, const char
         else return fail(GLASS_HIP_EINVAL, "unknown metric '%s'", metric);
         int32_t qk = parse_quant(q);
         if (qk == QK_NONE) return fail(GLASS_HIP_EINVAL, "unsupported quantizer '%s'", quantizer);
-        // make_refiner (graph_searcher.hpp:240-254): FP32 and FP16 re-rank rows
+        // make_refiner (graph_searcher.hpp:240-254): FP32, FP16, SQ8U and SQ8 re-rank rows
         int32_t rk = QK_NONE;
         if (!r.empty()) {
             if (r == "FP32") rk = QK_FP32;
             else if (r == "FP16") rk = QK_FP16;
+            else if (r == "SQ8U") rk = QK_SQ8U;
+            else if (r == "SQ8") rk = QK_SQ8;
             else return fail(GLASS_HIP_EINVAL, "unsupported refine quantizer '%s'", refine_quant);
         }
         if (g->K > 128) return fail(GLASS_HIP_EINVAL, "graph degree %d > 128 is not supported", g->K);
@@ -1119,6 +1191,7 @@ int glass_hip_searcher_set_data(glass_hip_searcher *s, const float *data, int32_
         s->vis_stride = slot_stride(*s);
         int32_t r_row_bytes = 0;
         if (s->refine != QK_NONE) quant_layout(s->refine, dim, &s->r_dim_align, &r_row_bytes);
+        s->r_code_size = r_row_bytes;
         // graph
         const glass_hip_graph &g = s->graph;
         HIP_TRY(s->d_adj.alloc(g.adj.size()));
@@ -1165,6 +1238,18 @@ int glass_hip_searcher_set_data(glass_hip_searcher *s, const float *data, int32_
             HIP_TRY(s->d_rerank.alloc((size_t)n * r_row_bytes));
             HIP_TRY(launch_encode(raw.p, n, dim, r_row_bytes, QK_FP16, s->metric, 0.0f, 0.0f, 0.0f, nullptr, nullptr,
                                   s->d_rerank.p, s->stream));
+        } else if (s->refine == QK_SQ8U || s->refine == QK_SQ8) {
+            // SQ8QuantizerUniform / SQ8Quantizer trained and encoded like search codes
+            if (int r = train_refine(*s, raw.p, n)) return r;
+            if (s->refine == QK_SQ8) {
+                HIP_TRY(s->d_rmins.alloc(s->r_dim_align));
+                HIP_TRY(s->d_rdifs.alloc(s->r_dim_align));
+                HIP_TRY(hipMemcpy(s->d_rmins.p, s->r_mins.data(), 4 * (size_t)s->r_dim_align, hipMemcpyHostToDevice));
+                HIP_TRY(hipMemcpy(s->d_rdifs.p, s->r_difs.data(), 4 * (size_t)s->r_dim_align, hipMemcpyHostToDevice));
+            }
+            HIP_TRY(s->d_rerank.alloc((size_t)n * r_row_bytes));
+            HIP_TRY(launch_encode(raw.p, n, dim, r_row_bytes, s->refine, s->metric, s->r_qmin, s->r_qdif, s->r_qmax,
+                                  s->d_rmins.p, s->d_rdifs.p, s->d_rerank.p, s->stream));
         }
         HIP_TRY(hipStreamSynchronize(s->stream));
         raw.release();
@@ -1274,6 +1359,22 @@ int glass_hip_searcher_search_batch_dev_noalloc(glass_hip_searcher *s, const flo
                         "%d rows need %lld bytes of scratch per wave: the %zu-byte no-alloc scratch holds fewer "
                         "than 4 waves",
                         s->graph.N, (long long)slot_bytes, bytes);
+        // The bitsets must be all zero between queries; every query leaves
+        // its own so, but a different slot layout (another searcher's N or
+        // PQ8 table) reads other words as bitsets: zero the array then.
+        {
+            static std::mutex lay_mu;
+            static int64_t lay_stride[64] = {}, lay_lut[64] = {};
+            const int d = s->device;
+            std::lock_guard<std::mutex> g(lay_mu);
+            if (d < 0 || d >= 64) return fail(GLASS_HIP_EINVAL, "device index out of range");
+            if (lay_stride[d] != slot_stride(*s) || lay_lut[d] != s->lut_words) {
+                hipStream_t st = stream ? (hipStream_t)stream : s->stream;
+                HIP_TRY(launch_fill_i32((int32_t *)base, (int64_t)(bytes / 4), 0, st));
+                lay_stride[d] = slot_stride(*s);
+                lay_lut[d] = s->lut_words;
+            }
+        }
         BatchState bs;
         bs.counter = (int32_t *)base;
         bs.visited = (uint32_t *)((uint8_t *)base + 256);
@@ -1342,15 +1443,17 @@ int glass_hip_searcher_optimize(glass_hip_searcher *s, int32_t num_threads) {
         std::vector<int32_t> ids((size_t)nq * kTryK);
         const int32_t saved_ef = s->ef;
         const int32_t saved_vo = s->vis_override, saved_ft = s->force_team, saved_fw = s->force_wpb,
-                      saved_fs = s->force_spec;
+                      saved_fs = s->force_spec, saved_fq = s->force_qwaves;
+        s->force_qwaves = 1;
         const bool int_codes = s->quant == QK_SQ8U || s->quant == QK_SQ4U || s->quant == QK_SQ2U;
-        auto timeit = [&]() -> double {
+        auto timeit_n = [&](int32_t n) -> double {
             auto t0 = std::chrono::high_resolution_clock::now();
-            int r = host_batch(*s, qs.data(), nq, nullptr, nullptr, kTryK, MODE_BATCH, ids.data(), nullptr, nullptr,
+            int r = host_batch(*s, qs.data(), n, nullptr, nullptr, kTryK, MODE_BATCH, ids.data(), nullptr, nullptr,
                                false);
             auto t1 = std::chrono::high_resolution_clock::now();
             return r ? 1e30 : std::chrono::duration<double>(t1 - t0).count();
         };
+        auto timeit = [&]() -> double { return timeit_n(nq); };
         for (int b = 0; b < glass_hip_searcher::kBuckets - 1; ++b) {
             s->ef = 16 << b;  // upper edge of the bucket
             double best = 1e30;
@@ -1382,6 +1485,37 @@ int glass_hip_searcher_optimize(glass_hip_searcher *s, int32_t num_threads) {
                 }
             }
             s->tune[b] = bt;
+            // Waves per query for batches that leave wave slots idle: a batch
+            // of a quarter of the resident waves at the tuned shape, timed at
+            // 1, 2 and 4 waves per query.
+            s->tune[b].qwaves_small = 1;
+            if (s->graph.K <= 64) {
+                s->force_team = bt.team;
+                s->force_wpb = bt.waves_per_block;
+                s->vis_override = bt.vis_hash;
+                s->force_spec = bt.spec;
+                LaunchConfig lc{};
+                lc.team = bt.team;
+                lc.waves_per_block = bt.waves_per_block;
+                lc.cap_max = round_cap(std::max(inner_k(*s, kTryK), s->ef));
+                lc.vis_hash = bt.vis_hash;
+                lc.spec = bt.spec;
+                lc.qwaves = 1;
+                const int64_t resident = (int64_t)std::max(1, max_blocks_per_cu(s->index(), lc)) *
+                                         bt.waves_per_block * std::max(1, s->cu_count);
+                const int32_t nsmall = (int32_t)std::max<int64_t>(1, std::min<int64_t>(nq, resident / 4));
+                double best_q = 1e30;
+                for (int32_t qw : {1, 2, 4}) {
+                    s->force_qwaves = qw;
+                    timeit_n(nsmall);
+                    const double e = std::min(timeit_n(nsmall), timeit_n(nsmall));
+                    if (e < best_q) {
+                        best_q = e;
+                        s->tune[b].qwaves_small = qw;
+                    }
+                }
+                s->force_qwaves = 1;
+            }
         }
         s->tune[glass_hip_searcher::kBuckets - 1] = s->tune[glass_hip_searcher::kBuckets - 2];
         s->ef = saved_ef;
@@ -1389,6 +1523,7 @@ int glass_hip_searcher_optimize(glass_hip_searcher *s, int32_t num_threads) {
         s->force_team = saved_ft;
         s->force_wpb = saved_fw;
         s->force_spec = saved_fs;
+        s->force_qwaves = saved_fq;
         return GLASS_HIP_OK;
     } catch (...) {
         return exception_status();
@@ -1424,6 +1559,26 @@ int glass_hip_searcher_set_launch(glass_hip_searcher *s, int32_t team, int32_t w
     } catch (...) {
         return exception_status();
     }
+}
+
+int glass_hip_searcher_set_query_waves(glass_hip_searcher *s, int32_t waves) {
+    try {
+        if (!s) return fail(GLASS_HIP_EINVAL, "null searcher");
+        if (waves != 0 && waves != 1 && waves != 2 && waves != 4)
+            return fail(GLASS_HIP_EINVAL, "waves per query must be 0 (tuned), 1, 2 or 4");
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->force_qwaves = waves;
+        return GLASS_HIP_OK;
+    } catch (...) {
+        return exception_status();
+    }
+}
+
+int glass_hip_searcher_get_query_waves(const glass_hip_searcher *s, int32_t *last, int32_t *tuned) {
+    if (!s) return fail(GLASS_HIP_EINVAL, "null searcher");
+    if (last) *last = s->last_launch.qwaves > 1 ? s->last_launch.qwaves : 1;
+    if (tuned) *tuned = s->tune[glass_hip_searcher::bucket(s->ef)].qwaves_small;
+    return GLASS_HIP_OK;
 }
 
 int glass_hip_searcher_set_gather(glass_hip_searcher *s, int32_t mode) {
@@ -1508,7 +1663,7 @@ int glass_hip_searcher_info(const glass_hip_searcher *s, int32_t *code_size, int
         if (!s) return fail(GLASS_HIP_EINVAL, "null searcher");
         if (code_size) *code_size = s->code_size;
         if (rerank_row_bytes)
-            *rerank_row_bytes = s->refine == QK_FP32 ? 4 * s->r_dim_align : (s->refine == QK_FP16 ? 2 * s->r_dim_align : 0);
+            *rerank_row_bytes = s->refine != QK_NONE ? s->r_code_size : 0;
         if (degree) *degree = s->graph.K;
         if (n) *n = s->graph.N;
         if (refine_factor) *refine_factor = s->refine != QK_NONE ? s->factor() : 1.0f;
@@ -1956,9 +2111,12 @@ int glass_hip_server_create(glass_hip_searcher *s, int32_t max_batch, int32_t ma
             }
             if (most == 0) most = 32 * std::max(1, s->cu_count);  // 8 waves per SIMD, 4 SIMDs per CU
             sv->slots_per_batch = most;
-            if (int r = ensure_slots(*s, 1)) return r;  // sets the bitset stride
-            // a query clears its wave's bitset when it first needs it
-            HIP_TRY(sv->visited.alloc((size_t)max_inflight * (size_t)most * (size_t)s->vis_stride));
+            s->vis_stride = slot_stride(*s);
+            // the server's own slot areas, zeroed (bitsets are all zero between queries)
+            const size_t words = (size_t)max_inflight * (size_t)most * (size_t)s->vis_stride;
+            HIP_TRY(sv->visited.alloc(words));
+            HIP_TRY(launch_fill_i32((int32_t *)sv->visited.p, (int64_t)words, 0, s->stream));
+            HIP_TRY(hipStreamSynchronize(s->stream));
         }
         for (int32_t i = 0; i < max_inflight; ++i) {
             sv->slots.push_back(std::make_unique<ServerSlot>());

@@ -37,12 +37,18 @@ struct DeviceIndex {
     const float *sq8_mins;   // dim_align (SQ8 / SQ6 / SQ4 per-dim)
     const float *sq8_difs;
     const float *pq_codebook; // PQ8: nsq x 256 x 4 centroids (nsq = code_size = dim / 4)
-    // re-rank rows (Refiner<FP32Quantizer> or Refiner<FP16Quantizer>), or
-    // null: N x r_dim_align fp32 (rerank_kind QK_FP32) or fp16 (QK_FP16)
+    // re-rank rows of the Refiner's quantizer (make_refiner,
+    // graph_searcher.hpp:240-254), or null: N x r_code_size bytes of fp32
+    // (rerank_kind QK_FP32) or fp16 (QK_FP16) values, SQ8U codes (QK_SQ8U,
+    // integer distances) or per-dim SQ8 codes (QK_SQ8)
     const void *rerank;
     int32_t r_dim_align;
     int32_t rerank_kind;
     float refine_factor;     // get_refine_factor(quantizer)
+    int32_t r_code_size;     // bytes per re-rank row
+    float r_qmin, r_qdif, r_qmax;  // SQ8U re-rank calibration (affine for L2, symmetric for IP)
+    const float *r_mins;     // SQ8 re-rank calibration, r_dim_align each
+    const float *r_difs;
 };
 
 struct QueryBatch {
@@ -68,13 +74,15 @@ struct LaunchConfig {
     int32_t cap_max;         // LDS pool capacity per wave (>= every query's max(k',ef))
     int32_t vis_hash;        // LDS visited-table entries per wave (power of two >= 256)
     int32_t spec;            // integer codes: gather every neighbour's row before the visited test
+    int32_t qwaves;          // waves per query: 1, or 2 / 4 (waves_per_block) waves of a workgroup on one query
 };
 
 struct SlotState {
-    uint32_t *visited;       // nslots x vis_stride words
+    uint32_t *visited;       // nslots x vis_stride words, each slot: [LUT | dirty list | visited bitset]; the
+                             // bitsets are all zero between queries
     int64_t vis_stride;      // words per slot (multiple of 4)
-    int64_t lut_words;       // PQ8: the slot's first lut_words words hold the query's LUT (multiple of 4); the
-                             // visited bitset follows
+    int64_t lut_words;       // PQ8: the slot's first lut_words words hold the query's LUT (multiple of 4)
+    int32_t dirty_words;     // then the bitset words a query set first (multiple of 4), then the bitset
 
     int32_t nslots;
     int32_t *work_counter;   // device counter for the persistent queue
@@ -89,6 +97,9 @@ hipError_t launch_search(const DeviceIndex &ix, const QueryBatch &qb, const Laun
 // address on device `device` (looked up once per device with that device
 // current) and its size.  0 or a GLASS_HIP_* status.
 int noalloc_scratch(int device, void **ptr, size_t *bytes);
+
+// Dynamic LDS bytes of one workgroup of a launch.
+size_t lds_bytes_per_block(const DeviceIndex &ix, const LaunchConfig &lc);
 
 // Per-wave dynamic LDS bytes for a configuration.
 size_t lds_bytes_per_wave(const DeviceIndex &ix, int32_t cap_max, int32_t vis_hash);

@@ -181,7 +181,8 @@ struct WaveLds {
     uint32_t *ci;   // candidate ids [64]
     float *qf;      // query, zero padded [qf_len]
     float *qm;      // SQ8 helpers: 256x-256min, 256min, dif; SQ6: x-min, min, dif; SQ4: 16(x-min), min, dif [3 x qf_len]
-    uint8_t *qcode; // encoded query [code_size rounded to 16]
+                    // (an SQ8 re-rank rewrites them with its own calibration after the search)
+    uint8_t *qcode; // encoded query [qcode_bytes]
 };
 
 __host__ __device__ inline int32_t round16(int64_t x) { return (int32_t)((x + 15) / 16 * 16); }
@@ -192,13 +193,24 @@ __host__ __device__ inline int32_t qf_len(const DeviceIndex &ix) {
     return round16(a);
 }
 
+// The per-dim helper rows (qm): the search quantizer's, or an SQ8 re-rank's.
+__host__ __device__ inline bool needs_qm(const DeviceIndex &ix) {
+    return ix.quant == QK_SQ8 || ix.quant == QK_SQ6 || ix.quant == QK_SQ4 || (ix.rerank && ix.rerank_kind == QK_SQ8);
+}
+
+// Encoded query bytes: the search codes', or an SQ8U re-rank's.
+__host__ __device__ inline int32_t qcode_bytes(const DeviceIndex &ix) {
+    const int32_t r = ix.rerank && ix.rerank_kind == QK_SQ8U ? ix.r_code_size : 0;
+    return round16(ix.code_size > r ? ix.code_size : r);
+}
+
 __host__ __device__ inline size_t wave_lds_bytes(const DeviceIndex &ix, int32_t cap_max, int32_t vis_hash) {
     size_t b = (size_t)vis_hash * 4;
     b += (size_t)round16((int64_t)cap_max * 4) * 2;
     b += 64 * 4 * 2;
     b += (size_t)qf_len(ix) * 4;
-    if (ix.quant == QK_SQ8 || ix.quant == QK_SQ6 || ix.quant == QK_SQ4) b += (size_t)qf_len(ix) * 4 * 3;
-    b += (size_t)round16(ix.code_size);
+    if (needs_qm(ix)) b += (size_t)qf_len(ix) * 4 * 3;
+    b += (size_t)qcode_bytes(ix);
     return b;
 }
 
@@ -219,7 +231,7 @@ __device__ inline WaveLds carve(uint8_t *base, const DeviceIndex &ix, int32_t ca
     w.qf = (float *)(base + off);
     off += (size_t)ql * 4;
     w.qm = (float *)(base + off);
-    if (ix.quant == QK_SQ8 || ix.quant == QK_SQ6 || ix.quant == QK_SQ4) off += (size_t)ql * 4 * 3;
+    if (needs_qm(ix)) off += (size_t)ql * 4 * 3;
     w.qcode = base + off;
     return w;
 }
@@ -244,7 +256,7 @@ __device__ __forceinline__ uint32_t sym_code(float x, float mx) {
 }
 
 template <int V>
-__device__ void encode_query(const DeviceIndex &ix, const WaveLds &w) {
+__device__ __forceinline__ void encode_query(const DeviceIndex &ix, const WaveLds &w) {
     const int l = lane_id();
     for (int b = l; b < ix.code_size; b += 64) {
         uint32_t byte = 0;
@@ -296,7 +308,7 @@ struct IntQuery {
     int32_t nparts;  // active parts per candidate
     int32_t part;    // this lane's part
 
-    __device__ void init(const DeviceIndex &ix, const WaveLds &w) {
+    __device__ __forceinline__ void init(const DeviceIndex &ix, const WaveLds &w) {
         const int code_dw = ix.code_size / 4;
         nparts = (code_dw + DPL - 1) / DPL;
         slot = 1;
@@ -377,12 +389,14 @@ struct IntQuery {
         return (V == V_SQ8U_IP) ? -acc : acc;
     }
 
-    // Distances of candidates ci[0..m) -> keys ck[0..m).
-    __device__ void distances(const DeviceIndex &ix, const WaveLds &w, int m) const {
+    // Distances of candidates ci[0..m) -> keys ck[0..m).  With several waves
+    // on one query, wave `first` of `stride` takes passes first, first +
+    // stride, ... of 64 / slot rows each.
+    __device__ __forceinline__ void distances(const DeviceIndex &ix, const WaveLds &w, int m, int first = 0, int stride = 1) const {
         const int l = lane_id();
         const int cpr = 64 / slot;
         const int cj = l / slot;
-        for (int base = 0; base < m; base += cpr) {
+        for (int base = first * cpr; base < m; base += stride * cpr) {
             const int j = base + cj;
             uint4 c4[DPL / 4];
             load_row(ix, j < m ? w.ci[j] : 0u, j < m, c4);
@@ -429,76 +443,99 @@ __device__ __forceinline__ float e5m2_to_float(uint32_t byte) {
 }
 
 // One row shared by a 4-lane group (p = lane & 3); returns the reduced sum in
-// every lane of the group.
-template <int V>
+// every lane of the group.  The row is read in batches of KB steps of 16
+// elements whose loads are all issued before any arithmetic, so a row of
+// 16 * KB * n elements waits on memory n times (a 768-d SQ8 row at KB = 8:
+// 6 times, against 12 at KB = 4); each accumulator still sums its elements
+// in increasing order.  FP32 rows load a float4 per step and use half of KB.
+template <int V, int KB = 8>
 __device__ __forceinline__ float float_row4(const DeviceIndex &ix, const WaveLds &w, const uint8_t *row, int p,
                                             int32_t mode, int32_t da) {
     Acc4 acc = {{0.0f, 0.0f, 0.0f, 0.0f}};
-    if (V == V_E5M2_L2 || V == V_E5M2_IP) {
-#pragma unroll 4
-        for (int t = 0; t < da; t += 16) {
-            const int j = t + 4 * p;
-            const uint32_t c = *(const uint32_t *)(row + j);
-            const float4 x = *(const float4 *)(w.qf + j);
-            const float xs[4] = {x.x, x.y, x.z, x.w};
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const float y = e5m2_to_float((c >> (8 * r)) & 0xffu);
-                if (V == V_E5M2_L2) {
-                    const float d = xs[r] - y;
-                    acc.a[r] = fmaf(d, d, acc.a[r]);
-                } else {
-                    acc.a[r] = fmaf(xs[r], y, acc.a[r]);
-                }
-            }
-        }
-        return reduce16_4lanes<true>(acc);
-    } else if (V == V_SQ8_L2 || V == V_SQ8_IP) {
+    if (V == V_E5M2_L2 || V == V_E5M2_IP || V == V_SQ8_L2 || V == V_SQ8_IP) {
+        constexpr int kB = KB;  // one code dword per step and lane
         const int ql = qf_len(ix);
         const float *m256 = w.qm + ql, *dif = w.qm + 2 * ql;
-#pragma unroll 4
-        for (int t = 0; t < da; t += 16) {
-            const int j = t + 4 * p;
-            const uint32_t c4 = *(const uint32_t *)(row + j);
+        for (int t0 = 0; t0 < da; t0 += 16 * kB) {
+            uint32_t cw[kB];
 #pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const float c = (float)((c4 >> (8 * r)) & 0xffu) + 0.5f;
-                if (V == V_SQ8_L2) {
-                    float d;
-                    if ((mode & 3) == MODE_BATCH) {
-                        // SearchImpl2 context: yy = fma(c, dif, 256min), d = 256x - yy
-                        d = w.qf[j + r] * 256.0f - fmaf(c, dif[j + r], m256[j + r]);
-                    } else {
-                        // Search / bare computer context: d = fma(-c, dif, 256x - 256min)
-                        d = fmaf(-c, dif[j + r], w.qm[j + r]);
+            for (int b = 0; b < kB; ++b) {
+                const int t = t0 + 16 * b;
+                cw[b] = t < da ? *(const uint32_t *)(row + t + 4 * p) : 0u;
+            }
+#pragma unroll
+            for (int b = 0; b < kB; ++b) {
+                const int t = t0 + 16 * b;
+                if (t >= da) break;
+                const int j = t + 4 * p;
+                if (V == V_E5M2_L2 || V == V_E5M2_IP) {
+                    const float4 x = *(const float4 *)(w.qf + j);
+                    const float xs[4] = {x.x, x.y, x.z, x.w};
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        const float y = e5m2_to_float((cw[b] >> (8 * r)) & 0xffu);
+                        if (V == V_E5M2_L2) {
+                            const float d = xs[r] - y;
+                            acc.a[r] = fmaf(d, d, acc.a[r]);
+                        } else {
+                            acc.a[r] = fmaf(xs[r], y, acc.a[r]);
+                        }
                     }
-                    acc.a[r] = fmaf(d, d, acc.a[r]);
                 } else {
-                    const float yy = fmaf(c, dif[j + r], m256[j + r]);
-                    acc.a[r] = fmaf(w.qf[j + r], yy, acc.a[r]);
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        const float c = (float)((cw[b] >> (8 * r)) & 0xffu) + 0.5f;
+                        if (V == V_SQ8_L2) {
+                            float d;
+                            if ((mode & 3) == MODE_BATCH) {
+                                // SearchImpl2 context: yy = fma(c, dif, 256min), d = 256x - yy
+                                d = w.qf[j + r] * 256.0f - fmaf(c, dif[j + r], m256[j + r]);
+                            } else {
+                                // Search / bare computer context: d = fma(-c, dif, 256x - 256min)
+                                d = fmaf(-c, dif[j + r], w.qm[j + r]);
+                            }
+                            acc.a[r] = fmaf(d, d, acc.a[r]);
+                        } else {
+                            const float yy = fmaf(c, dif[j + r], m256[j + r]);
+                            acc.a[r] = fmaf(w.qf[j + r], yy, acc.a[r]);
+                        }
+                    }
                 }
             }
         }
         return reduce16_4lanes<true>(acc);
     } else {  // FP32, FP16, BF16
-#pragma unroll 4
-        for (int t = 0; t < da; t += 16) {
-            const int j = t + 4 * p;
-            float4 yv;
-            if (V == V_FP32_L2 || V == V_FP32_IP) {
-                yv = *(const float4 *)((const float *)row + j);
-            } else {
-                yv = widen4<V>(*(const uint2 *)((const uint16_t *)row + j));
-            }
-            const float4 x = *(const float4 *)(w.qf + j);
-            const float xs[4] = {x.x, x.y, x.z, x.w}, ys[4] = {yv.x, yv.y, yv.z, yv.w};
+        constexpr bool kWide = V == V_FP32_L2 || V == V_FP32_IP;
+        constexpr int kB = kWide ? KB / 2 : KB;  // 2 KB VGPRs of loads per batch
+        for (int t0 = 0; t0 < da; t0 += 16 * kB) {
+            float4 yv[kB];
 #pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                if (is_l2_float(V)) {
-                    const float d = xs[r] - ys[r];
-                    acc.a[r] = fmaf(d, d, acc.a[r]);
+            for (int b = 0; b < kB; ++b) {
+                const int t = t0 + 16 * b;
+                const int j = t + 4 * p;
+                if (t >= da) {
+                    yv[b] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                } else if (kWide) {
+                    yv[b] = *(const float4 *)((const float *)row + j);
                 } else {
-                    acc.a[r] = fmaf(xs[r], ys[r], acc.a[r]);
+                    yv[b] = widen4<V>(*(const uint2 *)((const uint16_t *)row + j));
+                }
+            }
+#pragma unroll
+            for (int b = 0; b < kB; ++b) {
+                const int t = t0 + 16 * b;
+                if (t >= da) break;
+                const int j = t + 4 * p;
+                const float4 x = *(const float4 *)(w.qf + j);
+                const float xs[4] = {x.x, x.y, x.z, x.w}, ys[4] = {yv[b].x, yv[b].y, yv[b].z, yv[b].w};
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    if (is_l2_float(V)) {
+                        const float d = xs[r] - ys[r];
+                        acc.a[r] = fmaf(d, d, acc.a[r]);
+                    } else {
+                        acc.a[r] = fmaf(xs[r], ys[r], acc.a[r]);
+                    }
                 }
             }
         }
@@ -693,10 +730,88 @@ __device__ void rerank_rows(const DeviceIndex &ix, const WaveLds &w, int m) {
     }
 }
 
+// Re-rank with SQ8U codes (Refiner<SQ8QuantizerUniform>): the query is
+// encoded with the re-rank calibrator (ComputerImpl, computer.hpp:54-63) and
+// each row's integer distance (helpa::l2_s7_s7 / dot_s8_s8, exact in any
+// order) is taken by 4 lanes, 16 rows per pass.  The reference's MaxHeap
+// orders the integer distances and reports them as floats; both are exact
+// below 2^24, so the float keys order them the same way.
+// Compiled into the general path only (the common path's launches never
+// carry an SQ8U / SQ8 re-rank), which keeps the common path's registers and
+// leaves it without calls or a scratch segment.
+template <bool kL2>
+__device__ __forceinline__ void rerank_sq8u(const DeviceIndex &ix, const WaveLds &w, int m) {
+    const int l = lane_id();
+    for (int b = l; b < ix.r_code_size; b += 64) {
+        uint32_t byte = 0;
+        if (b < ix.dim) {
+            byte = (kL2 ? affine_code(w.qf[b], ix.r_qmin, ix.r_qdif, 127.0f) : sym_code(w.qf[b], ix.r_qmax)) & 0xffu;
+        }
+        w.qcode[b] = (uint8_t)byte;
+    }
+    wave_sync();
+    const int p4 = l & 3, g = l >> 2;
+    const int dw = ix.r_code_size / 4;
+    const uint32_t *qd = (const uint32_t *)w.qcode;
+    for (int base = 0; base < m; base += 16) {
+        const int j = base + g;
+        const uint32_t id = w.pid[j < m ? j : m - 1] & kIdMask;
+        const uint32_t *row = (const uint32_t *)((const uint8_t *)ix.rerank + (int64_t)id * ix.r_code_size);
+        uint32_t qc = 0, cc = 0, qq = 0;
+        int32_t ip = 0;
+#pragma unroll 8
+        for (int t = p4; t < dw; t += 4) {
+            const uint32_t c = row[t], qv = qd[t];
+            if (kL2) {
+                qc = udot4(qv, c, qc);
+                cc = udot4(c, c, cc);
+                qq = udot4(qv, qv, qq);
+            } else {
+                ip = sdot4(qv, c, ip);
+            }
+        }
+        int32_t acc = kL2 ? (int32_t)(qq + cc - 2u * qc) : ip;
+        acc += dpp_xor1(acc);
+        acc += dpp_xor2(acc);
+        wave_sync();
+        if (j < m && p4 == 0) w.pk[j] = key_of_float((float)(kL2 ? acc : -acc));
+        wave_sync();
+    }
+}
+
+// Re-rank with per-dim SQ8 codes (Refiner<SQ8Quantizer>): the helper rows
+// are rebuilt from the re-rank calibration and each row goes through the SQ8
+// kernel in the bare computer's rounding (glass::L2SqrSQ8_ext / IPSQ8_ext).
+// General path only, as rerank_sq8u.
+template <int VS, int KB>
+__device__ __forceinline__ void rerank_sq8(const DeviceIndex &ix, const WaveLds &w, int m) {
+    const int l = lane_id();
+    const int ql = qf_len(ix);
+    wave_sync();
+    for (int j = l; j < ql; j += 64) {
+        const float mn = j < ix.r_dim_align ? ix.r_mins[j] : 0.0f;
+        const float df = j < ix.r_dim_align ? ix.r_difs[j] : 0.0f;
+        w.qm[j] = w.qf[j] * 256.0f - mn * 256.0f;
+        w.qm[ql + j] = mn * 256.0f;
+        w.qm[2 * ql + j] = df;
+    }
+    wave_sync();
+    const int p4 = l & 3, g = l >> 2;
+    for (int base = 0; base < m; base += 16) {
+        const int j = base + g;
+        const uint32_t id = w.pid[j < m ? j : m - 1] & kIdMask;
+        const uint8_t *row = (const uint8_t *)ix.rerank + (int64_t)id * ix.r_code_size;
+        const float r = float_row4<VS, KB>(ix, w, row, p4, MODE_SEARCH, ix.r_dim_align);
+        wave_sync();
+        if (j < m && p4 == 0) w.pk[j] = key_of_float(VS == V_SQ8_IP ? -r : r);
+        wave_sync();
+    }
+}
+
 // Exact distances of pool entries [0, m) into pk; rows longer than 128
 // values go through float_row4, one row per 4-lane group per pass.
-template <int VF>
-__device__ void rerank(const DeviceIndex &ix, const WaveLds &w, int m) {
+template <int VF, int KB>
+__device__ __forceinline__ void rerank(const DeviceIndex &ix, const WaveLds &w, int m) {
     const int da = ix.r_dim_align;  // a multiple of 16
     switch (da) {
         case 16: return rerank_rows<VF, 1, 1>(ix, w, m);
@@ -715,7 +830,7 @@ __device__ void rerank(const DeviceIndex &ix, const WaveLds &w, int m) {
         const int j = base + g;
         const uint32_t id = w.pid[j < m ? j : m - 1] & kIdMask;
         const uint8_t *row = (const uint8_t *)ix.rerank + (int64_t)id * da * (is_f16_variant(VF) ? 2 : 4);
-        const float rsum = float_row4<VF>(ix, w, row, p4, MODE_BATCH, da);
+        const float rsum = float_row4<VF, KB>(ix, w, row, p4, MODE_BATCH, da);
         wave_sync();
         if (j < m && p4 == 0) w.pk[j] = key_of_float(is_l2_float(VF) ? rsum : -rsum);
         wave_sync();
@@ -729,7 +844,7 @@ __device__ void rerank(const DeviceIndex &ix, const WaveLds &w, int m) {
 // -((p0 + p2) + (p1 + p3)) with p_i = x_i * c_i.  Written by the wave into
 // its slot's scratch in HBM (nsq x 256 floats).
 template <int V>
-__device__ void build_lut(const DeviceIndex &ix, const WaveLds &w, float *lut) {
+__device__ __forceinline__ void build_lut(const DeviceIndex &ix, const WaveLds &w, float *lut) {
     const int n = ix.code_size * 256;
     for (int e = lane_id(); e < n; e += 64) {
         const float4 c = *(const float4 *)(ix.pq_codebook + (int64_t)e * 4);
@@ -782,12 +897,13 @@ __device__ __forceinline__ float pq_row4(const DeviceIndex &ix, const float *lut
     return r;
 }
 
-template <int V>
-__device__ void float_distances(const DeviceIndex &ix, const WaveLds &w, int m, int32_t mode, const float *lut) {
+template <int V, int KB = 8>
+__device__ __forceinline__ void float_distances(const DeviceIndex &ix, const WaveLds &w, int m, int32_t mode,
+                                                const float *lut, int first = 0, int stride = 1) {
     if (m <= 0) return;
     const int l = lane_id();
     const int p = l & 3, cj = l >> 2;
-    for (int base = 0; base < m; base += 16) {
+    for (int base = 16 * first; base < m; base += 16 * stride) {
         const int j = base + cj;
         const int jj = j < m ? j : (m - 1);
         const uint32_t id = w.ci[jj];
@@ -800,7 +916,7 @@ __device__ void float_distances(const DeviceIndex &ix, const WaveLds &w, int m, 
         } else if (is_pq_variant(V)) {
             r = pq_row4<V>(ix, lut, row, p);  // IP entries are already negated dot products
         } else {
-            r = float_row4<V>(ix, w, row, p, mode, ix.dim_align);
+            r = float_row4<V, KB>(ix, w, row, p, mode, ix.dim_align);
         }
         bool neg = (V == V_E5M2_IP || V == V_SQ8_IP || V == V_FP32_IP || V == V_FP16_IP || V == V_BF16_IP ||
                     V == V_SQ6_IP || V == V_SQ4_IP);
@@ -814,8 +930,13 @@ __device__ void float_distances(const DeviceIndex &ix, const WaveLds &w, int m, 
 // deduplicated adjacency row, so they are distinct: each lane either finds
 // its id or claims an empty slot with a compare-and-swap.  The table is kept
 // below half full; a query that would exceed that switches to the slot's
-// global bitset for the ids it has not seen yet (the bitset is cleared once
-// at the switch), and keeps consulting the table for the ids inserted before.
+// global bitset for the ids it has not seen yet, and keeps consulting the
+// table for the ids inserted before.  The bitset is all zero whenever no
+// query is using it: a query records each word it is the first to set (the
+// lane whose fetch-or returns 0) in the slot's dirty list, and clears those
+// words when it ends (the whole bitset if the list overflowed).  A query
+// thus writes about as many bitset words as it sets, instead of clearing
+// N/8 bytes (1.25 MB at 10M rows) when it switches.
 struct VisitedSet {
     uint32_t *tab;
     uint32_t mask, shift;
@@ -823,8 +944,12 @@ struct VisitedSet {
     bool global;
     uint32_t *gbits;
     int64_t gwords;
+    uint32_t *dirty;     // the slot's dirty list: bitset words this query set first
+    int32_t dirty_cap;   // its capacity in words
+    int32_t ndirty;      // words recorded (past dirty_cap: the whole bitset is cleared)
 
-    __device__ void reset(uint32_t *t, int32_t size, uint32_t *g, int64_t gw) {
+    __device__ __forceinline__ void reset(uint32_t *t, int32_t size, uint32_t *g, int64_t gw, uint32_t *dl,
+                                          int32_t dcap) {
         tab = t;
         mask = (uint32_t)size - 1u;
         shift = 32u - (uint32_t)__builtin_ctz((uint32_t)size);
@@ -833,6 +958,9 @@ struct VisitedSet {
         global = false;
         gbits = g;
         gwords = gw;
+        dirty = dl;
+        dirty_cap = dcap;
+        ndirty = 0;
         uint4 *t4 = (uint4 *)t;
         for (int i = lane_id(); i < size / 4; i += 64) t4[i] = make_uint4(0, 0, 0, 0);
         wave_sync();
@@ -841,8 +969,8 @@ struct VisitedSet {
     __device__ __forceinline__ uint32_t home(uint32_t v) const { return (v * 0x9E3779B1u) >> shift; }
 
     // true when v had not been visited; marks it visited.
-    __device__ bool test_set(uint32_t v, bool active) {
-        bool isnew = false;
+    __device__ __forceinline__ bool test_set(uint32_t v, bool active) {
+        bool isnew = false, fresh = false;
         if (active) {
             const uint32_t key = v + 1u;
             uint32_t h = home(v);
@@ -872,25 +1000,45 @@ struct VisitedSet {
                     const uint32_t old =
                         __hip_atomic_fetch_or(&gbits[v >> 5], bit, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
                     isnew = !(old & bit);
+                    fresh = old == 0u;  // lanes on one word are serialised: one sees 0
                 }
+            }
+        }
+        if (global) {
+            const uint64_t fm = ballot(fresh);
+            if (fm) {
+                const int pos = ndirty + mbcnt(fm);
+                if (fresh && pos < dirty_cap) dirty[pos] = v >> 5;
+                ndirty += __builtin_popcountll(fm);
             }
         }
         return isnew;
     }
 
-    // Account for the ids a row inserted; switch to the global bitset before
-    // the table could pass half full (a row inserts at most 64 ids).
-    __device__ void account(int inserted) {
-        if (global) return;
-        count += inserted;
-        if (count > thr - 64) {
-            global = true;
+    // End of a query: the bitset words it set back to zero.
+    __device__ __forceinline__ void cleanup() {
+        if (!global) return;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        __builtin_amdgcn_s_waitcnt(0);
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+        if (ndirty <= dirty_cap) {
+            for (int i = lane_id(); i < ndirty; i += 64) gbits[dirty[i]] = 0u;
+        } else {
             uint4 *g4 = (uint4 *)gbits;
             for (int64_t i = lane_id(); i < gwords / 4; i += 64) g4[i] = make_uint4(0, 0, 0, 0);
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-            __builtin_amdgcn_s_waitcnt(0);
-            wave_sync();
         }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        __builtin_amdgcn_s_waitcnt(0);
+        global = false;
+    }
+
+    // Account for the ids a row inserted; switch to the global bitset (all
+    // zero, see above) before the table could pass half full (a row inserts
+    // at most 64 ids).
+    __device__ __forceinline__ void account(int inserted) {
+        if (global) return;
+        count += inserted;
+        if (count > thr - 64) global = true;
     }
 };
 
@@ -955,7 +1103,7 @@ __device__ __forceinline__ int upper_bound_lds(const uint32_t *pk, int n, uint32
 // Merge candidates ck/ci[0..m) (in insertion order) into the pool.  Equivalent
 // to m sequential LinearPool::insert calls (neighbor.hpp:382-396).
 // General form for any capacity: binary search and chunked shifts in LDS.
-__device__ void pool_merge_lds(const WaveLds &w, PoolState &ps, int m) {
+__device__ __forceinline__ void pool_merge_lds(const WaveLds &w, PoolState &ps, int m) {
     const int l = lane_id();
     uint32_t kj = l < m ? w.ck[l] : kKeyMax;
     uint32_t ij = l < m ? w.ci[l] : 0u;
@@ -1038,7 +1186,7 @@ constexpr int kMaxChunks = 4;
 // Candidates: lanes with `cand` set (cm = the mask of those lanes), keys kj /
 // ids ij, insertion order = lane order.  nj is the adjacency-row entry this lane loads for the prefetch.
 template <int NCH>
-__device__ uint32_t pool_merge_reg(const DeviceIndex &ix, const WaveLds &w, PoolState &ps, uint32_t kj,
+__device__ __forceinline__ uint32_t pool_merge_reg(const DeviceIndex &ix, const WaveLds &w, PoolState &ps, uint32_t kj,
                                    uint32_t ij, bool cand, uint64_t cm, int nj, int32_t &pred_row) {
     const int l = lane_id();
     uint32_t pkc[NCH], pic[NCH];
@@ -1188,12 +1336,20 @@ __device__ __forceinline__ void put_result(const QueryBatch &qb, int64_t pos, in
 // The search kernel.  P = 1 is the common case compiled on its own: pools of
 // at most 128 entries (two-chunk register merge) and, for integer codes, the
 // all-neighbour gather; the other paths are left out, which keeps the hot
-// loop small and its scalar registers unspilled.  P = 0 handles everything.
+// loop small and its scalar registers unspilled.  P = 0 handles everything
+// with one wave per query; P = 2 is the same with the waves of a workgroup
+// sharing each query (LaunchConfig::qwaves), compiled apart so that the
+// other two carry none of its code.
 
-// At least 4 waves per SIMD (at most 128 VGPRs): left to itself the compiler
-// schedules the re-rank's loads into more registers than that.
+// Register budgets: the common path keeps at least 4 waves per SIMD (at most
+// 128 VGPRs; left to itself the compiler schedules the re-rank's loads into
+// more registers than that).  The general path runs at 3 (168 VGPRs): its
+// pools, long rows and visited bookkeeping spill at 128, and its launches are
+// mostly held to fewer waves by LDS anyway.  The shared-query path runs at 2:
+// a workgroup of 2 or 4 waves serves one query, so its waves are few by
+// design.
 template <int V, int DPL, int P>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void search_kernel(DeviceIndex ix,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(P == 1 ? 4 : (P == 0 ? 3 : 2)))) void search_kernel(DeviceIndex ix,
                                                                                             QueryBatch qb,
                                                                                             LaunchConfig lc,
                                                                                             SlotState ss) {
@@ -1202,23 +1358,40 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void s
     const int l = lane_id();
     const int wv = threadIdx.x >> 6;
     const size_t wbytes = wave_lds_bytes(ix, lc.cap_max, lc.vis_hash);
-    const WaveLds w = carve(lds_raw + (size_t)wv * wbytes, ix, lc.cap_max, lc.vis_hash);
-    const int slot = blockIdx.x * lc.waves_per_block + wv;
-    // the slot's HBM scratch: PQ8 lookup table (lut_words), then the visited bitset
+    // Waves per query: 1 (each wave its own query), or -- the P = 2 kernel --
+    // every wave of the workgroup on one query: wave 0 leads (pool, visited
+    // set, merges, initializer, outputs) and all waves share each expansion's
+    // distance passes.  The query's LDS area is then the workgroup's, and two
+    // control words follow it (the query index, the expansion's candidates).
+    const int W = P == 2 ? at_use(lc.qwaves) : 1;
+    const bool lead = W == 1 || wv == 0;
+    const WaveLds w = carve(lds_raw + (W > 1 ? (size_t)0 : (size_t)wv * wbytes), ix, lc.cap_max, lc.vis_hash);
+    int32_t *const ctl = (int32_t *)(lds_raw + wbytes);
+    const int slot = W > 1 ? (int)blockIdx.x : (int)blockIdx.x * lc.waves_per_block + wv;
+    // the slot's HBM scratch: PQ8 lookup table (lut_words), the visited
+    // bitset's dirty list (dirty_words), then the bitset
     uint32_t *const slot_mem = ss.visited + (int64_t)slot * ss.vis_stride;
     float *const lut = (float *)slot_mem;
-    uint32_t *const vis = slot_mem + ss.lut_words;
+    uint32_t *const dlist = slot_mem + ss.lut_words;
+    uint32_t *const vis = dlist + ss.dirty_words;
     const int ql = qf_len(ix);
 
     // The next query's index, and its row when ql <= 256 (xn, four floats
     // per lane), are requested before the current query's output phase, so
     // that their latency overlaps it.
     int qi_next = 0;
-    if (l == 0) qi_next = atomicAdd(ss.work_counter, 1);
-    qi_next = uni(qi_next);
+    if (W == 1) {
+        if (l == 0) qi_next = atomicAdd(ss.work_counter, 1);
+        qi_next = uni(qi_next);
+    }
     bool have_row = false;
     float xn[4] = {0.0f, 0.0f, 0.0f, 0.0f};
     for (;;) {
+        if (W > 1) {
+            if (wv == 0 && l == 0) ctl[0] = atomicAdd(ss.work_counter, 1);
+            __syncthreads();
+            qi_next = uni(ctl[0]);
+        }
         const int qi = qi_next;
         if (qi >= qb.nq) break;
 
@@ -1247,6 +1420,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void s
 
         // stage the fp32 query (zero padded, as MemCpyTag copies into a zeroed buffer)
         const float *x = qb.queries + (int64_t)qi * qb.dim;
+        if (lead) {
         if (have_row) {
 #pragma unroll
             for (int r = 0; r < 4; ++r)
@@ -1275,30 +1449,33 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void s
         }
         if (is_pq_variant(V)) {
             build_lut<V>(ix, w, lut);
-            __threadfence_block();  // the wave reads the table back from the same CU's L1
+            __threadfence_block();  // the waves read the table back from the same CU's L1
+            wave_sync();
+        }
+        if (is_int_variant(V)) encode_query<V>(ix, w);
+        }  // lead
+        if (W > 1) {
+            __syncthreads();
+        } else {
             wave_sync();
         }
         IntQuery<V, DPL> iq;
-        if (is_int_variant(V)) {
-            encode_query<V>(ix, w);
-            wave_sync();
-            iq.init(ix, w);
-        }
+        if (is_int_variant(V)) iq.init(ix, w);
         // empty visited set (Bitset::reset, neighbor.hpp:67-74)
         VisitedSet vs;
-        vs.reset(w.vt, lc.vis_hash, vis, ss.vis_stride - ss.lut_words);
+        if (lead) vs.reset(w.vt, lc.vis_hash, vis, ss.vis_stride - ss.lut_words - ss.dirty_words, dlist, ss.dirty_words);
 
-        auto dists = [&](int m, int32_t mode) {
+        auto dists = [&](int m, int32_t mode) __attribute__((always_inline)) {
             if (is_int_variant(V)) {
                 iq.distances(ix, w, m);
             } else {
-                float_distances<V>(ix, w, m, mode, lut);
+                float_distances<V, kFast ? 4 : 8>(ix, w, m, mode, lut);
             }
             wave_sync();
         };
 
         // initialize_search (graph.hpp:95-104)
-        if (!fits) {
+        if (!fits || !lead) {
         } else if (ix.has_init) {
             // HNSWInitializer::initialize (HNSWInitializer.hpp:41-62)
             uint32_t u = (uint32_t)ix.ep;
@@ -1391,6 +1568,62 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void s
         // that turn out visited are discarded.  Candidates stay in their
         // lanes, so lane order is adjacency order for the merge.
         const int32_t spec = kFast ? is_int_variant(V) : (is_int_variant(V) && lc.spec && iq.slot * ix.K <= 64);
+        if (W > 1) {
+            // Several waves on this query (rows of at most 64 neighbours).
+            // Per expansion: the leader pops nodes until one has unvisited
+            // neighbours (or the pool is done) and posts their count; every
+            // wave computes its share of their distances; the leader merges.
+            uint32_t pred = kIdMask;
+            int32_t pred_row = -1;
+            for (;;) {
+                if (lead) {
+                    int m = -1;
+                    while (fits && ps.cur < ps.size && ps.cur < ps.ef) {
+                        const uint32_t u = pool_pop(w, ps, pred);
+                        cnt_e += 1;
+                        const int32_t v = (u == pred) ? pred_row : (l < ix.K ? ix.adj[(int64_t)u * ix.K + l] : -1);
+                        const uint64_t inv = ballot(l < ix.K && v == -1);
+                        const int n = inv ? __builtin_ctzll(inv) : ix.K;
+                        const bool isnew = vs.test_set((uint32_t)v, l < n);
+                        const uint64_t nm = ballot(isnew);
+                        if (nm) {
+                            m = __builtin_popcountll(nm);
+                            vs.account(m);
+                            if (isnew) w.ci[mbcnt(nm)] = (uint32_t)v;
+                            break;
+                        }
+                        pred = kIdMask;
+                        pool_settle(w, ps);
+                        if (ps.cur < ps.size && ps.cur < ps.ef) {
+                            pred = uni(w.pid[ps.cur]) & kIdMask;
+                            pred_row = l < ix.K ? ix.adj[(int64_t)pred * ix.K + l] : -1;
+                        }
+                    }
+                    if (l == 0) ctl[1] = m;
+                }
+                __syncthreads();
+                const int m = uni(ctl[1]);
+                if (m < 0) break;
+                if (is_int_variant(V)) {
+                    iq.distances(ix, w, m, wv, W);
+                } else {
+                    float_distances<V>(ix, w, m, qb.mode, lut, wv, W);
+                }
+                __syncthreads();
+                if (lead) {
+                    cnt_d += m;
+                    if (at_use(lc.cap_max) <= 128) {
+                        pred = pool_merge_reg_lds<2>(ix, w, ps, m, pred_row);
+                    } else if (lc.cap_max <= 64 * kMaxChunks) {
+                        pred = pool_merge_reg_lds<kMaxChunks>(ix, w, ps, m, pred_row);
+                    } else {
+                        pool_settle(w, ps);
+                        pool_merge_lds(w, ps, m);
+                        pred = kIdMask;
+                    }
+                }
+            }
+        } else {
         const int nj = spec ? l / iq.slot : l;  // adjacency entry this lane reads
         const uint64_t lead_k = ballot(iq.part == 0 && nj < ix.K);  // first lane of each group with an entry
         uint32_t pred = kIdMask;
@@ -1481,8 +1714,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void s
             }
         }
 
+        }  // one wave per query
+        if (lead) vs.cleanup();
         // take the next query now (see above)
-        {
+        if (W == 1) {
             int t = 0;
             if (l == 0) t = atomicAdd(ss.work_counter, 1);
             qi_next = uni(t);
@@ -1499,7 +1734,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void s
 
         // outputs
         const int64_t ob = qb.out_off ? qb.out_off[qi] : (int64_t)qi * qb.k_uniform;
-        if (!refine) {
+        if (!lead) {
+        } else if (!refine) {
             // LinearPool::to_sorted (neighbor.hpp:421-428)
             for (int i = l; i < k; i += 64) {
                 int32_t id = -1;
@@ -1516,14 +1752,26 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void s
             const int m = kk < ps.size ? kk : ps.size;
             if (ix.rerank_kind == QK_FP16) {
                 if (ix.metric == MK_L2) {
-                    rerank<V_FP16_L2>(ix, w, m);
+                    rerank<V_FP16_L2, kFast ? 4 : 8>(ix, w, m);
                 } else {
-                    rerank<V_FP16_IP>(ix, w, m);
+                    rerank<V_FP16_IP, kFast ? 4 : 8>(ix, w, m);
+                }
+            } else if (!kFast && ix.rerank_kind == QK_SQ8U) {
+                if (ix.metric == MK_L2) {
+                    rerank_sq8u<true>(ix, w, m);
+                } else {
+                    rerank_sq8u<false>(ix, w, m);
+                }
+            } else if (!kFast && ix.rerank_kind == QK_SQ8) {
+                if (ix.metric == MK_L2) {
+                    rerank_sq8<V_SQ8_L2, kFast ? 4 : 8>(ix, w, m);
+                } else {
+                    rerank_sq8<V_SQ8_IP, kFast ? 4 : 8>(ix, w, m);
                 }
             } else if (ix.metric == MK_L2) {
-                rerank<V_FP32_L2>(ix, w, m);
+                rerank<V_FP32_L2, kFast ? 4 : 8>(ix, w, m);
             } else {
-                rerank<V_FP32_IP>(ix, w, m);
+                rerank<V_FP32_IP, kFast ? 4 : 8>(ix, w, m);
             }
             wave_sync();
             // Ranks by (dist, id), with equal pairs (a repeated entry point
@@ -1566,12 +1814,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void s
             }
             for (int i = m + l; i < k; i += 64) put_result(qb, ob + i, -1, __builtin_inff());
         }
-        if (qb.out_counters && l == 0) {
+        if (qb.out_counters && lead && l == 0) {
             qb.out_counters[(int64_t)qi * 3 + 0] = cnt_d;
             qb.out_counters[(int64_t)qi * 3 + 1] = cnt_e;
             qb.out_counters[(int64_t)qi * 3 + 2] = cnt_u;
         }
-        wave_sync();
+        if (W > 1) {
+            __syncthreads();
+        } else {
+            wave_sync();
+        }
     }
 }
 
@@ -1581,6 +1833,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void s
 template <int V, int DPL>
 bool fast_path(const DeviceIndex &ix, const LaunchConfig &lc) {
     if (lc.cap_max > 128) return false;
+    if (ix.rerank && (ix.rerank_kind == QK_SQ8U || ix.rerank_kind == QK_SQ8)) return false;
     if (!is_int_variant(V)) return true;
     if (!lc.spec) return false;
     const int code_dw = ix.code_size / 4;
@@ -1592,22 +1845,37 @@ bool fast_path(const DeviceIndex &ix, const LaunchConfig &lc) {
 
 template <int V, int DPL>
 const void *kernel_fn(const DeviceIndex &ix, const LaunchConfig &lc) {
+    if (lc.qwaves > 1) return (const void *)search_kernel<V, DPL, 2>;
     return fast_path<V, DPL>(ix, lc) ? (const void *)search_kernel<V, DPL, 1> : (const void *)search_kernel<V, DPL, 0>;
+}
+
+// A workgroup whose waves share one query: its query area plus the two
+// control words (query index, candidates of the expansion), rounded to 16.
+constexpr size_t kCtlBytes = 16;
+
+__host__ __device__ inline size_t block_lds_bytes(const DeviceIndex &ix, const LaunchConfig &lc) {
+    const size_t wb = wave_lds_bytes(ix, lc.cap_max, lc.vis_hash);
+    return lc.qwaves > 1 ? wb + kCtlBytes : wb * (size_t)lc.waves_per_block;
 }
 
 template <int V, int DPL>
 hipError_t launch_one(const DeviceIndex &ix, const QueryBatch &qb, const LaunchConfig &lc, const SlotState &ss,
                       hipStream_t stream) {
-    const size_t lds = wave_lds_bytes(ix, lc.cap_max, lc.vis_hash) * (size_t)lc.waves_per_block;
+    const size_t lds = block_lds_bytes(ix, lc);
     static bool attr_set = false;
     if (!attr_set) {
         (void)hipFuncSetAttribute((const void *)search_kernel<V, DPL, 0>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                   160 * 1024);
         (void)hipFuncSetAttribute((const void *)search_kernel<V, DPL, 1>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                   160 * 1024);
+        (void)hipFuncSetAttribute((const void *)search_kernel<V, DPL, 2>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  160 * 1024);
         attr_set = true;
     }
-    if (fast_path<V, DPL>(ix, lc)) {
+    if (lc.qwaves > 1) {
+        hipLaunchKernelGGL((search_kernel<V, DPL, 2>), dim3(lc.blocks), dim3(64 * lc.waves_per_block), lds, stream,
+                           ix, qb, lc, ss);
+    } else if (fast_path<V, DPL>(ix, lc)) {
         hipLaunchKernelGGL((search_kernel<V, DPL, 1>), dim3(lc.blocks), dim3(64 * lc.waves_per_block), lds, stream,
                            ix, qb, lc, ss);
     } else {
@@ -1622,7 +1890,7 @@ hipError_t launch_one(const DeviceIndex &ix, const QueryBatch &qb, const LaunchC
 // 160 KiB of LDS per CU.
 template <int V, int DPL>
 int occupancy_one(const DeviceIndex &ix, const LaunchConfig &lc) {
-    const size_t lds = wave_lds_bytes(ix, lc.cap_max, lc.vis_hash) * (size_t)lc.waves_per_block;
+    const size_t lds = block_lds_bytes(ix, lc);
     hipFuncAttributes attr;
     if (hipFuncGetAttributes(&attr, kernel_fn<V, DPL>(ix, lc)) != hipSuccess) return 0;
     int vgpr = attr.numRegs > 0 ? attr.numRegs : 128;
@@ -1690,6 +1958,8 @@ int noalloc_scratch(int device, void **ptr, size_t *bytes) {
 size_t lds_bytes_per_wave(const DeviceIndex &ix, int32_t cap_max, int32_t vis_hash) {
     return wave_lds_bytes(ix, cap_max, vis_hash);
 }
+
+size_t lds_bytes_per_block(const DeviceIndex &ix, const LaunchConfig &lc) { return block_lds_bytes(ix, lc); }
 
 template <int V, int DPL>
 int vgprs_one(const DeviceIndex &ix, const LaunchConfig &lc) {

@@ -134,6 +134,21 @@ def test_mixture_full_size(tmp_path):
         bi, bd = s.batch_search(Q[i:i + 1], 10)
         np.testing.assert_array_equal(bi[0], ids[i])
         np.testing.assert_array_equal(bd[0], dis[i])
+    # a small LDS table and ef = 2500: queries set more bitset words than the
+    # slot's dirty list holds and clear the whole bitset at their end; the
+    # same queries again on the same slots match the oracle
+    s.set_visited_capacity(256)
+    s.set_launch(0, 0, 2)
+    o.set_ef(2500)
+    s.set_ef(2500)
+    print("c1 1M: ef=2500 with a 256-entry table", flush=True)
+    Io, Do = o.batch_search(Q[:6], 10)
+    for _ in range(2):
+        Ib, Db = s.batch_search(Q[:6], 10)
+        np.testing.assert_array_equal(Ib, Io)
+        np.testing.assert_array_equal(Db, Do)
+    s.set_visited_capacity(0)
+    s.set_launch(0, 0, 0)
     # recall sanity at the operating point (exact kNN of the sample in float64)
     xs = X.astype(np.float64)
     hit = 0

@@ -1,7 +1,8 @@
 """SQ8U and SQ8 as re-rank quantizers (make_refiner,
 glass/searcher/graph_searcher.hpp:240-254; Refiner,
 glass/searcher/refiner.hpp:45-110) against the compiled reference's vectors
-(tests/golden/make_golden_refine.py): the oracle on the CPU."""
+(tests/golden/make_golden_refine.py): the oracle on the CPU, the HIP path on
+the GPU."""
 
 import os
 
@@ -49,3 +50,18 @@ def test_oracle_refine_quantizers(data, graph, quant, refine):
     o = O.Searcher(O.Graph(os.path.join(GOLD, f"{graph}.glass")), X, metric, quant, refine)
     _check(o, f"{graph}/{quant}/{refine}", Q, g)
 
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("graph", GRAPHS)
+@pytest.mark.parametrize("quant,refine", CASES)
+def test_hip_refine_quantizers(data, graph, quant, refine):
+    import pyglass_amd as pg
+
+    if pg.device_count() < 1:
+        pytest.skip("no HIP device")
+    ds, g = data
+    X, Q, metric = ds[graph]
+    print(f"refine {graph} {quant}+{refine}", flush=True)
+    s = pg.Searcher(pg.Graph(os.path.join(GOLD, f"{graph}.glass")), X, metric, quant, refine)
+    assert s.info()["rerank_row_bytes"] == (128 if refine == "SQ8U" else 128)
+    _check(s, f"{graph}/{quant}/{refine}", Q, g)
