@@ -200,15 +200,21 @@ int glass_hip_searcher_sync(glass_hip_searcher *s);
 /* The reference serves one call at a time (python/bindings.cc:100-154: the
  * GIL is released around Search/SearchBatch).  These entry points queue
  * requests from any thread; a dispatcher thread packs whatever is queued (up
- * to max_batch queries) into one launch as soon as one of max_inflight
- * staging slots is free, so a lone request never waits for a batch to fill.
- * Each query gets SearchBatch semantics with its request's ef and k; its
- * result does not depend on what else is in the batch.  The searcher must not
- * be reconfigured (set_ef/optimize/set_launch) while a server on it runs. */
+ * to max_batch queries; a larger request is split over several batches) into
+ * one launch as soon as one of the in-flight batch slots is free, so a lone
+ * request never waits for a batch to fill.  Batches in flight are admitted
+ * by device memory: up to max_inflight (at most 16), as many as half of the
+ * free device memory holds (each needs its waves' visited bitsets and its
+ * staging).  Each query gets SearchBatch semantics with its request's ef and
+ * k; its result does not depend on what else is in the batch.  The searcher
+ * must not be reconfigured (set_ef/optimize/set_launch) while a server on it
+ * runs. */
 typedef struct glass_hip_server glass_hip_server;
 
 int glass_hip_server_create(glass_hip_searcher *s, int32_t max_batch, int32_t max_k, int32_t max_inflight,
                             glass_hip_server **out);
+/* Batches in flight the device memory admitted, max_batch, max_k. */
+int glass_hip_server_info(const glass_hip_server *sv, int32_t *max_inflight, int32_t *max_batch, int32_t *max_k);
 /* Queue nq query rows (copied) with their ef and k; *ticket identifies the request. */
 int glass_hip_server_submit(glass_hip_server *sv, const float *qs, int32_t nq, int32_t ef, int32_t k,
                             int64_t *ticket);

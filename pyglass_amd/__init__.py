@@ -345,8 +345,11 @@ class Searcher:
 class Server:
     """Served form over a Searcher: requests queue from any thread and a
     dispatcher thread batches whatever is queued into one launch as soon as a
-    staging slot is free (glass_hip_server_* in include/glass_hip.h).  Each
-    query is answered with SearchBatch semantics at its request's ef and k."""
+    staging slot is free (glass_hip_server_* in include/glass_hip.h); a
+    request larger than max_batch is split over several batches, and the
+    batches in flight are admitted by free device memory (up to
+    max_inflight).  Each query is answered with SearchBatch semantics at its
+    request's ef and k."""
 
     def __init__(self, searcher: Searcher, max_batch: int = 8192, max_k: int = 100, max_inflight: int = 2):
         self._L = _lib.load()
@@ -415,6 +418,12 @@ class Server:
         _lib.check(self._L.glass_hip_server_replay(self._h, _p(q), nreq, _p(nq), _p(ef), _p(k), _p(arr), _p(lat),
                                                    _p(ids), _p(dis)))
         return lat, ids, dis
+
+    def info(self):
+        """Batches in flight the device memory admitted, max_batch, max_k."""
+        a, b, c = C.c_int32(), C.c_int32(), C.c_int32()
+        _lib.check(self._L.glass_hip_server_info(self._h, C.byref(a), C.byref(b), C.byref(c)))
+        return {"max_inflight": a.value, "max_batch": b.value, "max_k": c.value}
 
     def stats(self):
         b, q, r = C.c_int64(), C.c_int64(), C.c_int64()

@@ -71,6 +71,7 @@ SIGNATURES = {
     "glass_hip_server_wait": (C.c_int, [_vp, C.c_int64, _vp, _vp]),
     "glass_hip_server_cancel": (C.c_int, [_vp, C.c_int64]),
     "glass_hip_server_replay": (C.c_int, [_vp, _vp, C.c_int64, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "glass_hip_server_info": (C.c_int, [_vp, _pi32, _pi32, _pi32]),
     "glass_hip_server_stats": (C.c_int, [_vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
     "glass_hip_server_free": (None, [_vp]),
 }

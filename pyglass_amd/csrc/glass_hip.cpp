@@ -1805,6 +1805,8 @@ struct ServerReq {
     int32_t err = GLASS_HIP_OK;      // RQ_FAILED by its own batch: the status and message
     std::string err_msg;
     int64_t *group_done = nullptr;   // replay(): completed-request count of its call (under the server lock)
+    int32_t dispatched = 0;          // queries handed to batches so far (a request larger than a batch is split)
+    int32_t parts_open = 0;          // dispatched parts not yet complete
 };
 
 struct ServerSlot {
@@ -1818,11 +1820,18 @@ struct ServerSlot {
     uint32_t *visited = nullptr;     // this slot's share of the server's visited bitsets
     hipStream_t stream = nullptr;
     hipEvent_t done = nullptr;
-    std::vector<ServerReq *> reqs;
-    std::vector<int32_t> req_pos;    // first query of each request in the batch
-    std::vector<int64_t> req_res;    // first result of each request
-    std::vector<uint8_t> req_done;
-    int32_t pending = 0;             // requests of this batch not yet completed
+    // A part is queries [q0, q0 + nq) of one request.  r is cleared once the
+    // part completes: its request's owner may release the request from then on.
+    struct Part {
+        ServerReq *r;
+        int32_t q0, nq;
+        int32_t pos;                 // first query of the part in the batch
+        int64_t res;                 // first result of the part in h_out
+        int64_t checked;             // leading results of the part seen complete
+        bool done;
+    };
+    std::vector<Part> parts;
+    int32_t pending = 0;             // parts of this batch not yet complete
     int32_t nq = 0;
 };
 
@@ -1866,26 +1875,49 @@ void server_fail_all(glass_hip_server &sv, int code, const std::string &msg) {
     for (ServerReq *r : sv.pending) r->state = RQ_FAILED;
     sv.pending.clear();
     sv.pending_n.store(0);
-    // completed requests are no longer listed (server_collect clears them):
-    // their owners may already have released them
+    // completed parts no longer name their request (server_collect clears
+    // them): its owner may already have released it
     for (auto &sl : sv.slots)
-        for (ServerReq *r : sl->reqs)
-            if (r && r->state != RQ_DONE) r->state = RQ_FAILED;
+        for (auto &pt : sl->parts)
+            if (pt.r && pt.r->state != RQ_DONE) pt.r->state = RQ_FAILED;
     sv.cv_done.notify_all();
 }
 
+// Publishes a request whose every query has been dispatched and whose every
+// part has completed or failed (under the server lock): from here on its
+// owner may take or release it.
+void server_finish(glass_hip_server &sv, ServerReq *r, int64_t t) {
+    r->done_ns = t;
+    r->state = r->err ? RQ_FAILED : RQ_DONE;
+    if (r->group_done) *r->group_done += 1;
+    if (!r->err) sv.n_requests += 1;
+}
+
 // A batch that could not be launched fails its own requests only; the
-// dispatcher and every other request carry on.
+// dispatcher and every other request carry on.  A split request fails as a
+// whole: its queries not yet dispatched leave the queue, and it is published
+// once its parts in other batches have completed (they still write into it).
 void server_fail_batch(glass_hip_server &sv, ServerSlot &sl, int code, const std::string &msg) {
     (void)hipStreamSynchronize(sl.stream);
     std::lock_guard<std::mutex> lk(sv.mu);
-    for (ServerReq *r : sl.reqs) {
-        r->state = RQ_FAILED;
-        r->err = code;
-        r->err_msg = msg;
-        if (r->group_done) *r->group_done += 1;
+    for (auto &pt : sl.parts) {
+        ServerReq *r = pt.r;
+        if (!r) continue;
+        pt.r = nullptr;
+        r->parts_open -= 1;
+        if (!r->err) {
+            r->err = code;
+            r->err_msg = msg;
+        }
+        if (r->dispatched < r->nq) {
+            auto it = std::find(sv.pending.begin(), sv.pending.end(), r);
+            if (it != sv.pending.end()) sv.pending.erase(it);
+            sv.pending_n.store((int64_t)sv.pending.size());
+            r->dispatched = r->nq;
+        }
+        if (r->parts_open == 0 && r->state == RQ_RUNNING) server_finish(sv, r, now_ns());
     }
-    sl.reqs.clear();
+    sl.parts.clear();
     sv.cv_done.notify_all();
 }
 
@@ -1899,24 +1931,39 @@ int server_launch(glass_hip_server &sv, ServerSlot &sl) {
     float *h_q = (float *)(sl.h_in + sv.in_meta);
     int32_t pos = 0, max_cap = 1, max_ef = 1;
     int64_t res = 0;
-    sl.req_pos.clear();
-    sl.req_res.clear();
-    for (ServerReq *r : sl.reqs) {
-        sl.req_pos.push_back(pos);
-        sl.req_res.push_back(res);
-        std::memcpy(h_q + (size_t)pos * sv.dim, r->q, sizeof(float) * (size_t)r->nq * sv.dim);
-        for (int32_t j = 0; j < r->nq; ++j) {
+    const float *run_src = nullptr;  // rows of consecutive parts that are contiguous are copied at once
+    int32_t run_pos = 0, run_len = 0;
+    auto flush = [&]() {
+        if (run_len) std::memcpy(h_q + (size_t)run_pos * sv.dim, run_src, sizeof(float) * (size_t)run_len * sv.dim);
+        run_len = 0;
+    };
+    for (auto &pt : sl.parts) {
+        ServerReq *r = pt.r;
+        pt.pos = pos;
+        pt.res = res;
+        pt.checked = 0;
+        pt.done = false;
+        const float *src = r->q + (size_t)pt.q0 * sv.dim;
+        if (run_len && src == run_src + (size_t)run_len * sv.dim) {
+            run_len += pt.nq;
+        } else {
+            flush();
+            run_src = src;
+            run_pos = pos;
+            run_len = pt.nq;
+        }
+        for (int32_t j = 0; j < pt.nq; ++j) {
             h_ef[pos + j] = r->ef;
             h_k[pos + j] = r->k;
             h_off[pos + j] = res + (int64_t)j * r->k;
         }
-        pos += r->nq;
-        res += (int64_t)r->nq * r->k;
+        pos += pt.nq;
+        res += (int64_t)pt.nq * r->k;
         max_cap = std::max(max_cap, std::max(inner_k(s, r->k), r->ef));
         max_ef = std::max(max_ef, r->ef);
     }
-    sl.req_done.assign(sl.reqs.size(), 0);
-    sl.pending = (int32_t)sl.reqs.size();
+    flush();
+    sl.pending = (int32_t)sl.parts.size();
     sl.nq = pos;
     // sentinel id 0xFEFEFEFE in every result this batch writes (ids are -1 or
     // in [0, N)); no kernel of this slot is running
@@ -1948,34 +1995,34 @@ constexpr uint32_t kSentinelId = 0xFEFEFEFEu;
 bool server_collect(glass_hip_server &sv, ServerSlot &sl, bool all) {
     int64_t t = 0;
     bool completed = false;
-    for (size_t i = 0; i < sl.reqs.size(); ++i) {
-        if (sl.req_done[i]) continue;
-        ServerReq *r = sl.reqs[i];
-        const size_t n = (size_t)r->nq * r->k;
-        const uint64_t *res = sl.h_out + sl.req_res[i];
-        bool ready = true;
+    for (auto &pt : sl.parts) {
+        if (pt.done) continue;
+        ServerReq *r = pt.r;
+        const int64_t n = (int64_t)pt.nq * r->k;
+        const uint64_t *res = sl.h_out + pt.res;
         if (!all) {
-            for (size_t j = 0; j < n && ready; ++j)
-                ready = (uint32_t)__atomic_load_n(&res[j], __ATOMIC_ACQUIRE) != kSentinelId;
+            // results arrive roughly in order: resume where the last scan stopped
+            while (pt.checked < n && (uint32_t)__atomic_load_n(&res[pt.checked], __ATOMIC_ACQUIRE) != kSentinelId)
+                ++pt.checked;
+            if (pt.checked < n) continue;
         }
-        if (!ready) continue;
         if (!t) t = now_ns();
         std::lock_guard<std::mutex> lk(sv.mu);
         completed = true;
-        for (size_t j = 0; j < n; ++j) {
+        int32_t *ids = r->ids + (int64_t)pt.q0 * r->k;
+        float *dists = r->dists + (int64_t)pt.q0 * r->k;
+        for (int64_t j = 0; j < n; ++j) {
             const uint64_t pr = __atomic_load_n(&res[j], __ATOMIC_ACQUIRE);
-            r->ids[j] = (int32_t)(uint32_t)pr;
+            ids[j] = (int32_t)(uint32_t)pr;
             uint32_t db = (uint32_t)(pr >> 32);
-            std::memcpy(&r->dists[j], &db, 4);
+            std::memcpy(&dists[j], &db, 4);
         }
-        r->done_ns = t;
-        r->state = RQ_DONE;
-        if (r->group_done) *r->group_done += 1;
-        sl.req_done[i] = 1;
-        sl.reqs[i] = nullptr;  // its owner may release it from now on
+        pt.done = true;
+        pt.r = nullptr;  // the request's owner may release it once it completes
         sl.pending -= 1;
-        sv.n_requests += 1;
-        sv.n_queries += r->nq;
+        r->parts_open -= 1;
+        sv.n_queries += pt.nq;
+        if (r->parts_open == 0 && r->dispatched == r->nq && r->state == RQ_RUNNING) server_finish(sv, r, t);
     }
     if (completed) {
         std::lock_guard<std::mutex> lk(sv.mu);
@@ -2014,7 +2061,7 @@ void server_loop(glass_hip_server *psv) {
                     std::lock_guard<std::mutex> lk(sv.mu);
                     sv.n_batches += 1;
                 }
-                sl.reqs.clear();
+                sl.parts.clear();
                 free_slots.push_back(running[i]);
                 running.erase(running.begin() + (ptrdiff_t)i);
                 progressed = true;
@@ -2030,17 +2077,22 @@ void server_loop(glass_hip_server *psv) {
                     if (sv.stop) return;
                     sv.cv_work.wait_for(lk, std::chrono::milliseconds(2));
                 }
+                // fill the batch; a request larger than what is left of it
+                // is split, its remaining queries staying at the queue's head
                 int32_t total = 0;
-                while (!sv.pending.empty() && total + sv.pending.front()->nq <= sv.max_batch) {
+                while (!sv.pending.empty() && total < sv.max_batch) {
                     ServerReq *r = sv.pending.front();
-                    sv.pending.pop_front();
+                    const int32_t take = std::min(r->nq - r->dispatched, sv.max_batch - total);
+                    sl.parts.push_back(ServerSlot::Part{r, r->dispatched, take, 0, 0, 0, false});
+                    r->dispatched += take;
+                    r->parts_open += 1;
                     r->state = RQ_RUNNING;
-                    total += r->nq;
-                    sl.reqs.push_back(r);
+                    total += take;
+                    if (r->dispatched == r->nq) sv.pending.pop_front();
                 }
                 sv.pending_n.store((int64_t)sv.pending.size());
             }
-            if (!sl.reqs.empty()) {
+            if (!sl.parts.empty()) {
                 if (int rc = server_launch(sv, sl)) {
                     server_fail_batch(sv, sl, rc, g_last_error);
                 } else {
@@ -2059,8 +2111,7 @@ void server_loop(glass_hip_server *psv) {
 }
 
 int server_enqueue(glass_hip_server &sv, ServerReq *r) {
-    if (r->nq <= 0 || r->nq > sv.max_batch)
-        return fail(GLASS_HIP_EINVAL, "request of %d queries (max_batch %d)", r->nq, sv.max_batch);
+    if (r->nq <= 0) return fail(GLASS_HIP_EINVAL, "request of %d queries", r->nq);
     if (r->k <= 0 || r->k > sv.max_k || r->ef <= 0)
         return fail(GLASS_HIP_EINVAL, "bad ef %d / k %d (max_k %d)", r->ef, r->k, sv.max_k);
     if (int rc = check_cap(*sv.s, std::max(inner_k(*sv.s, r->k), r->ef))) return rc;
@@ -2074,7 +2125,7 @@ extern "C" {
 int glass_hip_server_create(glass_hip_searcher *s, int32_t max_batch, int32_t max_k, int32_t max_inflight,
                             glass_hip_server **out) {
     try {
-        if (!s || !out || max_batch <= 0 || max_k <= 0 || max_inflight <= 0 || max_inflight > 8)
+        if (!s || !out || max_batch <= 0 || max_k <= 0 || max_inflight <= 0 || max_inflight > 16)
             return fail(GLASS_HIP_EINVAL, "bad server arguments");
         if (!s->has_data) return fail(GLASS_HIP_ESTATE, "server before set_data");
         auto sv = std::make_unique<glass_hip_server>();
@@ -2112,6 +2163,16 @@ int glass_hip_server_create(glass_hip_searcher *s, int32_t max_batch, int32_t ma
             if (most == 0) most = 32 * std::max(1, s->cu_count);  // 8 waves per SIMD, 4 SIMDs per CU
             sv->slots_per_batch = most;
             s->vis_stride = slot_stride(*s);
+            // Batches in flight are admitted by device memory: each needs its
+            // waves' slot areas (visited bitsets, PQ8 tables) and its staging;
+            // at most half of the free memory goes to them.
+            size_t free_b = 0, total_b = 0;
+            HIP_TRY(hipMemGetInfo(&free_b, &total_b));
+            const size_t per_batch = (size_t)most * (size_t)s->vis_stride * 4 + sv->in_bytes;
+            const size_t fit = per_batch ? (free_b / 2) / per_batch : (size_t)max_inflight;
+            if (fit < 1) return fail(GLASS_HIP_ENOMEM, "no device memory for one batch in flight (%zu bytes)", per_batch);
+            max_inflight = (int32_t)std::min<size_t>((size_t)max_inflight, fit);
+            sv->max_inflight = max_inflight;
             // the server's own slot areas, zeroed (bitsets are all zero between queries)
             const size_t words = (size_t)max_inflight * (size_t)most * (size_t)s->vis_stride;
             HIP_TRY(sv->visited.alloc(words));
@@ -2294,6 +2355,14 @@ int glass_hip_server_replay(glass_hip_server *sv, const float *qs, int64_t nreq,
     } catch (...) {
         return exception_status();
     }
+}
+
+int glass_hip_server_info(const glass_hip_server *sv, int32_t *max_inflight, int32_t *max_batch, int32_t *max_k) {
+    if (!sv) return fail(GLASS_HIP_EINVAL, "null argument");
+    if (max_inflight) *max_inflight = sv->max_inflight;
+    if (max_batch) *max_batch = sv->max_batch;
+    if (max_k) *max_k = sv->max_k;
+    return GLASS_HIP_OK;
 }
 
 int glass_hip_server_stats(glass_hip_server *sv, int64_t *batches, int64_t *queries, int64_t *requests) {

@@ -179,3 +179,36 @@ def test_served_768d_ip_requests(tmp_path, quant):
         np.testing.assert_array_equal(mi, oi, err_msg=f"request {j} vs oracle")
         np.testing.assert_array_equal(md, od, err_msg=f"request {j} vs oracle")
     assert np.all(lat > 0)
+
+
+def test_request_larger_than_a_batch_is_split(setup):
+    """A request of more queries than max_batch runs over several batches and
+    completes once all of them have; its results equal the batch path's,
+    alone and among other requests (submit and replay)."""
+    X, Q, path, s = setup
+    srv = pg.Server(s, max_batch=256, max_k=16, max_inflight=3)
+    inf = srv.info()
+    assert 1 <= inf["max_inflight"] <= 3 and inf["max_batch"] == 256
+    print("server split request", flush=True)
+    t_small = srv.submit(Q[5:8], k=10, ef=48)
+    t_big = srv.submit(Q[:1000], k=10, ef=64)
+    t_after = srv.submit(Q[9], k=5, ef=32)
+    ib, db = srv.wait(t_big)
+    bi, bd = s.batch_search_efk(Q[:1000], 64, 10)
+    np.testing.assert_array_equal(ib, np.stack(bi))
+    np.testing.assert_array_equal(db, np.stack(bd))
+    np.testing.assert_array_equal(srv.wait(t_small)[0], np.stack(s.batch_search_efk(Q[5:8], 48, 10)[0]))
+    np.testing.assert_array_equal(srv.wait(t_after)[0], np.stack(s.batch_search_efk(Q[9:10], 32, 5)[0]))
+    nq = np.array([700, 3, 900, 1], np.int32)
+    ef = np.array([64, 32, 128, 16], np.int32)
+    Q2 = np.concatenate([Q, Q])[:1604]
+    lat, ids, dis = srv.replay(Q2, nq, ef, 10, np.zeros(4, np.int64))
+    start = 0
+    for n_, e_ in zip(nq, ef):
+        bi, bd = s.batch_search_efk(Q2[start:start + n_], int(e_), 10)
+        np.testing.assert_array_equal(ids[start * 10:(start + n_) * 10].reshape(-1, 10), np.stack(bi))
+        np.testing.assert_array_equal(dis[start * 10:(start + n_) * 10].reshape(-1, 10), np.stack(bd))
+        start += n_
+    st = srv.stats()
+    assert st["requests"] == 7 and st["queries"] == 1004 + 1604
+    srv.close()
