@@ -332,13 +332,18 @@ struct IntQuery {
     }
 
     // This lane's slice of code row `id` (zeros for slices past the row).
+    // Unconditional loads (no exec-mask branch per load): a lane with no row
+    // (on false) reads row 0, whose distance is never used; slices past the
+    // row read as zero.
     __device__ __forceinline__ void load_row(const DeviceIndex &ix, uint32_t id, bool on, uint4 (&c4)[DPL / 4]) const {
         const int code_q = ix.code_size / 16;  // uint4 per row
-        const uint4 *src = (const uint4 *)(ix.codes + (int64_t)id * ix.code_size);
+        const uint4 *src = (const uint4 *)(ix.codes + (int64_t)(on ? id : 0u) * ix.code_size);
 #pragma unroll
         for (int i = 0; i < DPL / 4; ++i) {
             const int qi = part * (DPL / 4) + i;
-            c4[i] = (on && part < nparts && qi < code_q) ? src[qi] : make_uint4(0, 0, 0, 0);
+            const bool ok = part < nparts && qi < code_q;
+            const uint4 x = src[ok ? qi : 0];
+            c4[i] = ok ? x : make_uint4(0, 0, 0, 0);
         }
     }
 
@@ -378,14 +383,25 @@ struct IntQuery {
             }
             acc = (V == V_SQ8U_IP) ? ip : (int32_t)((uint32_t)qq + cc - 2u * qc);
         }
+        // nested, so that a group of two lanes leaves after two compares
         const int32_t s = at_use(slot);
-        if (s >= 2) acc += dpp_xor1(acc);
-        if (s >= 4) acc += dpp_xor2(acc);
-        if (s >= 8) acc += dpp_down4(acc);
-        if (s >= 16) acc += dpp_down8(acc);
-        // across rows of 16: the first lane of each 16-lane group holds its partial sum
-        if (s >= 32) acc += __shfl_xor(acc, 16);
-        if (s >= 64) acc += __shfl_xor(acc, 32);
+        if (s >= 2) {
+            acc += dpp_xor1(acc);
+            if (s >= 4) {
+                acc += dpp_xor2(acc);
+                if (s >= 8) {
+                    acc += dpp_down4(acc);
+                    if (s >= 16) {
+                        acc += dpp_down8(acc);
+                        // across rows of 16: the first lane of each 16-lane group holds its partial sum
+                        if (s >= 32) {
+                            acc += __shfl_xor(acc, 16);
+                            if (s >= 64) acc += __shfl_xor(acc, 32);
+                        }
+                    }
+                }
+            }
+        }
         return (V == V_SQ8U_IP) ? -acc : acc;
     }
 
@@ -975,15 +991,13 @@ struct VisitedSet {
             const uint32_t key = v + 1u;
             uint32_t h = home(v);
             if (!global) {
-                for (;;) {
-                    const uint32_t old = atomicCAS(&tab[h], 0u, key);
-                    if (old == 0u) {
-                        isnew = true;
-                        break;
-                    }
-                    if (old == key) break;
+                // the first probe decides almost always: it stays out of the loop
+                uint32_t old = atomicCAS(&tab[h], 0u, key);
+                while (old != 0u && old != key) {
                     h = (h + 1u) & mask;
+                    old = atomicCAS(&tab[h], 0u, key);
                 }
+                isnew = old == 0u;
             } else {
                 bool found = false;
                 for (;;) {
@@ -1056,10 +1070,9 @@ struct PoolState {
 // call pool_settle.  Until then ps.cur is a lower bound (every entry before
 // it is checked).
 __device__ __forceinline__ uint32_t pool_pop(const WaveLds &w, PoolState &ps, uint32_t known = kIdMask) {
-    const int l = lane_id();
     const uint32_t uid = known != kIdMask ? known : uni(w.pid[ps.cur]);
     wave_sync();
-    if (l == 0) w.pid[ps.cur] = uid | kChecked;
+    w.pid[ps.cur] = uid | kChecked;  // every lane stores the same value: no exec mask
     wave_sync();
     ps.cur += 1;
     return uid & kIdMask;
@@ -1253,7 +1266,11 @@ __device__ __forceinline__ uint32_t pool_merge_reg(const DeviceIndex &ix, const 
     new_size = uni(new_size);
     new_cur = uni(new_cur);
     if (new_cur < new_size && new_cur < ps.ef) next = uni((new_cur == np0) ? id0 : idc);
-    if (next != kIdMask) pred_row = nj < ix.K ? ix.adj[(int64_t)next * ix.K + nj] : -1;
+    // an unconditional load (row 0 stands in when there is no next node, the
+    // row's last entry for lanes past K; neither is used): a conditional one
+    // would keep the old row live and make the loop copy the new one, which
+    // waits for the load at the end of the expansion
+    pred_row = ix.adj[(int64_t)(next != kIdMask ? next : 0u) * ix.K + (nj < ix.K ? nj : ix.K - 1)];
     int r = 0, ub = 0;
     // le[c]: active candidates whose key is >= pool entry 64c + l; the
     // entry's shift in the merge is the number of the others, ma - le[c]
@@ -1268,7 +1285,7 @@ __device__ __forceinline__ uint32_t pool_merge_reg(const DeviceIndex &ix, const 
         uint64_t mm = am;
         while (mm) {
             const int i = __builtin_ctzll(mm);
-            mm &= mm - 1;
+            mm &= ~(1ull << i);
             const uint32_t ki = (uint32_t)__builtin_amdgcn_readlane((int)kj, i);
             r += (((uint64_t)ki << 32) | (uint32_t)i) < kl;
             // unused chunks hold kKeyMax, which no distance key reaches, so
@@ -1628,10 +1645,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(P == 1 ? 4 
         const uint64_t lead_k = ballot(iq.part == 0 && nj < ix.K);  // first lane of each group with an entry
         uint32_t pred = kIdMask;
         int32_t pred_row = -1;
+        const int njc = nj < ix.K ? nj : ix.K - 1;  // lanes past K read the last entry and drop it
         while (fits && ps.cur < ps.size && ps.cur < ps.ef) {
             const uint32_t u = pool_pop(w, ps, pred);
             cnt_e += 1;
-            const int32_t v = (u == pred) ? pred_row : (nj < ix.K ? ix.adj[(int64_t)u * ix.K + nj] : -1);
+            const int32_t vr = (u == pred) ? pred_row : ix.adj[(int64_t)u * ix.K + njc];
+            const int32_t v = nj < ix.K ? vr : -1;
             if (kFast ? is_int_variant(V) : at_use(spec) != 0) {
                 const bool lead = iq.part == 0;
                 const uint64_t inv = lead_k & ballot(v == -1);
@@ -1644,10 +1663,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(P == 1 ? 4 
                 pred = kIdMask;
                 if (nm == 0) {
                     pool_settle(w, ps);
-                    if (ps.cur < ps.size && ps.cur < ps.ef) {
-                        pred = uni(w.pid[ps.cur]) & kIdMask;
-                        pred_row = nj < ix.K ? ix.adj[(int64_t)pred * ix.K + nj] : -1;
-                    }
+                    if (ps.cur < ps.size && ps.cur < ps.ef) pred = uni(w.pid[ps.cur]) & kIdMask;
+                    pred_row = ix.adj[(int64_t)(pred != kIdMask ? pred : 0u) * ix.K + njc];
                     continue;
                 }
                 const int m = __builtin_popcountll(nm);
@@ -1659,6 +1676,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(P == 1 ? 4 
                 } else if (lc.cap_max <= 64 * kMaxChunks) {
                     pred = pool_merge_reg<kMaxChunks>(ix, w, ps, kj, (uint32_t)v, isnew, nm, nj, pred_row);
                 } else {
+                    pred_row = -1;
                     if (isnew) {
                         w.ck[mbcnt(nm)] = kj;
                         w.ci[mbcnt(nm)] = (uint32_t)v;
@@ -1707,10 +1725,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(P == 1 ? 4 
                 // the pool did not change since the last merge gave no next
                 // node: the next pop takes the first unchecked entry
                 pool_settle(w, ps);
-                if (ps.cur < ps.size && ps.cur < ps.ef) {
-                    pred = uni(w.pid[ps.cur]) & kIdMask;
-                    pred_row = nj < ix.K ? ix.adj[(int64_t)pred * ix.K + nj] : -1;
-                }
+                if (ps.cur < ps.size && ps.cur < ps.ef) pred = uni(w.pid[ps.cur]) & kIdMask;
+                pred_row = ix.adj[(int64_t)(pred != kIdMask ? pred : 0u) * ix.K + njc];
             }
         }
 
