@@ -432,7 +432,8 @@ def quant_mode(spec, gpath, X, Q, gt, k, args, pg, torch, dev):
                         break
                 del r
             cpu = {"qps_at_0.90": qps_at_recall(cpts, 0.90), "qps_at_0.95": qps_at_recall(cpts, 0.95),
-                   "points": cpts, "cores": nthreads, "kind": "reference", "ids_equal_gpu": min(eq) if eq else None}
+                   "points": cpts, "cores": nthreads, "cpu": host_cpu(), "kind": "reference",
+                   "ids_equal_gpu": min(eq) if eq else None}
     out = {
         "quantizer": quant, "refine": refine or None, "refine_factor": factor,
         "workload": f"{X.shape[0] // 1000}k x {X.shape[1]} uint8-valued Gaussian mixture, HNSW R=32 L=200, L2, "
@@ -1235,6 +1236,7 @@ def main():
             "value": cpu_value,
             "unit": "queries/s",
             "cores": cpu[FACTORS[0]]["threads"],
+            "cpu": host_cpu(),
             "kind": "reference",
             "sample": f"{nq} queries per ef, ef swept until recall 0.99 or ~12 s per factor, best of <=5 "
                       f"batch_search runs each, reference Searcher (SQ4U + FP32 Refiner, factors {FACTORS}) after "
