@@ -254,7 +254,7 @@ def _pmc_values(db_path, last=3):
     return kern, copy
 
 
-def pmc_run(counters, point: str, launch: str, timeout_s: int = 240, cfg: str | None = None):
+def pmc_run(counters, point: str, launch: str, timeout_s: int = 240, cfg: str | None = None, qwaves: int = 0):
     """One rocprofv3 --pmc child run of `bench.py --point` with a 512 MiB
     device copy; returns (search-kernel values, copy values) per counter, or
     (None, error)."""
@@ -270,7 +270,7 @@ def pmc_run(counters, point: str, launch: str, timeout_s: int = 240, cfg: str | 
     try:
         cmd = [prof, "--pmc", *counters, "-d", tmp, "-o", "run", "--", sys.executable, os.path.abspath(__file__),
                "--point", point, "--launch", launch, "--steps", "2", "--warmup", "1", "--no-cpu", "--calib-copy",
-               *(["--config", cfg] if cfg else [])]
+               *(["--config", cfg] if cfg else []), *(["--query-waves", str(qwaves)] if qwaves else [])]
         log(f"pmc: {' '.join(counters)} pass")
         try:
             subprocess.run(cmd, cwd="/tmp", env=env, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL,
@@ -288,21 +288,58 @@ def pmc_run(counters, point: str, launch: str, timeout_s: int = 240, cfg: str | 
         shutil.rmtree(tmp, ignore_errors=True)
 
 
+def rocpd_summary(paths, last=3):
+    """Text summary of rocprofv3 databases: per kernel name (trimmed), the
+    dispatch count and mean / min / max duration in microseconds; per
+    counter, the mean per-dispatch value over the last `last` search_kernel
+    dispatches."""
+    import sqlite3
+    from collections import defaultdict
+
+    for path in paths:
+        print(f"# {path}")
+        con = sqlite3.connect(path)
+        try:
+            views = {r[0] for r in con.execute("select name from sqlite_master where type in ('table', 'view')")}
+            if "kernels" in views:
+                rows = list(con.execute("select name, duration, grid_x, workgroup_x, lds_size, vgpr_count, "
+                                        "scratch_size from kernels order by start"))
+                per = defaultdict(list)
+                meta = {}
+                for name, dur, gx, wx, lds, vg, scr in rows:
+                    short = name.replace("(anonymous namespace)::", "").split("(")[0][-90:]
+                    per[short].append(dur / 1e3)
+                    meta[short] = (gx, wx, lds, vg, scr)
+                if rows:
+                    print("| kernel | dispatches | mean us | min us | max us | grid | workgroup | LDS B | VGPRs | scratch |")
+                    print("|---|---|---|---|---|---|---|---|---|---|")
+                for short, d in sorted(per.items(), key=lambda kv: -sum(kv[1]))[:8]:
+                    gx, wx, lds, vg, scr = meta[short]
+                    print(f"| {short} | {len(d)} | {np.mean(d):.1f} | {np.min(d):.1f} | {np.max(d):.1f} | {gx} | {wx} "
+                          f"| {lds} | {vg} | {scr} |")
+            if "counters_collection" in views:
+                kern, _ = _pmc_values(path, last=last)
+                for ctr, val in kern.items():
+                    print(f"{ctr} = {val:.6g}")
+        finally:
+            con.close()
+
+
 def info_launch_waves(s):
     """Waves per CU the persistent grid of the current shape was sized for."""
     ki, lz = s.kernel_info(), s.get_launch()
     return ki["blocks_per_cu"] * lz["waves_per_block"]
 
 
-def pmc_traffic(point: str, launch: str, cfg: str | None = None):
+def pmc_traffic(point: str, launch: str, cfg: str | None = None, qwaves: int = 0):
     """HBM bytes per search launch at an operating point: one rocprofv3 --pmc
     run per counter (FETCH_SIZE, WRITE_SIZE), each counter scaled by what it
     reads for a 512 MiB device copy in the same run (bytes = KiB * 1024 *
     512 MiB / copy bytes counted)."""
-    detail = {"point": point, "launch": launch}
+    detail = {"point": point, "launch": launch, "query_waves": qwaves or 1}
     total = 0.0
     for ctr in ("FETCH_SIZE", "WRITE_SIZE"):
-        kern, copy = pmc_run([ctr], point, launch, timeout_s=480 if cfg else 240, cfg=cfg)
+        kern, copy = pmc_run([ctr], point, launch, timeout_s=480 if cfg else 240, cfg=cfg, qwaves=qwaves)
         if kern is None or ctr not in copy:
             return None, dict(detail, error=f"{ctr}: {copy if kern is None else 'copy dispatch not found'}")
         scale = (512 << 20) / (copy[ctr] * 1024.0)
@@ -653,6 +690,8 @@ def config_mode(cfg, args, pg, torch, dev):
         lt, lw, lv = (int(v) for v in args.launch.split(":"))
         s.set_launch(lt, lw, 0)
         s.set_visited_capacity(lv)
+    if args.query_waves:
+        s.set_query_waves(args.query_waves)
     opt_s = time.time() - t
     info = s.info()
     log(f"{cfg}: set_data {setup_s:.1f}s, optimize {opt_s:.1f}s, info {info}, device {pg.device_info()}")
@@ -698,9 +737,10 @@ def strip(p):
     return {kk: v for kk, v in p.items() if kk not in ("ids", "cnt")}
 
 
-def pmc_for(cfg, point, launch):
-    """FETCH_SIZE / WRITE_SIZE traffic of the operating point (child runs)."""
-    return pmc_traffic(point, launch, cfg)
+def pmc_for(cfg, point, launch, qwaves=0):
+    """FETCH_SIZE / WRITE_SIZE traffic of the operating point (child runs,
+    with the launch shape and waves per query the operating point ran at)."""
+    return pmc_traffic(point, launch, cfg, qwaves)
 
 
 def c5_run(w, s, X, Q, gpath, gmeta, info, args, torch, dev, stream):
@@ -813,7 +853,8 @@ def c3_run(w, s, X, Q, gpath, gmeta, info, args, torch, dev, stream):
     ab = alg_bytes(cnt, info, X.shape[1], kv, 0)
     mix = {"ms_per_step": wall / args.steps * 1e3, "qps": nq / (wall / args.steps), "kernel_ms": kern,
            "dist_cmps": float(cnt[:, 0].mean()), "algorithmic_bytes_per_launch": ab,
-           "achieved_gbs": ab / (kern / 1e3) / 1e9, "launch": s.get_launch(), "kernel": s.kernel_info()}
+           "achieved_gbs": ab / (kern / 1e3) / 1e9, "launch": s.get_launch(), "kernel": s.kernel_info(),
+           "query_waves": s.get_query_waves()[0]}
     log(f"c3 mix: {mix}")
     if args.point:
         if args.calib_copy:
@@ -834,7 +875,8 @@ def c3_run(w, s, X, Q, gpath, gmeta, info, args, torch, dev, stream):
     traffic, tdet = (None, None)
     if not args.no_pmc:
         lz = mix["launch"]
-        traffic, tdet = pmc_for("c3", "mix", f"{lz['team']}:{lz['waves_per_block']}:{lz['vis_entries']}")
+        traffic, tdet = pmc_for("c3", "mix", f"{lz['team']}:{lz['waves_per_block']}:{lz['vis_entries']}",
+                                mix["query_waves"])
     cpu = None
     if not args.no_cpu:
         r = ref_points(gpath, X, Q, gt[:, :10], 10, w, efs, budget_s=30.0)
@@ -908,7 +950,8 @@ def c4_run(w, s, X, Q, gpath, gmeta, info, args, torch, dev, stream):
     ab = alg_bytes(cnt, info, X.shape[1], k, int(k * info["refine_factor"]))
     batched = {"queries": total, "ms_per_step": wall / args.steps * 1e3, "qps": total / (wall / args.steps),
                "kernel_ms": kern, "algorithmic_bytes_per_launch": ab, "achieved_gbs": ab / (kern / 1e3) / 1e9,
-               "dist_cmps": float(cnt[:, 0].mean()), "launch": s.get_launch(), "kernel": s.kernel_info()}
+               "dist_cmps": float(cnt[:, 0].mean()), "launch": s.get_launch(), "kernel": s.kernel_info(),
+               "query_waves": s.get_query_waves()[0]}
     log(f"c4 batched mix: {batched}")
     if args.point:
         if args.calib_copy:
@@ -955,7 +998,8 @@ def c4_run(w, s, X, Q, gpath, gmeta, info, args, torch, dev, stream):
     traffic, tdet = (None, None)
     if not args.no_pmc:
         lz = batched["launch"]
-        traffic, tdet = pmc_for("c4", "mix", f"{lz['team']}:{lz['waves_per_block']}:{lz['vis_entries']}")
+        traffic, tdet = pmc_for("c4", "mix", f"{lz['team']}:{lz['waves_per_block']}:{lz['vis_entries']}",
+                                batched["query_waves"])
     cpu = None
     if not args.no_cpu:
         ns = min(Q.shape[0], 4000)
@@ -1016,11 +1060,18 @@ def main():
     ap.add_argument("--config", type=str, default=None, choices=sorted(WORKLOADS),
                     help="measure another workload at full size instead of the headline (c3, c4, c5)")
     ap.add_argument("--requests", type=int, default=2000, help="--config c4: requests per served run")
+    ap.add_argument("--rocpd-summary", nargs="+", default=None, metavar="DB",
+                    help="print kernel times and counters of rocprofv3 databases and exit")
+    ap.add_argument("--query-waves", type=int, default=0,
+                    help="--config modes: waves per query (1, 2, 4; 0 = tuned), e.g. for profiling an operating point")
     ap.add_argument("--max-batch", type=int, default=16384, help="--config c4: server max_batch")
     ap.add_argument("--max-inflight", type=int, default=3, help="--config c4: server staging slots")
     ap.add_argument("--quant", type=str, default=None,
                     help="Q[:REFINE] - measure quantizer Q (with optional re-rank quantizer) instead of the headline")
     args = ap.parse_args()
+    if args.rocpd_summary:
+        rocpd_summary(args.rocpd_summary)
+        return
     if args.factors:
         FACTORS[:] = [float(f) for f in args.factors.split(",")]
     if args.efs:

@@ -790,17 +790,28 @@ int run_batch(glass_hip_searcher &s, const float *qs_dev, int32_t nq, const int3
     lc.spec = s.force_spec >= 0 ? s.force_spec : tn.spec;
     lc.qwaves = 1;
     {
-        // Several waves per query when the batch leaves most wave slots idle
-        // (tuned per ef bucket), or when the caller forces it; rows of at
-        // most 64 neighbours.
+        // Several waves per query when the batch is small (tuned per ef
+        // bucket), or when the caller forces it; rows of at most 64
+        // neighbours.  A tuned choice is kept while the batch fits in one
+        // round of the queries that kernel holds resident, or in two when
+        // every query has its own ef: such a batch waits for its few
+        // slowest queries, and more waves on each outweigh a partial second
+        // round (1M x 960, ef 16-512 and k 1-100 per query, 1,000 queries:
+        // 4 waves per query in 1.3 rounds, 4.30 ms, against 2 waves in one
+        // round, 5.04 ms; with ef 256 for every query the 1.3 rounds lose).
         const int32_t want = s.force_qwaves ? s.force_qwaves : tn.qwaves_small;
+        const int64_t rounds = ef_dev ? 2 : 1;
         if (want > 1 && s.graph.K <= 64) {
             int32_t qw = want;
             if (!s.force_qwaves) {
-                LaunchConfig one = lc;
-                const int64_t slots_now = (int64_t)std::max(1, max_blocks_per_cu(s.index(), one)) *
-                                          one.waves_per_block * std::max(1, s.cu_count);
-                while (qw > 1 && (int64_t)nq * qw > slots_now) qw /= 2;
+                auto resident_queries = [&](int32_t w) {
+                    LaunchConfig mw = lc;
+                    mw.qwaves = w;
+                    mw.waves_per_block = w;
+                    mw.spec = 0;
+                    return (int64_t)std::max(1, max_blocks_per_cu(s.index(), mw)) * std::max(1, s.cu_count);
+                };
+                while (qw > 1 && (int64_t)nq > rounds * resident_queries(qw)) qw /= 2;
             }
             if (qw > 1) {
                 lc.qwaves = qw;
@@ -1628,8 +1639,7 @@ int glass_hip_searcher_kernel_info(const glass_hip_searcher *s, int32_t *vgprs, 
         if (!s->has_data || !s->last_launch.team) return fail(GLASS_HIP_ESTATE, "no launch yet");
         const LaunchConfig &lc = s->last_launch;
         if (vgprs) *vgprs = kernel_vgprs(s->index(), lc);
-        if (lds_per_block)
-            *lds_per_block = (int32_t)(lds_bytes_per_wave(s->index(), lc.cap_max, lc.vis_hash) * lc.waves_per_block);
+        if (lds_per_block) *lds_per_block = (int32_t)lds_bytes_per_block(s->index(), lc);
         if (blocks_per_cu) *blocks_per_cu = max_blocks_per_cu(s->index(), lc);
         return GLASS_HIP_OK;
     } catch (...) {

@@ -1362,11 +1362,15 @@ __device__ __forceinline__ void put_result(const QueryBatch &qb, int64_t pos, in
 // 128 VGPRs; left to itself the compiler schedules the re-rank's loads into
 // more registers than that).  The general path runs at 3 (168 VGPRs): its
 // pools, long rows and visited bookkeeping spill at 128, and its launches are
-// mostly held to fewer waves by LDS anyway.  The shared-query path runs at 2:
-// a workgroup of 2 or 4 waves serves one query, so its waves are few by
-// design.
+// mostly held to fewer waves by LDS anyway.  The shared-query path runs at 3
+// too: at 2, a batch of 1,000 queries at 4 waves each does not fit the device
+// in one round (it spills at 128).  Its SQ2U instance with 16 dwords per lane
+// spills at 168 and runs at 2.
 template <int V, int DPL, int P>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(P == 1 ? 4 : (P == 0 ? 3 : 2)))) void search_kernel(DeviceIndex ix,
+constexpr int kWavesPerEu = P == 1 ? 4 : ((P == 2 && V == V_SQ2U && DPL == 16) ? 2 : 3);
+
+template <int V, int DPL, int P>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kWavesPerEu<V, DPL, P>))) void search_kernel(DeviceIndex ix,
                                                                                             QueryBatch qb,
                                                                                             LaunchConfig lc,
                                                                                             SlotState ss) {
