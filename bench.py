@@ -182,7 +182,7 @@ def gpu_sweep(s, q_dev, nq, k, gt, ids_dev, d_dev, cnt_dev, args, torch, stream,
     return pts
 
 
-def served_bench(s, Q, gt, k, ef, factor, n_sat=200_000, n_load=400_000, loads=(0.25, 0.5, 0.75, 0.9),
+def served_bench(s, Q, gt, k, ef, factor, n_sat=500_000, n_load=400_000, loads=(0.25, 0.5, 0.75, 0.9),
                  max_batch=8192, max_inflight=3):
     """Served form: single-query requests through the continuous-batching
     server.  Saturation = completion rate with every request queued at t=0;

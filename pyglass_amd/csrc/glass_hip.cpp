@@ -1841,6 +1841,7 @@ struct ServerSlot {
         bool done;
     };
     std::vector<Part> parts;
+    size_t first_open = 0;           // parts before it are complete
     int32_t pending = 0;             // parts of this batch not yet complete
     int32_t nq = 0;
 };
@@ -1974,6 +1975,7 @@ int server_launch(glass_hip_server &sv, ServerSlot &sl) {
     }
     flush();
     sl.pending = (int32_t)sl.parts.size();
+    sl.first_open = 0;
     sl.nq = pos;
     // sentinel id 0xFEFEFEFE in every result this batch writes (ids are -1 or
     // in [0, N)); no kernel of this slot is running
@@ -2003,10 +2005,25 @@ constexpr uint32_t kSentinelId = 0xFEFEFEFEu;
 // (no pair still holds the sentinel id; every request when `all` is set: the
 // batch has finished).  Returns true when every request of the slot is complete.
 bool server_collect(glass_hip_server &sv, ServerSlot &sl, bool all) {
+    // Copy the results of every part whose pairs have all arrived into its
+    // request (no lock: a request is not published until server_finish),
+    // then publish the parts under one lock.
+    // Queries start in batch order, so completions cluster after the first
+    // open part: a poll looks at no more than kScanWindow open parts past
+    // it (the batch's final collect, `all`, takes every part).
+    constexpr int32_t kScanWindow = 1024;
     int64_t t = 0;
-    bool completed = false;
-    for (auto &pt : sl.parts) {
+    size_t first_done = sl.parts.size();
+    while (sl.first_open < sl.parts.size() && sl.parts[sl.first_open].done) ++sl.first_open;
+    int32_t open_seen = 0;
+    size_t scan_end = sl.parts.size();
+    for (size_t i = sl.first_open; i < sl.parts.size(); ++i) {
+        auto &pt = sl.parts[i];
         if (pt.done) continue;
+        if (!all && ++open_seen > kScanWindow) {
+            scan_end = i;
+            break;
+        }
         ServerReq *r = pt.r;
         const int64_t n = (int64_t)pt.nq * r->k;
         const uint64_t *res = sl.h_out + pt.res;
@@ -2017,8 +2034,6 @@ bool server_collect(glass_hip_server &sv, ServerSlot &sl, bool all) {
             if (pt.checked < n) continue;
         }
         if (!t) t = now_ns();
-        std::lock_guard<std::mutex> lk(sv.mu);
-        completed = true;
         int32_t *ids = r->ids + (int64_t)pt.q0 * r->k;
         float *dists = r->dists + (int64_t)pt.q0 * r->k;
         for (int64_t j = 0; j < n; ++j) {
@@ -2028,14 +2043,23 @@ bool server_collect(glass_hip_server &sv, ServerSlot &sl, bool all) {
             std::memcpy(&dists[j], &db, 4);
         }
         pt.done = true;
-        pt.r = nullptr;  // the request's owner may release it once it completes
-        sl.pending -= 1;
-        r->parts_open -= 1;
-        sv.n_queries += pt.nq;
-        if (r->parts_open == 0 && r->dispatched == r->nq && r->state == RQ_RUNNING) server_finish(sv, r, t);
+        pt.checked = -1;  // done and not yet published
+        if (first_done == sl.parts.size()) first_done = i;
     }
-    if (completed) {
+    if (first_done == sl.parts.size()) return sl.pending == 0;
+    {
         std::lock_guard<std::mutex> lk(sv.mu);
+        for (size_t i = first_done; i < scan_end; ++i) {
+            auto &pt = sl.parts[i];
+            if (pt.checked != -1) continue;
+            ServerReq *r = pt.r;
+            pt.checked = 0;
+            pt.r = nullptr;  // the request's owner may release it once it completes
+            sl.pending -= 1;
+            r->parts_open -= 1;
+            sv.n_queries += pt.nq;
+            if (r->parts_open == 0 && r->dispatched == r->nq && r->state == RQ_RUNNING) server_finish(sv, r, t);
+        }
         sv.cv_done.notify_all();
     }
     return sl.pending == 0;
@@ -2337,7 +2361,9 @@ int glass_hip_server_replay(glass_hip_server *sv, const float *qs, int64_t nreq,
             {
                 std::lock_guard<std::mutex> lk(sv->mu);
                 if (sv->err_code) break;
-                while (next < nreq && arrival_ns[next] <= t) {
+                // at most 1,024 at a time, so that the dispatcher is not
+                // kept from the queue while a burst is pushed
+                for (int c = 0; c < 1024 && next < nreq && arrival_ns[next] <= t; ++c) {
                     ServerReq &r = reqs[(size_t)next];
                     r.arrival_ns = t0 + arrival_ns[next];
                     sv->pending.push_back(&r);
