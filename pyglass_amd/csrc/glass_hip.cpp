@@ -202,6 +202,28 @@ struct GraphKey {
     hipStream_t stream;
 };
 
+// Launch state of one stream: its work counter, slot areas and captured
+// graph.  Launches on one stream share it (the stream orders them); launches
+// on different streams use different states, so they run concurrently (one
+// batch's last round overlaps the next batch's first).
+struct StreamState {
+    hipStream_t stream = nullptr;
+    DevBuf<uint32_t> visited;
+    int32_t nslots = 0;
+    DevBuf<int32_t> counter;
+    hipGraphExec_t gexec = nullptr;
+    GraphKey gkey{};
+    hipEvent_t done = nullptr;  // recorded after each launch that uses this state
+    uint64_t last_use = 0;
+    StreamState() = default;
+    StreamState(const StreamState &) = delete;
+    StreamState &operator=(const StreamState &) = delete;
+    ~StreamState() {
+        if (gexec) (void)hipGraphExecDestroy(gexec);
+        if (done) (void)hipEventDestroy(done);
+    }
+};
+
 struct glass_hip_searcher {
     glass_hip_graph graph;
     int32_t metric = MK_L2, quant = QK_SQ8U, refine = QK_NONE;
@@ -256,11 +278,13 @@ struct glass_hip_searcher {
     DevBuf<float> d_mins, d_difs, d_codebook;
     DevBuf<float> d_rmins, d_rdifs;  // SQ8 re-rank calibration
     int64_t lut_words = 0;  // PQ8: per-slot LUT words ahead of the visited bitset
-    // per-slot state
-    DevBuf<uint32_t> d_visited;
+    // per-slot scratch of every stream state: words per slot
     int64_t vis_stride = 0;
-    int32_t nslots = 0;
-    DevBuf<int32_t> d_counter;
+    // launch states of the streams that launched last (least recently used
+    // first to be reused)
+    static constexpr int kStreamStates = 4;
+    std::vector<std::unique_ptr<StreamState>> sstates;
+    uint64_t use_clock = 0;
     // staging for host-pointer calls
     DevBuf<float> d_q;
     DevBuf<int32_t> d_ids, d_counters, d_ef, d_k;
@@ -275,17 +299,9 @@ struct glass_hip_searcher {
 
     // device-pointer batches replay a captured graph when their arguments repeat
     bool graph_capture = true;
-    hipGraphExec_t gexec = nullptr;
-    GraphKey gkey{};
-
-    // Launches without their own batch state share d_counter and d_visited:
-    // a launch on another stream than the previous one waits for it.
-    hipEvent_t shared_done = nullptr;
-    hipStream_t shared_stream = nullptr;
 
     ~glass_hip_searcher() {
-        if (gexec) (void)hipGraphExecDestroy(gexec);
-        if (shared_done) (void)hipEventDestroy(shared_done);
+        sstates.clear();
         if (stream) (void)hipStreamDestroy(stream);
     }
 
@@ -685,7 +701,6 @@ int ensure_stream(glass_hip_searcher &s) {
     if (!s.stream) {
         HIP_TRY(hipGetDevice(&s.device));
         HIP_TRY(hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking));
-        HIP_TRY(hipEventCreateWithFlags(&s.shared_done, hipEventDisableTiming));
     }
     if (!s.cu_count) {
         int dev = 0;
@@ -725,12 +740,37 @@ int64_t slot_stride(const glass_hip_searcher &s) {
 // Slot areas start zeroed (the visited bitsets must be all zero between
 // queries); a fill kernel on the launch's stream, not hipMemsetAsync, so the
 // launch that follows is ordered after it.
-int ensure_slots(glass_hip_searcher &s, int32_t slots, hipStream_t stream) {
-    if (slots <= s.nslots && s.d_visited.p) return GLASS_HIP_OK;
+int ensure_slots(glass_hip_searcher &s, StreamState &st, int32_t slots, hipStream_t stream) {
+    if (slots <= st.nslots && st.visited.p) return GLASS_HIP_OK;
     s.vis_stride = slot_stride(s);
-    HIP_TRY(s.d_visited.alloc((size_t)slots * (size_t)s.vis_stride));
-    HIP_TRY(launch_fill_i32((int32_t *)s.d_visited.p, (int64_t)slots * s.vis_stride, 0, stream));
-    s.nslots = slots;
+    HIP_TRY(st.visited.alloc((size_t)slots * (size_t)s.vis_stride));
+    HIP_TRY(launch_fill_i32((int32_t *)st.visited.p, (int64_t)slots * s.vis_stride, 0, stream));
+    st.nslots = slots;
+    return GLASS_HIP_OK;
+}
+
+// The launch state of `stream`: its own if it has one, else a new one, else
+// the least recently used one, which `stream` first waits for (that state's
+// last launch may still be running on its old stream).
+int state_for(glass_hip_searcher &s, hipStream_t stream, StreamState **out) {
+    StreamState *st = nullptr;
+    for (auto &p : s.sstates)
+        if (p->stream == stream) st = p.get();
+    if (!st && (int)s.sstates.size() < glass_hip_searcher::kStreamStates) {
+        auto n = std::make_unique<StreamState>();
+        HIP_TRY(hipEventCreateWithFlags(&n->done, hipEventDisableTiming));
+        n->stream = stream;
+        st = n.get();
+        s.sstates.push_back(std::move(n));
+    }
+    if (!st) {
+        for (auto &p : s.sstates)
+            if (!st || p->last_use < st->last_use) st = p.get();
+        HIP_TRY(hipStreamWaitEvent(stream, st->done, 0));
+        st->stream = stream;
+    }
+    st->last_use = ++s.use_clock;
+    *out = st;
     return GLASS_HIP_OK;
 }
 
@@ -831,11 +871,13 @@ int run_batch(glass_hip_searcher &s, const float *qs_dev, int32_t nq, const int3
     int32_t blocks = grid_blocks(s, lc, nq);
     if (bs) blocks = std::max(1, std::min(blocks, bs->max_slots / slots_per_block(lc)));
     lc.blocks = blocks;
+    StreamState *st = nullptr;
     if (!bs) {
-        if (int r = ensure_slots(s, blocks * slots_per_block(lc), stream)) return r;
-        if (!s.d_counter.p) HIP_TRY(s.d_counter.alloc(1));
+        if (int r = state_for(s, stream, &st)) return r;
+        if (int r = ensure_slots(s, *st, blocks * slots_per_block(lc), stream)) return r;
+        if (!st->counter.p) HIP_TRY(st->counter.alloc(1));
     }
-    int32_t *counter = bs ? bs->counter : s.d_counter.p;
+    int32_t *counter = bs ? bs->counter : st->counter.p;
     QueryBatch qb{};
     qb.queries = qs_dev;
     qb.nq = nq;
@@ -851,11 +893,11 @@ int run_batch(glass_hip_searcher &s, const float *qs_dev, int32_t nq, const int3
     qb.out_counters = counters_dev;
     qb.out_pairs = bs ? bs->out_pairs : nullptr;
     SlotState ss{};
-    ss.visited = bs ? bs->visited : s.d_visited.p;
+    ss.visited = bs ? bs->visited : st->visited.p;
     ss.vis_stride = s.vis_stride;
     ss.lut_words = s.lut_words;
     ss.dirty_words = kDirtyWords;
-    ss.nslots = bs ? bs->max_slots : s.nslots;
+    ss.nslots = bs ? bs->max_slots : st->nslots;
     ss.work_counter = counter;
     const DeviceIndex ix = s.index();
     // counter reset (a fill kernel, not hipMemsetAsync, so it can sit in a graph) + search
@@ -864,16 +906,11 @@ int run_batch(glass_hip_searcher &s, const float *qs_dev, int32_t nq, const int3
         return e == hipSuccess ? launch_search(ix, qb, lc, ss, stream) : e;
     };
     s.last_launch = lc;
-    if (!bs) {
-        // the shared counter and bitsets: order after the previous launch
-        if (s.shared_stream && s.shared_stream != stream) HIP_TRY(hipStreamWaitEvent(stream, s.shared_done, 0));
-        s.shared_stream = stream;
-    }
     auto mark = [&]() -> int {
-        if (!bs) HIP_TRY(hipEventRecord(s.shared_done, stream));
+        if (st) HIP_TRY(hipEventRecord(st->done, stream));
         return GLASS_HIP_OK;
     };
-    if (!graph) {
+    if (!graph || !st) {
         HIP_TRY(enqueue());
         return mark();
     }
@@ -885,17 +922,17 @@ int run_batch(glass_hip_searcher &s, const float *qs_dev, int32_t nq, const int3
     key.lc = lc;
     key.ss = ss;
     key.stream = stream;
-    if (s.gexec && std::memcmp(&key, &s.gkey, sizeof(key)) == 0) {
-        HIP_TRY(hipGraphLaunch(s.gexec, stream));
+    if (st->gexec && std::memcmp(&key, &st->gkey, sizeof(key)) == 0) {
+        HIP_TRY(hipGraphLaunch(st->gexec, stream));
         return mark();
     }
     // first call with these arguments: run eagerly (this also does any
     // one-time set-up), then capture the same two launches for next time
     HIP_TRY(enqueue());
     if (int r = mark()) return r;
-    if (s.gexec) {
-        (void)hipGraphExecDestroy(s.gexec);
-        s.gexec = nullptr;
+    if (st->gexec) {
+        (void)hipGraphExecDestroy(st->gexec);
+        st->gexec = nullptr;
     }
     hipGraph_t gr = nullptr;
     HIP_TRY(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
@@ -905,13 +942,13 @@ int run_batch(glass_hip_searcher &s, const float *qs_dev, int32_t nq, const int3
         if (gr) (void)hipGraphDestroy(gr);
         return fail(GLASS_HIP_EDEVICE, "graph capture failed: %s", hipGetErrorString(ce != hipSuccess ? ce : ee));
     }
-    const hipError_t ie = hipGraphInstantiate(&s.gexec, gr, nullptr, nullptr, 0);
+    const hipError_t ie = hipGraphInstantiate(&st->gexec, gr, nullptr, nullptr, 0);
     (void)hipGraphDestroy(gr);
     if (ie != hipSuccess) {
-        s.gexec = nullptr;
+        st->gexec = nullptr;
         return fail(GLASS_HIP_EDEVICE, "graph instantiate failed: %s", hipGetErrorString(ie));
     }
-    s.gkey = key;
+    st->gkey = key;
     return GLASS_HIP_OK;
 }
 
@@ -1198,7 +1235,7 @@ int glass_hip_searcher_set_data(glass_hip_searcher *s, const float *data, int32_
         s->dim = dim;
         quant_layout(s->quant, dim, &s->dim_align, &s->code_size);
         s->lut_words = s->quant == QK_PQ8 ? (int64_t)s->code_size * 256 : 0;
-        s->nslots = 0;  // the slot stride depends on lut_words and N: buffers are re-sized at the next launch
+        for (auto &st : s->sstates) st->nslots = 0;  // the slot stride depends on lut_words and N: re-sized at the next launch
         s->vis_stride = slot_stride(*s);
         int32_t r_row_bytes = 0;
         if (s->refine != QK_NONE) quant_layout(s->refine, dim, &s->r_dim_align, &r_row_bytes);
