@@ -1881,6 +1881,12 @@ struct ServerSlot {
     size_t first_open = 0;           // parts before it are complete
     int32_t pending = 0;             // parts of this batch not yet complete
     int32_t nq = 0;
+    // The slot's worker packs, launches and collects its batches, so the
+    // host work of batches in flight runs on as many threads.  phase (under
+    // the server lock): 0 free, 1 filled by the dispatcher, -1 exit.
+    std::thread worker;
+    std::condition_variable cv;
+    int32_t phase = 0;
 };
 
 }  // namespace
@@ -1905,6 +1911,19 @@ struct glass_hip_server {
     int64_t n_batches = 0, n_queries = 0, n_requests = 0;
 
     ~glass_hip_server() {
+        // workers still running (creation failed after they started): stop them
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            stop = true;
+            for (auto &sl : slots) {
+                sl->phase = -1;
+                sl->cv.notify_all();
+            }
+        }
+        cv_work.notify_all();
+        if (worker.joinable()) worker.join();
+        for (auto &sl : slots)
+            if (sl->worker.joinable()) sl->worker.join();
         for (auto &sl : slots) {
             if (sl->h_in) (void)hipHostFree(sl->h_in);
             if (sl->h_out) (void)hipHostFree(sl->h_out);
@@ -1916,18 +1935,28 @@ struct glass_hip_server {
 
 namespace {
 
+// A device error ends the server: the queued requests fail here, and each
+// slot's worker fails the requests of its own batch when it sees err_code.
 void server_fail_all(glass_hip_server &sv, int code, const std::string &msg) {
     std::lock_guard<std::mutex> lk(sv.mu);
-    sv.err_code = code;
-    sv.err_msg = msg;
+    if (!sv.err_code) {
+        sv.err_code = code;
+        sv.err_msg = msg;
+    }
     for (ServerReq *r : sv.pending) r->state = RQ_FAILED;
     sv.pending.clear();
     sv.pending_n.store(0);
-    // completed parts no longer name their request (server_collect clears
-    // them): its owner may already have released it
-    for (auto &sl : sv.slots)
-        for (auto &pt : sl->parts)
-            if (pt.r && pt.r->state != RQ_DONE) pt.r->state = RQ_FAILED;
+    sv.cv_done.notify_all();
+    sv.cv_work.notify_all();
+}
+
+// The requests of a slot's batch after a device error (under the server
+// lock).  Completed parts no longer name their request (server_collect
+// clears them): its owner may already have released it.
+void server_fail_slot_parts(glass_hip_server &sv, ServerSlot &sl) {
+    for (auto &pt : sl.parts)
+        if (pt.r && pt.r->state != RQ_DONE) pt.r->state = RQ_FAILED;
+    sl.parts.clear();
     sv.cv_done.notify_all();
 }
 
@@ -2102,82 +2131,92 @@ bool server_collect(glass_hip_server &sv, ServerSlot &sl, bool all) {
     return sl.pending == 0;
 }
 
+// One slot's worker: waits for the dispatcher to fill the slot, packs and
+// launches the batch, then completes its requests as their results arrive.
+void slot_loop(glass_hip_server *psv, ServerSlot *psl) {
+    glass_hip_server &sv = *psv;
+    ServerSlot &sl = *psl;
+    const bool dev_ok = hipSetDevice(sv.s->device) == hipSuccess;
+    for (;;) {
+        {
+            std::unique_lock<std::mutex> lk(sv.mu);
+            sl.cv.wait(lk, [&] { return sl.phase != 0; });
+            if (sl.phase < 0) return;
+        }
+        bool launched = false;
+        if (!dev_ok) {
+            server_fail_all(sv, GLASS_HIP_EDEVICE, "server thread: hipSetDevice failed");
+        } else if (int rc = server_launch(sv, sl)) {
+            server_fail_batch(sv, sl, rc, g_last_error);
+        } else {
+            launched = true;
+            for (;;) {
+                const hipError_t e = hipEventQuery(sl.done);
+                if (e != hipSuccess && e != hipErrorNotReady) {
+                    server_fail_all(sv, GLASS_HIP_EDEVICE, std::string("search batch failed: ") + hipGetErrorString(e));
+                    (void)hipStreamSynchronize(sl.stream);
+                    break;
+                }
+                const bool finished = e == hipSuccess;
+                const bool progressed = sl.pending && server_collect(sv, sl, finished);
+                if (finished) {
+                    if (sl.pending) server_collect(sv, sl, true);
+                    break;
+                }
+                if (!progressed) std::this_thread::yield();
+            }
+        }
+        std::lock_guard<std::mutex> lk(sv.mu);
+        if (sv.err_code) server_fail_slot_parts(sv, sl);
+        if (launched) sv.n_batches += 1;
+        sl.parts.clear();
+        if (sl.phase > 0) sl.phase = 0;
+        sv.cv_work.notify_all();
+    }
+}
+
+// The dispatcher: fills a free slot with whatever is queued (up to
+// max_batch queries; a larger request is split, its remaining queries
+// staying at the queue's head) and hands it to the slot's worker.
 void server_loop(glass_hip_server *psv) {
     glass_hip_server &sv = *psv;
-    if (hipSetDevice(sv.s->device) != hipSuccess) {
-        server_fail_all(sv, GLASS_HIP_EDEVICE, "server thread: hipSetDevice failed");
-        return;
-    }
-    std::vector<int> running;  // slot indices with a batch in flight
-    std::vector<int> free_slots;
-    for (int i = (int)sv.slots.size() - 1; i >= 0; --i) free_slots.push_back(i);
-    auto drain = [&]() {
-        for (auto &sl : sv.slots) (void)hipStreamSynchronize(sl->stream);
-    };
+    std::unique_lock<std::mutex> lk(sv.mu);
     for (;;) {
-        bool progressed = false;
-        for (size_t i = 0; i < running.size();) {
-            ServerSlot &sl = *sv.slots[running[i]];
-            const hipError_t e = hipEventQuery(sl.done);
-            if (e != hipSuccess && e != hipErrorNotReady) {
-                server_fail_all(sv, GLASS_HIP_EDEVICE, std::string("search batch failed: ") + hipGetErrorString(e));
-                drain();
-                return;
+        ServerSlot *free_sl = nullptr;
+        sv.cv_work.wait(lk, [&] {
+            if (sv.err_code || (sv.stop && sv.pending.empty())) return true;
+            if (sv.pending.empty()) return false;
+            for (auto &p : sv.slots)
+                if (p->phase == 0) return true;
+            return false;
+        });
+        if (sv.err_code || (sv.stop && sv.pending.empty())) break;
+        for (auto &p : sv.slots)
+            if (p->phase == 0) {
+                free_sl = p.get();
+                break;
             }
-            const bool finished = e == hipSuccess;
-            if (sl.pending && server_collect(sv, sl, finished)) progressed = true;
-            if (finished) {
-                if (sl.pending) server_collect(sv, sl, true);
-                {
-                    std::lock_guard<std::mutex> lk(sv.mu);
-                    sv.n_batches += 1;
-                }
-                sl.parts.clear();
-                free_slots.push_back(running[i]);
-                running.erase(running.begin() + (ptrdiff_t)i);
-                progressed = true;
-                continue;
-            }
-            ++i;
+        ServerSlot &sl = *free_sl;
+        int32_t total = 0;
+        while (!sv.pending.empty() && total < sv.max_batch) {
+            ServerReq *r = sv.pending.front();
+            const int32_t take = std::min(r->nq - r->dispatched, sv.max_batch - total);
+            sl.parts.push_back(ServerSlot::Part{r, r->dispatched, take, 0, 0, 0, false});
+            r->dispatched += take;
+            r->parts_open += 1;
+            r->state = RQ_RUNNING;
+            total += take;
+            if (r->dispatched == r->nq) sv.pending.pop_front();
         }
-        if (!free_slots.empty() && (running.empty() || sv.pending_n.load(std::memory_order_relaxed) > 0)) {
-            ServerSlot &sl = *sv.slots[free_slots.back()];
-            {
-                std::unique_lock<std::mutex> lk(sv.mu);
-                if (sv.pending.empty() && running.empty()) {
-                    if (sv.stop) return;
-                    sv.cv_work.wait_for(lk, std::chrono::milliseconds(2));
-                }
-                // fill the batch; a request larger than what is left of it
-                // is split, its remaining queries staying at the queue's head
-                int32_t total = 0;
-                while (!sv.pending.empty() && total < sv.max_batch) {
-                    ServerReq *r = sv.pending.front();
-                    const int32_t take = std::min(r->nq - r->dispatched, sv.max_batch - total);
-                    sl.parts.push_back(ServerSlot::Part{r, r->dispatched, take, 0, 0, 0, false});
-                    r->dispatched += take;
-                    r->parts_open += 1;
-                    r->state = RQ_RUNNING;
-                    total += take;
-                    if (r->dispatched == r->nq) sv.pending.pop_front();
-                }
-                sv.pending_n.store((int64_t)sv.pending.size());
-            }
-            if (!sl.parts.empty()) {
-                if (int rc = server_launch(sv, sl)) {
-                    server_fail_batch(sv, sl, rc, g_last_error);
-                } else {
-                    running.push_back(free_slots.back());
-                    free_slots.pop_back();
-                }
-                progressed = true;
-            }
-        }
-        if (!progressed && !running.empty()) std::this_thread::yield();
-        if (!progressed && running.empty()) {
-            std::lock_guard<std::mutex> lk(sv.mu);
-            if (sv.stop && sv.pending.empty()) return;
-        }
+        sv.pending_n.store((int64_t)sv.pending.size());
+        sl.phase = 1;
+        sl.cv.notify_one();
+    }
+    // every worker finishes its batch, then exits
+    for (auto &p : sv.slots) {
+        sv.cv_work.wait(lk, [&] { return p->phase <= 0; });
+        p->phase = -1;
+        p->cv.notify_one();
     }
 }
 
@@ -2263,6 +2302,7 @@ int glass_hip_server_create(glass_hip_searcher *s, int32_t max_batch, int32_t ma
             HIP_TRY(hipEventCreateWithFlags(&sl.done, hipEventDisableTiming));
         }
         glass_hip_server *p = sv.get();
+        for (auto &sl : sv->slots) sl->worker = std::thread(slot_loop, p, sl.get());
         sv->worker = std::thread(server_loop, p);
         *out = sv.release();
         return GLASS_HIP_OK;
@@ -2462,7 +2502,9 @@ void glass_hip_server_free(glass_hip_server *sv) {
         sv->cv_done.notify_all();
     }
     sv->cv_work.notify_all();
-    if (sv->worker.joinable()) sv->worker.join();
+    if (sv->worker.joinable()) sv->worker.join();  // it lets every slot's worker finish its batch and exit
+    for (auto &sl : sv->slots)
+        if (sl->worker.joinable()) sl->worker.join();
     for (auto &sl : sv->slots)
         if (sl->stream) (void)hipStreamSynchronize(sl->stream);
     delete sv;
