@@ -200,7 +200,7 @@ def gpu_sweep(s, q_dev, nq, k, gt, lanes, args, torch, factor, efs):
 
 
 def served_bench(s, Q, gt, k, ef, factor, n_sat=500_000, n_load=400_000, loads=(0.25, 0.5, 0.75, 0.9),
-                 max_batch=8192, max_inflight=3):
+                 max_batch=8192, max_inflight=6):
     """Served form: single-query requests through the continuous-batching
     server.  Saturation = completion rate with every request queued at t=0;
     then Poisson arrivals (seed ARRIVAL_SEED) at fractions of it, with each
@@ -1151,7 +1151,7 @@ def main():
     if args.served:
         parts = args.served.split(":")
         mb = int(parts[2]) if len(parts) > 2 else 8192
-        mi = int(parts[3]) if len(parts) > 3 else 3
+        mi = int(parts[3]) if len(parts) > 3 else 6
         print(json.dumps(served_bench(s, Q, gt, k, int(parts[1]), float(parts[0]), max_batch=mb, max_inflight=mi)))
         return
     if args.point:
