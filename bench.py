@@ -139,6 +139,14 @@ def load_or_build_graph(X, path, R=32, L=200):
     return m
 
 
+def per_step(alg, ms_per_step):
+    """The job's rate: a step's algorithmic bytes over the time per step
+    (with launches alternating over streams, a launch's own duration spans
+    more than one step)."""
+    gbs = alg / (ms_per_step / 1e3) / 1e9
+    return {"achieved": gbs, "frac": gbs / HBM_PEAK_GBS, "ms_per_step": ms_per_step}
+
+
 def make_lanes(torch, dev, n, nq, k):
     """`n` launch lanes: a stream each, with its own output buffers."""
     return [(torch.cuda.Stream(device=dev), torch.empty((nq, k), dtype=torch.int32, device=dev),
@@ -721,19 +729,19 @@ def point_or_sweep(s, w, X, Q, gt, k, info, args, torch, dev, stream, factors, e
     """Device-resident batched sweep (W warm-up + K timed launches per point)."""
     nq = Q.shape[0]
     q_dev = torch.from_numpy(Q).to(dev)
-    ids_dev = torch.empty((nq, k), dtype=torch.int32, device=dev)
-    d_dev = torch.empty((nq, k), dtype=torch.float32, device=dev)
-    cnt_dev = torch.empty((nq, 3), dtype=torch.int32, device=dev)
+    lanes = make_lanes(torch, dev, max(1, args.streams), nq, k)
     pts = []
     for f in factors:
         for ef in efs:
             if f:
                 s.set_refine_factor(f)
             s.set_ef(ef)
-            wall, kern = time_batched(s, q_dev, nq, k, [(stream, ids_dev, d_dev, cnt_dev)], args.steps, args.warmup,
-                                      torch)
-            ids = ids_dev.cpu().numpy()
-            cnt = cnt_dev.cpu().numpy()
+            wall, kern = time_batched(s, q_dev, nq, k, lanes, args.steps, args.warmup, torch)
+            ids = lanes[0][1].cpu().numpy()
+            cnt = lanes[0][3].cpu().numpy()
+            for ln in lanes[1:]:
+                if not (np.array_equal(ln[1].cpu().numpy(), ids) and np.array_equal(ln[3].cpu().numpy(), cnt)):
+                    raise RuntimeError("results differ between launch lanes")
             rq = int(k * (f or info["refine_factor"])) if w["refine"] else 0
             p = {"factor": f or info["refine_factor"], "ef": ef, "recall": recall_at(ids, gt, k) if gt is not None
                  else None, "ms_per_step": wall / args.steps * 1e3, "qps": nq / (wall / args.steps),
@@ -805,7 +813,8 @@ def c5_run(w, s, X, Q, gpath, gmeta, info, args, torch, dev, stream):
         "roofline": {"bound": "hbm", "achieved": op["achieved_gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": op["achieved_gbs"] / HBM_PEAK_GBS, "traffic": traffic, "traffic_detail": tdet,
                      "algorithmic_bytes_per_launch": op["algorithmic_bytes_per_launch"],
-                     "kernel_ms": op["kernel_ms"]},
+                     "kernel_ms": op["kernel_ms"], "streams": args.streams,
+                     "per_step": per_step(op["algorithmic_bytes_per_launch"], op["ms_per_step"])},
         "cpu_baseline": None if not cpu else {
             "value": cpu.get("4.0", {}).get("qps"), "unit": "queries/s", "cores": os.cpu_count(),
             "cpu": host_cpu(), "kind": "reference",
@@ -842,29 +851,36 @@ def c3_run(w, s, X, Q, gpath, gmeta, info, args, torch, dev, stream):
     ef_dev = torch.from_numpy(efv).to(dev)
     k_dev = torch.from_numpy(kv).to(dev)
     off_dev = torch.from_numpy(off[:-1].copy()).to(dev)
-    ids_dev = torch.empty(int(off[-1]), dtype=torch.int32, device=dev)
-    d_dev = torch.empty(int(off[-1]), dtype=torch.float32, device=dev)
-    cnt_dev = torch.empty((nq, 3), dtype=torch.int32, device=dev)
-    sp = stream.cuda_stream
+    # steps alternate over args.streams lanes (a stream and outputs each)
+    lanes = [(stream if i == 0 else torch.cuda.Stream(device=dev),
+              torch.empty(int(off[-1]), dtype=torch.int32, device=dev),
+              torch.empty(int(off[-1]), dtype=torch.float32, device=dev),
+              torch.empty((nq, 3), dtype=torch.int32, device=dev)) for i in range(max(1, args.streams))]
+    ids_dev, d_dev, cnt_dev = lanes[0][1], lanes[0][2], lanes[0][3]
 
-    def launch():
+    def launch(i):
+        st, ids_l, d_l, cnt_l = lanes[i % len(lanes)]
         s.search_batch_efk_dev(q_dev.data_ptr(), nq, ef_dev.data_ptr(), k_dev.data_ptr(), off_dev.data_ptr(),
-                               ids_dev.data_ptr(), int(efv.max()), int(kv.max()), d_dev.data_ptr(),
-                               cnt_dev.data_ptr(), stream=sp)
+                               ids_l.data_ptr(), int(efv.max()), int(kv.max()), d_l.data_ptr(),
+                               cnt_l.data_ptr(), stream=st.cuda_stream)
 
-    for _ in range(args.warmup):
-        launch()
+    for i in range(args.warmup):
+        launch(i)
     torch.cuda.synchronize()
     ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
     t0 = time.perf_counter()
     for i in range(args.steps):
-        ev[i][0].record(stream)
-        launch()
-        ev[i][1].record(stream)
+        st = lanes[i % len(lanes)][0]
+        ev[i][0].record(st)
+        launch(i)
+        ev[i][1].record(st)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     kern = float(np.mean([a.elapsed_time(b) for a, b in ev]))
     cnt = cnt_dev.cpu().numpy()
+    for ln in lanes[1:]:
+        if not (np.array_equal(ln[1].cpu().numpy(), ids_dev.cpu().numpy()) and np.array_equal(ln[3].cpu().numpy(), cnt)):
+            raise RuntimeError("results differ between launch lanes")
     ab = alg_bytes(cnt, info, X.shape[1], kv, 0)
     mix = {"ms_per_step": wall / args.steps * 1e3, "qps": nq / (wall / args.steps), "kernel_ms": kern,
            "dist_cmps": float(cnt[:, 0].mean()), "algorithmic_bytes_per_launch": ab,
@@ -910,7 +926,8 @@ def c3_run(w, s, X, Q, gpath, gmeta, info, args, torch, dev, stream):
                    "qps_at_recall10_0.95": v10, "sweep_k10": [strip(p) for p in pts], "graph": gmeta},
         "roofline": {"bound": "hbm", "achieved": mix["achieved_gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": mix["achieved_gbs"] / HBM_PEAK_GBS, "traffic": traffic, "traffic_detail": tdet,
-                     "algorithmic_bytes_per_launch": ab, "kernel_ms": kern},
+                     "algorithmic_bytes_per_launch": ab, "kernel_ms": kern, "streams": args.streams,
+                     "per_step": per_step(ab, mix["ms_per_step"])},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))
