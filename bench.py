@@ -221,7 +221,8 @@ def served_bench(s, Q, gt, k, ef, factor, n_sat=500_000, n_load=400_000, loads=(
     nq = len(Q)
     rows = np.arange(max(n_sat, n_load)) % nq
     Qt = Q[rows]
-    srv.replay(Qt[:20000], 1, ef, k, np.zeros(20000, np.int64))  # warm-up
+    nw = min(len(Qt), 2 * max_batch * max_inflight)  # warm-up: every slot runs twice
+    srv.replay(Qt[:nw], 1, ef, k, np.zeros(nw, np.int64))
     lat, ids, _ = srv.replay(Qt[:n_sat], 1, ef, k, np.zeros(n_sat, np.int64))
     sat = n_sat / (lat.max() / 1e9)
     rec = recall_at(ids.reshape(-1, k)[:nq], gt[: min(nq, n_sat)], k)

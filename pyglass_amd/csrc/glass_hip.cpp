@@ -1848,7 +1848,10 @@ struct ServerReq {
     std::vector<int32_t> own_ids;
     std::vector<float> own_dists;
     int64_t arrival_ns = 0, done_ns = 0;
-    int32_t state = RQ_QUEUED;
+    // Written under the server lock, except that a request lying whole in one
+    // batch is completed by that batch's worker without it (done_ns first,
+    // then a release store of RQ_DONE; the worker touches the request no more).
+    std::atomic<int32_t> state{RQ_QUEUED};
     int32_t err = GLASS_HIP_OK;      // RQ_FAILED by its own batch: the status and message
     std::string err_msg;
     int64_t *group_done = nullptr;   // replay(): completed-request count of its call (under the server lock)
@@ -1876,8 +1879,10 @@ struct ServerSlot {
         int64_t res;                 // first result of the part in h_out
         int64_t checked;             // leading results of the part seen complete
         bool done;
+        bool whole;                  // the part is its whole request
     };
     std::vector<Part> parts;
+    std::vector<std::pair<int64_t *, int64_t>> groups;  // server_collect: completions per replay() call
     size_t first_open = 0;           // parts before it are complete
     int32_t pending = 0;             // parts of this batch not yet complete
     int32_t nq = 0;
@@ -2113,17 +2118,45 @@ bool server_collect(glass_hip_server &sv, ServerSlot &sl, bool all) {
         if (first_done == sl.parts.size()) first_done = i;
     }
     if (first_done == sl.parts.size()) return sl.pending == 0;
+    // A part that is its whole request completes here, without the lock: its
+    // owner reads the request's state (an atomic) and may take or release it
+    // right after the store.  Parts of split requests, the counters and the
+    // wake-up of waiters go under one lock.
+    int64_t nq_done = 0, whole_done = 0;
+    bool split = false;
+    sl.groups.clear();
+    for (size_t i = first_done; i < scan_end; ++i) {
+        auto &pt = sl.parts[i];
+        if (pt.checked != -1) continue;
+        nq_done += pt.nq;
+        sl.pending -= 1;
+        if (!pt.whole) {
+            split = true;
+            continue;
+        }
+        ServerReq *r = pt.r;
+        pt.checked = 0;
+        pt.r = nullptr;
+        if (int64_t *g = r->group_done) {
+            if (sl.groups.empty() || sl.groups.back().first != g) sl.groups.emplace_back(g, 0);
+            sl.groups.back().second += 1;
+        }
+        r->done_ns = t;
+        r->state.store(RQ_DONE, std::memory_order_release);
+        whole_done += 1;
+    }
     {
         std::lock_guard<std::mutex> lk(sv.mu);
-        for (size_t i = first_done; i < scan_end; ++i) {
+        sv.n_queries += nq_done;
+        sv.n_requests += whole_done;
+        for (auto &g : sl.groups) *g.first += g.second;
+        for (size_t i = first_done; split && i < scan_end; ++i) {
             auto &pt = sl.parts[i];
             if (pt.checked != -1) continue;
             ServerReq *r = pt.r;
             pt.checked = 0;
             pt.r = nullptr;  // the request's owner may release it once it completes
-            sl.pending -= 1;
             r->parts_open -= 1;
-            sv.n_queries += pt.nq;
             if (r->parts_open == 0 && r->dispatched == r->nq && r->state == RQ_RUNNING) server_finish(sv, r, t);
         }
         sv.cv_done.notify_all();
@@ -2201,10 +2234,10 @@ void server_loop(glass_hip_server *psv) {
         while (!sv.pending.empty() && total < sv.max_batch) {
             ServerReq *r = sv.pending.front();
             const int32_t take = std::min(r->nq - r->dispatched, sv.max_batch - total);
-            sl.parts.push_back(ServerSlot::Part{r, r->dispatched, take, 0, 0, 0, false});
+            sl.parts.push_back(ServerSlot::Part{r, r->dispatched, take, 0, 0, 0, false, take == r->nq});
             r->dispatched += take;
             r->parts_open += 1;
-            r->state = RQ_RUNNING;
+            r->state.store(RQ_RUNNING, std::memory_order_relaxed);  // readers hold the lock
             total += take;
             if (r->dispatched == r->nq) sv.pending.pop_front();
         }
@@ -2429,23 +2462,29 @@ int glass_hip_server_replay(glass_hip_server *sv, const float *qs, int64_t nreq,
         }
         const int64_t t0 = now_ns();
         int64_t next = 0;
+        std::vector<ServerReq *> burst;
+        burst.reserve(1024);
         while (next < nreq) {
             const int64_t t = now_ns() - t0;
             if (arrival_ns[next] > t) {
                 if (arrival_ns[next] - t > 200000) std::this_thread::sleep_for(std::chrono::microseconds(50));
                 continue;
             }
+            // at most 1,024 at a time, stamped outside the lock and spliced
+            // into the queue in one insert: the lock is held about a
+            // microsecond per burst, so a backlog being pushed does not keep
+            // the dispatcher and the workers from it
+            burst.clear();
+            for (int c = 0; c < 1024 && next < nreq && arrival_ns[next] <= t; ++c) {
+                ServerReq &r = reqs[(size_t)next];
+                r.arrival_ns = t0 + arrival_ns[next];
+                burst.push_back(&r);
+                ++next;
+            }
             {
                 std::lock_guard<std::mutex> lk(sv->mu);
                 if (sv->err_code) break;
-                // at most 1,024 at a time, so that the dispatcher is not
-                // kept from the queue while a burst is pushed
-                for (int c = 0; c < 1024 && next < nreq && arrival_ns[next] <= t; ++c) {
-                    ServerReq &r = reqs[(size_t)next];
-                    r.arrival_ns = t0 + arrival_ns[next];
-                    sv->pending.push_back(&r);
-                    ++next;
-                }
+                sv->pending.insert(sv->pending.end(), burst.begin(), burst.end());
                 sv->pending_n.store((int64_t)sv->pending.size());
             }
             sv->cv_work.notify_one();

@@ -226,6 +226,42 @@ def test_device_batches_on_two_streams_back_to_back(c1):
         np.testing.assert_array_equal(out[1][1].cpu().numpy(), D2)
 
 
+def test_more_streams_than_launch_states(c1):
+    """Batches cycling over six streams with no host sync between them: the
+    searcher keeps four per-stream launch states, so the fifth and sixth
+    streams take over the least recently used states, each after waiting for
+    that state's last launch.  Every batch matches the host path."""
+    import torch
+
+    X, Q, path = c1
+    s = pg.Searcher(pg.Graph(path), X, "L2", "SQ4U", "FP32")
+    parts = [(0, 150, 32), (150, 350, 64), (350, 500, 16), (500, 700, 128), (700, 850, 48), (850, 1000, 96)]
+    want = []
+    for a, b, ef in parts:
+        s.set_ef(ef)
+        want.append(s.batch_search(Q[a:b], 10))
+    dev = torch.device("cuda:0")
+    q = torch.from_numpy(Q[:1000].copy()).to(dev)
+    out = [(torch.full((b - a, 10), -7, dtype=torch.int32, device=dev),
+            torch.zeros((b - a, 10), dtype=torch.float32, device=dev)) for a, b, _ in parts]
+    st = [torch.cuda.Stream(device=dev) for _ in parts]
+    torch.cuda.synchronize()
+    s.set_graph_capture(False)
+    for rep in range(3):
+        print(f"six streams, round {rep}", flush=True)
+        for j, (a, b, ef) in enumerate(parts):
+            out[j][0].fill_(-7)
+        torch.cuda.synchronize()
+        for j, (a, b, ef) in enumerate(parts):
+            s.set_ef(ef)
+            s.search_batch_dev(q[a:].data_ptr(), b - a, 10, out[j][0].data_ptr(), out[j][1].data_ptr(), 0,
+                               stream=st[(j + rep) % len(st)].cuda_stream)
+        torch.cuda.synchronize()
+        for j in range(len(parts)):
+            np.testing.assert_array_equal(out[j][0].cpu().numpy(), want[j][0])
+            np.testing.assert_array_equal(out[j][1].cpu().numpy(), want[j][1])
+
+
 @pytest.mark.parametrize("ef", [1500, 3500])
 def test_large_pools_fit_lds_by_fewer_waves(c1, ef):
     """Pools whose LDS at the tuned shape exceeds a CU run with fewer waves
