@@ -212,3 +212,91 @@ def test_request_larger_than_a_batch_is_split(setup):
     st = srv.stats()
     assert st["requests"] == 7 and st["queries"] == 1004 + 1604
     srv.close()
+
+
+def test_concurrent_clients_and_replay(setup):
+    """Six client threads submit, wait and poll at once while a seventh
+    replays: requests that lie whole in one batch (completed by that batch's
+    worker without the server lock) and split ones (settled under it) all
+    equal the batch path, and every request is counted once."""
+    import threading
+
+    X, Q, path, s = setup
+    srv = pg.Server(s, max_batch=256, max_k=16, max_inflight=4)
+    rng = np.random.default_rng(synth.EFK_SEED + 7)
+    plans = []
+    row = 0
+    for t in range(6):
+        reqs = []
+        for i in range(120):
+            nq = int(rng.choice([1, 1, 1, 2, 5, 300]))
+            ef = int(rng.choice([16, 64, 128]))
+            k = int(rng.choice([1, 10, 16]))
+            rows = np.arange(row, row + nq) % len(Q)
+            row += nq
+            reqs.append((rows, ef, k))
+        plans.append(reqs)
+    rep_nq = rng.integers(1, 9, 400).astype(np.int32)
+    rep_ef = rng.choice([32, 64], 400).astype(np.int32)
+    rep_rows = np.arange(int(rep_nq.sum())) % len(Q)
+    # expected results: the batch path with per-query ef and k
+    allreq = [r for p in plans for r in p]
+    Qa = np.concatenate([Q[rows] for rows, _, _ in allreq])
+    efa = np.concatenate([np.full(len(rows), ef, np.int32) for rows, ef, _ in allreq])
+    ka = np.concatenate([np.full(len(rows), k, np.int32) for rows, _, k in allreq])
+    bi, bd = s.batch_search_efk(Qa, efa, ka)
+    ri, rd = s.batch_search_efk(Q[rep_rows], np.repeat(rep_ef, rep_nq), 10)
+    got = [None] * 6
+    errors = []
+    print("server: 6 client threads + replay", flush=True)
+
+    def client(t):
+        try:
+            tickets = [srv.submit(Q[rows], k=k, ef=ef) for rows, ef, k in plans[t]]
+            out = [None] * len(tickets)
+            for j in range(len(tickets) - 1, -1, -2):
+                out[j] = srv.wait(tickets[j])
+            todo = [j for j in range(len(tickets)) if out[j] is None]
+            while todo:
+                left = []
+                for j in todo:
+                    r = srv.poll(tickets[j])
+                    if r is None:
+                        left.append(j)
+                    else:
+                        out[j] = r
+                todo = left
+            got[t] = out
+        except Exception as e:  # re-raised below
+            errors.append(e)
+
+    rep = {}
+
+    def replayer():
+        try:
+            rep["r"] = srv.replay(Q[rep_rows], rep_nq, rep_ef, 10, np.zeros(len(rep_nq), np.int64))
+        except Exception as e:
+            errors.append(e)
+
+    th = [threading.Thread(target=client, args=(t,)) for t in range(6)] + [threading.Thread(target=replayer)]
+    for x in th:
+        x.start()
+    for x in th:
+        x.join(timeout=120)
+    assert not any(x.is_alive() for x in th), "a client thread did not finish"
+    if errors:
+        raise errors[0]
+    q0 = 0
+    for t in range(6):
+        for (rows, ef, k), (ids, dis) in zip(plans[t], got[t]):
+            n = len(rows)
+            np.testing.assert_array_equal(ids, np.stack(bi[q0:q0 + n]))
+            np.testing.assert_array_equal(dis, np.stack(bd[q0:q0 + n]))
+            q0 += n
+    _, rids, rdis = rep["r"]
+    np.testing.assert_array_equal(rids.reshape(-1, 10), np.stack(ri))
+    np.testing.assert_array_equal(rdis.reshape(-1, 10), np.stack(rd))
+    st = srv.stats()
+    assert st["requests"] == len(allreq) + len(rep_nq)
+    assert st["queries"] == len(Qa) + len(rep_rows)
+    srv.close()
