@@ -1236,6 +1236,10 @@ def main():
     plain_recall = recall_at(plain_ids, gt[:npl], k)
     log(f"plain: {npl} queries, factor={best_f} ef={op['ef']}: {plain_qps:.0f} QPS, recall {plain_recall:.4f}")
     served = None if args.no_served else served_bench(s, Q, gt, k, op["ef"], best_f)
+    if served:
+        # the batched rate at the same ef and factor (the served runs use the operating point)
+        served["batched_qps_same_ef"] = op["qps"]
+        served["served_over_batched"] = served["saturation_qps"] / op["qps"]
     traffic, traffic_detail, occ = None, None, None
     if not args.no_pmc:
         s.set_ef(op["ef"])
