@@ -507,6 +507,9 @@ def quant_mode(spec, gpath, X, Q, gt, k, args, pg, torch, dev):
         "sweep": [{kk: v for kk, v in p.items() if kk not in ("ids", "cnt")} for p in pts],
         "cpu_baseline": cpu,
     }
+    if args.calib_copy:
+        # a plain device copy of known size, to scale the HBM byte counters of a --pmc run
+        out["calib_copy_bytes"] = calib_copy(torch, dev)
     print(json.dumps(out))
 
 
@@ -1081,7 +1084,8 @@ def main():
                     help="FACTOR:EF[:MAX_BATCH[:MAX_INFLIGHT]] - run only the served-form measurement")
     ap.add_argument("--small", action="store_true", help="100k-row version for quick checks")
     ap.add_argument("--plain", type=int, default=200, help="queries timed in the plain one-query form")
-    ap.add_argument("--calib-copy", action="store_true", help="with --point: also run a 512 MiB device copy")
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="with --point or --quant: also run a 512 MiB device copy (scales the byte counters)")
     ap.add_argument("--launch", type=str, default=None,
                     help="TEAM:WAVES_PER_BLOCK:VISITED_ENTRIES - pin the launch shape after optimize() "
                          "(profiler runs perturb the autotuner's timings)")
