@@ -752,9 +752,15 @@ static float reduce16_helpa(const float a[16]) {
 
 /* ctx selects the FMA contraction GCC -Ofast chose where the reference's
  * SQ8 L2 kernel is inlined: ORQ_CTX_COMPUTER (a bare computer call) and
- * ORQ_CTX_SEARCH (Search/SearchImpl1) fuse as fma(-c, dif, 256x - 256min);
- * ORQ_CTX_BATCH (SearchBatch/SearchImpl2) fuses yy = fma(c, dif, 256min),
- * t = 256x - yy.  Every other kernel compiles the same in all contexts. */
+ * ORQ_CTX_SEARCH (SearchImpl1's expansions) fuse as fma(-c, dif, 256x -
+ * 256min); ORQ_CTX_BATCH (SearchBatch/SearchImpl2) and ORQ_CTX_INIT
+ * (initialize_search, graph.hpp:95-104, in Search as in SearchBatch) fuse
+ * yy = fma(c, dif, 256min), t = 256x - yy.  Pinned by the golden file's
+ * single-query distances: with the initializer's evaluations in the second
+ * form, all of them are bit-equal; in the first form, 1 in 270 HNSW and NSG
+ * results (the entry points and the upper-layer descent's last node) is one
+ * ulp off.  SQ6 below splits the same way; every other kernel compiles the
+ * same in all contexts. */
 double orq_distance_ctx(const orq_quant *q, const uint8_t *qbuf, const uint8_t *code, int32_t ctx) {
     int32_t da = q->dim_align;
     switch (q->kind) {
@@ -817,7 +823,7 @@ double orq_distance_ctx(const orq_quant *q, const uint8_t *qbuf, const uint8_t *
                     float c = (float)code[i + l] + 0.5f;
                     if (q->metric == ORQ_L2) {
                         float t;
-                        if ((ctx & 3) == ORQ_CTX_BATCH) {
+                        if ((ctx & 3) == ORQ_CTX_BATCH || (ctx & ORQ_CTX_INIT)) {
                             t = x[i + l] * 256.0f - fmaf(c, q->difs[i + l], q->mins[i + l] * 256.0f);
                         } else {
                             t = fmaf(-c, q->difs[i + l], x[i + l] * 256.0f - q->mins[i + l] * 256.0f);

@@ -503,11 +503,12 @@ __device__ __forceinline__ float float_row4(const DeviceIndex &ix, const WaveLds
                         const float c = (float)((cw[b] >> (8 * r)) & 0xffu) + 0.5f;
                         if (V == V_SQ8_L2) {
                             float d;
-                            if ((mode & 3) == MODE_BATCH) {
-                                // SearchImpl2 context: yy = fma(c, dif, 256min), d = 256x - yy
+                            if ((mode & 3) == MODE_BATCH || (mode & kInitCtx)) {
+                                // SearchImpl2 and initialize_search contexts (in Search too):
+                                // yy = fma(c, dif, 256min), d = 256x - yy
                                 d = w.qf[j + r] * 256.0f - fmaf(c, dif[j + r], m256[j + r]);
                             } else {
-                                // Search / bare computer context: d = fma(-c, dif, 256x - 256min)
+                                // Search's expansions / bare computer context: d = fma(-c, dif, 256x - 256min)
                                 d = fmaf(-c, dif[j + r], w.qm[j + r]);
                             }
                             acc.a[r] = fmaf(d, d, acc.a[r]);

@@ -106,6 +106,32 @@ def test_golden_reference_ids_directly(dsets, golden_dir):
                 np.testing.assert_array_equal(D, z[f"{key}/batch_dists"], err_msg=key)
 
 
+@pytest.mark.parametrize("gname", ["c1_hnsw_l2", "nsg_l2", "ip_hnsw"])
+def test_sq8_single_and_batch_distances_equal_reference(dsets, golden_dir, gname):
+    """Per-dim SQ8 distances are rounded as the reference's compiled code
+    rounds them in each context: Search's expansions, SearchBatch's, and
+    initialize_search's (HNSW descent or entry points) in both.  HIP single
+    and batch ids and distances equal the stored reference values bit for
+    bit, with and without the FP32 re-rank."""
+    z = np.load(os.path.join(golden_dir, "search.npz"))
+    XX, QQ, metric = dsets[gname]
+    for refine in ["none", "FP32"]:
+        s = pg.Searcher(pg.Graph(os.path.join(golden_dir, f"{gname}.glass")), XX, metric, "SQ8",
+                        "" if refine == "none" else refine)
+        for ef in [16, 40, 96]:
+            for k in [1, 10, 50]:
+                key = f"{gname}/SQ8/{refine}/{ef}/{k}"
+                print(key, flush=True)
+                s.set_ef(ef)
+                I, D = s.batch_search(QQ[:64], k)
+                np.testing.assert_array_equal(I, z[f"{key}/batch_ids"], err_msg=key)
+                np.testing.assert_array_equal(D, z[f"{key}/batch_dists"], err_msg=key)
+                for i in range(64):
+                    si, sd = s.search(QQ[i], k)
+                    np.testing.assert_array_equal(si, z[f"{key}/single_ids"][i], err_msg=f"{key} single {i}")
+                    np.testing.assert_array_equal(sd, z[f"{key}/single_dists"][i], err_msg=f"{key} single {i}")
+
+
 @pytest.mark.parametrize("quant,factor", [("SQ4U", 4.0), ("SQ2U", 4.0), ("SQ8U", 8.0)])
 def test_refine_factor_override_matches_oracle(dsets, golden_dir, quant, factor):
     """Re-rank depth override (Refiner's constructor argument): HIP == oracle."""

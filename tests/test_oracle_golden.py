@@ -99,13 +99,7 @@ def test_search_matches_reference(sz, dsets, golden_dir, gname, quant, refine):
             si = np.stack([s.search(QQ[i], k)[0] for i in range(nq)])
             sd = np.stack([s.search(QQ[i], k)[1] for i in range(nq)])
             np.testing.assert_array_equal(si, sz[f"{key}/single_ids"], err_msg=key)
-            if quant == "SQ8" and metric == "L2" and refine == "none" or (quant == "SQ8" and k == 1):
-                # The reference's single-query SQ8 L2 kernel is inlined with a
-                # reduction association the oracle does not reproduce exactly;
-                # it agrees to one ulp.
-                np.testing.assert_allclose(sd, sz[f"{key}/single_dists"], rtol=2.5e-7, err_msg=key)
-            else:
-                np.testing.assert_array_equal(sd, sz[f"{key}/single_dists"], err_msg=key)
+            np.testing.assert_array_equal(sd, sz[f"{key}/single_dists"], err_msg=key)
 
 
 # FP16 / BF16 (tests/golden/make_golden_f16.py)
