@@ -171,8 +171,10 @@ int glass_hip_searcher_get_query_waves(const glass_hip_searcher *s, int32_t *las
  * tuned / sized from ef).  A query that visits more than half as many nodes
  * continues on the global bitset of its slot. */
 int glass_hip_searcher_set_visited_capacity(glass_hip_searcher *s, int32_t entries);
-/* Shape used at the current ef: team, waves per workgroup, blocks of the
- * last launch, visited-table entries. */
+/* Shape of the last launch: team, waves per workgroup, workgroups,
+ * visited-table entries (a batch with per-query ef runs with the tuning of
+ * its largest ef).  Before the first launch: the shape a batch at the current
+ * ef would use, with 0 workgroups. */
 int glass_hip_searcher_get_launch(const glass_hip_searcher *s, int32_t *team, int32_t *waves_per_block,
                                   int32_t *blocks, int32_t *vis_entries);
 /* Last launch: kernel VGPRs, LDS bytes per workgroup, resident workgroups

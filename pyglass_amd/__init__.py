@@ -313,6 +313,8 @@ class Searcher:
         return {"vgprs": v.value, "lds_per_block": l.value, "blocks_per_cu": b.value}
 
     def get_launch(self):
+        """Shape of the last launch (team, waves per workgroup, workgroups,
+        visited entries); before the first one, the shape at the current ef."""
         t, w, b, v = C.c_int32(), C.c_int32(), C.c_int32(), C.c_int32()
         _lib.check(self._L.glass_hip_searcher_get_launch(self._h, C.byref(t), C.byref(w), C.byref(b), C.byref(v)))
         return {"team": t.value, "waves_per_block": w.value, "blocks": b.value, "vis_entries": v.value}

@@ -1657,11 +1657,21 @@ int glass_hip_searcher_get_launch(const glass_hip_searcher *s, int32_t *team, in
                                   int32_t *blocks, int32_t *slots) {
     try {
         if (!s) return fail(GLASS_HIP_EINVAL, "null searcher");
+        const LaunchConfig &ll = s->last_launch;
+        if (ll.team) {
+            // the shape the last launch ran with: a batch with per-query ef
+            // takes the tuning of its largest ef, not of the current one
+            if (team) *team = ll.team;
+            if (waves_per_block) *waves_per_block = ll.waves_per_block;
+            if (blocks) *blocks = ll.blocks;
+            if (slots) *slots = ll.vis_hash;
+            return GLASS_HIP_OK;
+        }
+        // no launch yet: the shape a batch at the current ef would use (as run_batch picks it)
         const glass_hip_searcher::Tune &tn = s->tune[glass_hip_searcher::bucket(s->ef)];
         if (team) *team = s->force_team ? s->force_team : tn.team;
         if (waves_per_block) *waves_per_block = s->force_wpb ? s->force_wpb : tn.waves_per_block;
-        if (blocks) *blocks = s->last_launch.blocks;
-        // the table a batch at the current ef would use (as run_batch picks it)
+        if (blocks) *blocks = 0;
         if (slots) *slots = s->vis_override ? s->vis_override : (tn.vis_hash ? tn.vis_hash : glass_hip_searcher::auto_vis(s->ef));
         return GLASS_HIP_OK;
     } catch (...) {

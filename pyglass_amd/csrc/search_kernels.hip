@@ -181,7 +181,9 @@ struct WaveLds {
     uint32_t *ci;   // candidate ids [64]
     float *qf;      // query, zero padded [qf_len]
     float *qm;      // SQ8 helpers: 256x-256min, 256min, dif; SQ6: x-min, min, dif; SQ4: 16(x-min), min, dif [3 x qf_len]
-                    // (an SQ8 re-rank rewrites them with its own calibration after the search)
+                    // (an SQ8 re-rank rewrites them with its own calibration after the search).  Only the L2
+                    // kernels read the first, query-dependent row: for IP the area holds the last two rows and
+                    // qm is based one row before it (qm[0, qf_len) is neither read nor written then)
     uint8_t *qcode; // encoded query [qcode_bytes]
 };
 
@@ -198,6 +200,9 @@ __host__ __device__ inline bool needs_qm(const DeviceIndex &ix) {
     return ix.quant == QK_SQ8 || ix.quant == QK_SQ6 || ix.quant == QK_SQ4 || (ix.rerank && ix.rerank_kind == QK_SQ8);
 }
 
+// Helper rows the kernels read: the x - min row only for L2.
+__host__ __device__ inline int32_t qm_rows(const DeviceIndex &ix) { return ix.metric == MK_L2 ? 3 : 2; }
+
 // Encoded query bytes: the search codes', or an SQ8U re-rank's.
 __host__ __device__ inline int32_t qcode_bytes(const DeviceIndex &ix) {
     const int32_t r = ix.rerank && ix.rerank_kind == QK_SQ8U ? ix.r_code_size : 0;
@@ -209,7 +214,7 @@ __host__ __device__ inline size_t wave_lds_bytes(const DeviceIndex &ix, int32_t 
     b += (size_t)round16((int64_t)cap_max * 4) * 2;
     b += 64 * 4 * 2;
     b += (size_t)qf_len(ix) * 4;
-    if (needs_qm(ix)) b += (size_t)qf_len(ix) * 4 * 3;
+    if (needs_qm(ix)) b += (size_t)qf_len(ix) * 4 * qm_rows(ix);
     b += (size_t)qcode_bytes(ix);
     return b;
 }
@@ -230,8 +235,8 @@ __device__ inline WaveLds carve(uint8_t *base, const DeviceIndex &ix, int32_t ca
     int32_t ql = qf_len(ix);
     w.qf = (float *)(base + off);
     off += (size_t)ql * 4;
-    w.qm = (float *)(base + off);
-    if (needs_qm(ix)) off += (size_t)ql * 4 * 3;
+    w.qm = (float *)(base + off) - (needs_qm(ix) && qm_rows(ix) == 2 ? ql : 0);
+    if (needs_qm(ix)) off += (size_t)ql * 4 * qm_rows(ix);
     w.qcode = base + off;
     return w;
 }
@@ -808,7 +813,7 @@ __device__ __forceinline__ void rerank_sq8(const DeviceIndex &ix, const WaveLds 
     for (int j = l; j < ql; j += 64) {
         const float mn = j < ix.r_dim_align ? ix.r_mins[j] : 0.0f;
         const float df = j < ix.r_dim_align ? ix.r_difs[j] : 0.0f;
-        w.qm[j] = w.qf[j] * 256.0f - mn * 256.0f;
+        if (VS == V_SQ8_L2) w.qm[j] = w.qf[j] * 256.0f - mn * 256.0f;
         w.qm[ql + j] = mn * 256.0f;
         w.qm[2 * ql + j] = df;
     }
@@ -1455,7 +1460,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kWavesPerEu
             for (int j = l; j < ql; j += 64) {
                 float mn = j < ix.dim_align ? ix.sq8_mins[j] : 0.0f;
                 float df = j < ix.dim_align ? ix.sq8_difs[j] : 0.0f;
-                w.qm[j] = w.qf[j] * 256.0f - mn * 256.0f;
+                if (V == V_SQ8_L2) w.qm[j] = w.qf[j] * 256.0f - mn * 256.0f;
                 w.qm[ql + j] = mn * 256.0f;
                 w.qm[2 * ql + j] = df;
             }
@@ -1464,7 +1469,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kWavesPerEu
             for (int j = l; j < ql; j += 64) {
                 float mn = j < ix.dim_align ? ix.sq8_mins[j] : 0.0f;
                 float df = j < ix.dim_align ? ix.sq8_difs[j] : 0.0f;
-                w.qm[j] = (w.qf[j] - mn) * scale;
+                if (V == V_SQ6_L2 || V == V_SQ4_L2) w.qm[j] = (w.qf[j] - mn) * scale;
                 w.qm[ql + j] = mn;
                 w.qm[2 * ql + j] = df;
             }
