@@ -203,10 +203,13 @@ __host__ __device__ inline bool needs_qm(const DeviceIndex &ix) {
 // Helper rows the kernels read: the x - min row only for L2.
 __host__ __device__ inline int32_t qm_rows(const DeviceIndex &ix) { return ix.metric == MK_L2 ? 3 : 2; }
 
-// Encoded query bytes: the search codes', or an SQ8U re-rank's.
+// Encoded query bytes: the integer search codes' (SQ8U, SQ4U, SQ2U), or an
+// SQ8U re-rank's; the float kernels read the query from qf and need none.
 __host__ __device__ inline int32_t qcode_bytes(const DeviceIndex &ix) {
+    const bool int_codes = ix.quant == QK_SQ8U || ix.quant == QK_SQ4U || ix.quant == QK_SQ2U;
+    const int32_t c = int_codes ? ix.code_size : 0;
     const int32_t r = ix.rerank && ix.rerank_kind == QK_SQ8U ? ix.r_code_size : 0;
-    return round16(ix.code_size > r ? ix.code_size : r);
+    return round16(c > r ? c : r);
 }
 
 __host__ __device__ inline size_t wave_lds_bytes(const DeviceIndex &ix, int32_t cap_max, int32_t vis_hash) {
