@@ -223,6 +223,7 @@ def served_bench(s, Q, gt, k, ef, factor, n_sat=500_000, n_load=400_000, loads=(
     Qt = Q[rows]
     nw = min(len(Qt), 2 * max_batch * max_inflight)  # warm-up: every slot runs twice
     srv.replay(Qt[:nw], 1, ef, k, np.zeros(nw, np.int64))
+    time.sleep(0.2)  # a pause that sets the saturation run apart in a kernel trace
     lat, ids, _ = srv.replay(Qt[:n_sat], 1, ef, k, np.zeros(n_sat, np.int64))
     sat = n_sat / (lat.max() / 1e9)
     rec = recall_at(ids.reshape(-1, k)[:nq], gt[: min(nq, n_sat)], k)
@@ -245,9 +246,10 @@ def served_bench(s, Q, gt, k, ef, factor, n_sat=500_000, n_load=400_000, loads=(
         log(f"served: offered {rate:.0f} QPS -> p50 {row['p50_ms']:.3f} ms, p99 {row['p99_ms']:.3f} ms, "
             f"mean batch {row['mean_batch']:.0f}")
     srv.close()
-    half = next(r for r in out["loads"] if r["offered_fraction"] == 0.5)
-    out["p50_ms_at_half_saturation"] = half["p50_ms"]
-    out["p99_ms_at_half_saturation"] = half["p99_ms"]
+    half = next((r for r in out["loads"] if r["offered_fraction"] == 0.5), None)
+    if half:
+        out["p50_ms_at_half_saturation"] = half["p50_ms"]
+        out["p99_ms_at_half_saturation"] = half["p99_ms"]
     return out
 
 
@@ -330,6 +332,10 @@ def rocpd_summary(paths, last=3):
             if "kernels" in views:
                 rows = list(con.execute("select name, duration, grid_x, workgroup_x, lds_size, vgpr_count, "
                                         "scratch_size from kernels order by start"))
+                busy = search_busy([(s_, e_) for n_, s_, e_ in con.execute("select name, start, end from kernels "
+                                                                           "order by start") if "search_kernel" in n_])
+                if busy:
+                    print(busy)
                 per = defaultdict(list)
                 meta = {}
                 for name, dur, gx, wx, lds, vg, scr in rows:
@@ -349,6 +355,32 @@ def rocpd_summary(paths, last=3):
                     print(f"{ctr} = {val:.6g}")
         finally:
             con.close()
+
+
+def search_busy(spans, pause_ns=100e6):
+    """For the search_kernel dispatches after the trace's last pause of more
+    than 100 ms (a served run's saturation phase, a bench's timed steps): how
+    long the GPU runs at least one of them, and how many run at once."""
+    if not spans:
+        return None
+    st = np.array([a for a, _ in spans], np.float64)
+    en = np.array([b for _, b in spans], np.float64)
+    gaps = st[1:] - np.maximum.accumulate(en)[:-1]
+    cut = int(np.nonzero(gaps > pause_ns)[0][-1]) + 1 if np.any(gaps > pause_ns) else 0
+    st, en = st[cut:], en[cut:]
+    busy, cs, ce = 0.0, st[0], en[0]
+    for a, b in zip(st[1:], en[1:]):
+        if a > ce:
+            busy += ce - cs
+            cs, ce = a, b
+        else:
+            ce = max(ce, b)
+    busy += ce - cs
+    span = en.max() - st.min()
+    d = (en - st) / 1e3
+    return (f"last phase: {len(st)} search_kernel dispatches, mean {d.mean():.1f} us; first start to last end "
+            f"{span / 1e6:.3f} ms, a search kernel running {100 * busy / span:.1f}% of it, "
+            f"{d.sum() * 1e3 / span:.2f} in flight on average")
 
 
 def info_launch_waves(s):
@@ -1081,7 +1113,8 @@ def main():
     ap.add_argument("--no-served", action="store_true", help="skip the served-form (continuous batching) run")
     ap.add_argument("--no-pmc", action="store_true", help="skip the two rocprofv3 --pmc traffic passes")
     ap.add_argument("--served", type=str, default=None,
-                    help="FACTOR:EF[:MAX_BATCH[:MAX_INFLIGHT]] - run only the served-form measurement")
+                    help="FACTOR:EF[:MAX_BATCH[:MAX_INFLIGHT[:sat]]] - run only the served-form measurement "
+                         "(sat: saturation only, no offered-load runs)")
     ap.add_argument("--small", action="store_true", help="100k-row version for quick checks")
     ap.add_argument("--plain", type=int, default=200, help="queries timed in the plain one-query form")
     ap.add_argument("--calib-copy", action="store_true",
@@ -1174,7 +1207,9 @@ def main():
         parts = args.served.split(":")
         mb = int(parts[2]) if len(parts) > 2 else 8192
         mi = int(parts[3]) if len(parts) > 3 else 6
-        print(json.dumps(served_bench(s, Q, gt, k, int(parts[1]), float(parts[0]), max_batch=mb, max_inflight=mi)))
+        loads = (0.25, 0.5, 0.75, 0.9) if len(parts) < 5 or parts[4] != "sat" else ()
+        print(json.dumps(served_bench(s, Q, gt, k, int(parts[1]), float(parts[0]), max_batch=mb, max_inflight=mi,
+                                      loads=loads)))
         return
     if args.point:
         pf, pe = args.point.split(":")
