@@ -1929,7 +1929,11 @@ struct glass_hip_server {
     bool stop = false;
     int err_code = GLASS_HIP_OK;
     std::string err_msg;
-    int64_t n_batches = 0, n_queries = 0, n_requests = 0;
+    int64_t n_batches = 0;                 // under the lock
+    // Completed queries and requests: added to before any of them is
+    // published, so a caller that has seen its request complete also sees it
+    // counted by glass_hip_server_stats.
+    std::atomic<int64_t> n_queries{0}, n_requests{0};
 
     ~glass_hip_server() {
         // workers still running (creation failed after they started): stop them
@@ -1985,10 +1989,10 @@ void server_fail_slot_parts(glass_hip_server &sv, ServerSlot &sl) {
 // part has completed or failed (under the server lock): from here on its
 // owner may take or release it.
 void server_finish(glass_hip_server &sv, ServerReq *r, int64_t t) {
+    if (!r->err) sv.n_requests.fetch_add(1, std::memory_order_relaxed);
+    if (r->group_done) *r->group_done += 1;
     r->done_ns = t;
     r->state = r->err ? RQ_FAILED : RQ_DONE;
-    if (r->group_done) *r->group_done += 1;
-    if (!r->err) sv.n_requests += 1;
 }
 
 // A batch that could not be launched fails its own requests only; the
@@ -2140,16 +2144,25 @@ bool server_collect(glass_hip_server &sv, ServerSlot &sl, bool all) {
     // wake-up of waiters go under one lock.
     int64_t nq_done = 0, whole_done = 0;
     bool split = false;
+    for (size_t i = first_done; i < scan_end; ++i) {
+        const auto &pt = sl.parts[i];
+        if (pt.checked != -1) continue;
+        nq_done += pt.nq;
+        if (pt.whole)
+            whole_done += 1;
+        else
+            split = true;
+    }
+    // counted before any of them is published (the release stores below
+    // order these adds before a caller's view of its completed request)
+    sv.n_queries.fetch_add(nq_done, std::memory_order_relaxed);
+    sv.n_requests.fetch_add(whole_done, std::memory_order_relaxed);
     sl.groups.clear();
     for (size_t i = first_done; i < scan_end; ++i) {
         auto &pt = sl.parts[i];
         if (pt.checked != -1) continue;
-        nq_done += pt.nq;
         sl.pending -= 1;
-        if (!pt.whole) {
-            split = true;
-            continue;
-        }
+        if (!pt.whole) continue;
         ServerReq *r = pt.r;
         pt.checked = 0;
         pt.r = nullptr;
@@ -2159,12 +2172,9 @@ bool server_collect(glass_hip_server &sv, ServerSlot &sl, bool all) {
         }
         r->done_ns = t;
         r->state.store(RQ_DONE, std::memory_order_release);
-        whole_done += 1;
     }
     {
         std::lock_guard<std::mutex> lk(sv.mu);
-        sv.n_queries += nq_done;
-        sv.n_requests += whole_done;
         for (auto &g : sl.groups) *g.first += g.second;
         for (size_t i = first_done; split && i < scan_end; ++i) {
             auto &pt = sl.parts[i];
@@ -2538,8 +2548,8 @@ int glass_hip_server_stats(glass_hip_server *sv, int64_t *batches, int64_t *quer
         if (!sv) return fail(GLASS_HIP_EINVAL, "null argument");
         std::lock_guard<std::mutex> lk(sv->mu);
         if (batches) *batches = sv->n_batches;
-        if (queries) *queries = sv->n_queries;
-        if (requests) *requests = sv->n_requests;
+        if (queries) *queries = sv->n_queries.load();
+        if (requests) *requests = sv->n_requests.load();
         return GLASS_HIP_OK;
     } catch (...) {
         return exception_status();
