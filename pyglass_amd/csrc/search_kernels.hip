@@ -506,23 +506,25 @@ __device__ __forceinline__ float float_row4(const DeviceIndex &ix, const WaveLds
                         }
                     }
                 } else {
+                    // SearchImpl2 and initialize_search contexts (in Search too):
+                    // yy = fma(c, dif, 256min), d = 256x - yy.  Search's expansions and a
+                    // bare computer: d = fma(-c, dif, 256x - 256min).  One 16-byte LDS read
+                    // per row and 4 dimensions (every LDS row starts on 16 bytes, j % 4 == 0).
+                    const bool bform = (mode & 3) == MODE_BATCH || (mode & kInitCtx);
+                    const float4 dv = *(const float4 *)(dif + j), mv = *(const float4 *)(m256 + j);
+                    const float4 xv = *(const float4 *)((V == V_SQ8_L2 && !bform ? w.qm : w.qf) + j);
+                    const float dif4[4] = {dv.x, dv.y, dv.z, dv.w}, m4[4] = {mv.x, mv.y, mv.z, mv.w};
+                    const float x4[4] = {xv.x, xv.y, xv.z, xv.w};
 #pragma unroll
                     for (int r = 0; r < 4; ++r) {
                         const float c = (float)((cw[b] >> (8 * r)) & 0xffu) + 0.5f;
                         if (V == V_SQ8_L2) {
-                            float d;
-                            if ((mode & 3) == MODE_BATCH || (mode & kInitCtx)) {
-                                // SearchImpl2 and initialize_search contexts (in Search too):
-                                // yy = fma(c, dif, 256min), d = 256x - yy
-                                d = w.qf[j + r] * 256.0f - fmaf(c, dif[j + r], m256[j + r]);
-                            } else {
-                                // Search's expansions / bare computer context: d = fma(-c, dif, 256x - 256min)
-                                d = fmaf(-c, dif[j + r], w.qm[j + r]);
-                            }
+                            const float d =
+                                bform ? x4[r] * 256.0f - fmaf(c, dif4[r], m4[r]) : fmaf(-c, dif4[r], x4[r]);
                             acc.a[r] = fmaf(d, d, acc.a[r]);
                         } else {
-                            const float yy = fmaf(c, dif[j + r], m256[j + r]);
-                            acc.a[r] = fmaf(w.qf[j + r], yy, acc.a[r]);
+                            const float yy = fmaf(c, dif4[r], m4[r]);
+                            acc.a[r] = fmaf(x4[r], yy, acc.a[r]);
                         }
                     }
                 }
